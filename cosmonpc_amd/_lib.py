@@ -1,0 +1,90 @@
+"""ctypes binding of libcnpc_hip.so (the C ABI declared in include/cnpc.h).
+
+The library is required: importing the GPU path without it raises, there is
+no CPU fallback.  Device buffers are torch tensors on the current HIP device;
+only their data pointers cross the boundary.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcnpc_hip.so")
+
+
+class cnpc_poly(ctypes.Structure):
+    """Mirror of ``cnpc_poly`` in include/cnpc.h."""
+
+    _fields_ = [
+        ("m", ctypes.c_int32),
+        ("deg", ctypes.c_int32),
+        ("part", ctypes.c_int32),
+        ("pad", ctypes.c_int32),
+        ("zc", ctypes.c_double * 9),
+        ("pre_re", ctypes.c_double),
+        ("pre_im", ctypes.c_double),
+        ("k0_re", ctypes.c_double),
+        ("k0_im", ctypes.c_double),
+        ("rx", ctypes.c_double),
+        ("ry", ctypes.c_double),
+        ("rz", ctypes.c_double),
+    ]
+
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_L = ctypes.c_int64
+_D = ctypes.c_double
+_F = ctypes.c_float
+
+# name -> argtypes (all return int status)
+SIGNATURES = {
+    "cnpc_version": [],
+    "cnpc_ctx_create": [ctypes.POINTER(_P), _I],
+    "cnpc_ctx_destroy": [_P],
+    "cnpc_ctx_set_stream": [_P, _P],
+    "cnpc_ctx_reserve": [_P, _L],
+    "cnpc_ctx_clear_plans": [_P],
+    "cnpc_paint": [_P, _P, _P, _L, _I, _D, _I, _I, _P, _I, _P, _P],
+    "cnpc_fft_r2c": [_P, _I, _I, _I, _P, _P],
+    "cnpc_fft_c2r": [_P, _I, _I, _I, _P, _P],
+    "cnpc_prepare_spectrum": [_P, _I, _D, _I, _P, _P, _I, _I, _I, _D, _D, _I, _P],
+    "cnpc_bin_spectrum": [_P, _I, _D, _I, _P, _P, _P, _I, _I, _P, _D, _I, _D, _P],
+    "cnpc_shell_filter": [_P, _I, _D, _I, _P, _P, _I, _I, _I, _P, _P],
+    "cnpc_contract_pairs": [_P, _I, _L, _P, _P, _P, _I, _P, _P, _P],
+    "cnpc_contract_gemm": [_P, _I, _L, _P, _P, _I, _P, _I, _I, _F, _F, _I, _P],
+    "cnpc_absmax_f32": [_P, _P, _L, _P],
+    "cnpc_zero": [_P, _P, _L],
+    "cnpc_sync": [_P],
+}
+
+_lib = None
+
+
+class CnpcError(RuntimeError):
+    pass
+
+
+def load(path=LIB_PATH):
+    """Load the shared library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise CnpcError(
+            f"{path} not found: build it with `python -m cosmonpc_amd.build_lib` "
+            "(the MI355X path has no CPU fallback)")
+    lib = ctypes.CDLL(path)
+    for name, args in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = ctypes.c_int
+    lib.cnpc_last_error.argtypes = []
+    lib.cnpc_last_error.restype = ctypes.c_char_p
+    _lib = lib
+    return lib
+
+
+def check(rc, what):
+    if rc != 0:
+        msg = _lib.cnpc_last_error().decode(errors="replace") if _lib is not None else ""
+        raise CnpcError(f"{what} failed (status {rc}): {msg}")

@@ -1,0 +1,255 @@
+"""Host-side angular tables: spherical-harmonic / Legendre weight polynomials
+for the device kernels, Wigner 3j symbols, associated Legendre functions and
+the q_ell kernels of the analytic S3 shot noise.
+
+The reference builds Y_lm symbolically with sympy and evaluates it with
+numexpr (``math_evaluator.py:35-131``); L_ell(r.khat) the same way
+(``:134-197``); 3j symbols and P_l^m with sympy
+(``clustering_estimator.py:5``, ``math_evaluator.py:389-450``).  Here they are
+closed forms:
+
+* Y_lm(n) = c_lm (x + i y)^m d^m P_l/dz^m (z) (Condon-Shortley phase folded
+  into c_lm), which equals the reference's polynomial on the unit sphere;
+* the value at the zero vector (the k = 0 mode, where khat := 0,
+  ``math_evaluator.py:19-21``) is the constant term of the reference's
+  simplified sympy polynomial, tabulated below;
+* 3j symbols from the Racah formula in exact rational arithmetic.
+"""
+import math
+from fractions import Fraction
+
+import numpy as np
+from numpy.polynomial import legendre as npleg
+from numpy.polynomial import polynomial as nppoly
+
+from ._lib import cnpc_poly
+
+PART_FULL, PART_RE, PART_IM, PART_CONJ = 0, 1, 2, 3
+
+# Y_l0 at the zero vector as produced by the reference's sympy construction
+# (Racah normalisation); odd l and m != 0 give 0.  The sympy form of Y_20 is
+# homogeneous (-x^2/2 - y^2/2 + z^2) so its constant is 0, while Y_40, Y_60
+# and Y_80 come out as polynomials in z alone with constant P_l(0).
+YLM_K0_RACAH = {0: 1.0, 2: 0.0, 4: 0.375, 6: -0.3125, 8: 0.2734375}
+
+
+def _legendre_deriv_coefs(ell, m):
+    """Power-basis coefficients of d^m P_ell / dz^m."""
+    c = npleg.leg2poly([0] * ell + [1])
+    if m:
+        c = nppoly.polyder(c, m)
+    return np.asarray(c, dtype=np.float64)
+
+
+def ylm_norm(ell, m, racah):
+    n = math.sqrt(math.factorial(ell - abs(m)) / math.factorial(ell + abs(m)))
+    if not racah:
+        n *= math.sqrt((2 * ell + 1) / (4 * math.pi))
+    return n
+
+
+def ylm_k0(ell, m, racah):
+    if m != 0 or ell % 2 == 1:
+        return 0.0
+    v = YLM_K0_RACAH[ell]
+    if not racah:
+        v *= math.sqrt((2 * ell + 1) / (4 * math.pi))
+    return v
+
+
+def make_poly(m=0, zc=(1.0,), part=PART_FULL, pre=1.0 + 0j, k0=0j, axis=(0.0, 0.0, 1.0)):
+    p = cnpc_poly()
+    zc = list(zc)
+    if len(zc) > 9:
+        raise ValueError("polynomial degree above 8 is not supported")
+    p.m = int(m)
+    p.deg = len(zc) - 1
+    p.part = int(part)
+    for i, v in enumerate(zc):
+        p.zc[i] = float(v)
+    pre = complex(pre)
+    k0 = complex(k0)
+    p.pre_re, p.pre_im = pre.real, pre.imag
+    p.k0_re, p.k0_im = k0.real, k0.imag
+    p.rx, p.ry, p.rz = (float(a) for a in axis)
+    return p
+
+
+def ylm_poly(ell, m, racah=True, part=PART_FULL, factor=1.0 + 0j):
+    """Poly spec of factor * part(Y_lm) (part: full, Re, Im; conj for m<0 is
+    folded in).  The k = 0 value follows the same part/factor."""
+    ell, m = int(ell), int(m)
+    if abs(m) > ell:
+        raise ValueError("abs(m) must be <= l")
+    am = abs(m)
+    c = ylm_norm(ell, m, racah)
+    zc = _legendre_deriv_coefs(ell, am)
+    if m >= 0:
+        pre = c * (-1) ** am
+        base_part = PART_FULL
+    else:
+        # Y_l,-m = (-1)^m conj(Y_lm) = c (x - i y)^m P^(m)(z)
+        pre = c
+        base_part = PART_CONJ
+    k0full = ylm_k0(ell, m, racah)
+    if part == PART_FULL:
+        spec_part = base_part
+        k0 = k0full
+    elif part == PART_RE:
+        spec_part = PART_RE
+        k0 = complex(k0full).real
+    elif part == PART_IM:
+        spec_part = PART_IM
+        k0 = complex(k0full).imag
+        if m < 0:
+            pre = -pre  # Im(conj(P)) = -Im(P)
+    else:
+        raise ValueError(part)
+    return make_poly(am, zc, spec_part, complex(factor) * pre, complex(factor) * k0)
+
+
+def ylm_value(ell, m, xh, yh, zh, racah=True):
+    """Numpy evaluation of the same closed form (host reference for tests)."""
+    am = abs(m)
+    c = ylm_norm(ell, m, racah)
+    zc = _legendre_deriv_coefs(ell, am)
+    P = nppoly.polyval(zh, zc) * (xh + 1j * yh) ** am
+    if m >= 0:
+        return c * (-1) ** am * P
+    return c * np.conj(P)
+
+
+def legendre_poly(ell, rhat):
+    """L_ell(rhat . khat) (``math_evaluator.py:134-197``)."""
+    zc = npleg.leg2poly([0] * int(ell) + [1])
+    k0 = float(nppoly.polyval(0.0, zc))
+    return make_poly(0, zc, PART_FULL, 1.0, k0, axis=rhat)
+
+
+def shell_components(ell, m, racah=True):
+    """Split F = c2r(Y_lm(khat) delta mask) into real fields for a Hermitian
+    delta:  F = sum_c phase_c * c2r(w_c delta mask)  with each w_c delta
+    Hermitian (so each c2r is real).  Returns [(poly, phase), ...].
+
+    Y(-khat) = (-1)^l Y(khat):
+      l even: F = c2r(Re Y delta) + i c2r(Im Y delta)
+      l odd : F = c2r(i Im Y delta) + i c2r(-i Re Y delta)"""
+    out = []
+    has_im = m != 0
+    if ell % 2 == 0:
+        out.append((ylm_poly(ell, m, racah, PART_RE), 1.0 + 0j))
+        if has_im:
+            out.append((ylm_poly(ell, m, racah, PART_IM), 1j))
+    else:
+        if has_im:
+            out.append((ylm_poly(ell, m, racah, PART_IM, factor=1j), 1.0 + 0j))
+        out.append((ylm_poly(ell, m, racah, PART_RE, factor=-1j), 1j))
+    return out
+
+
+# ---------------------------------------------------------------- 3j symbols
+def _f(n):
+    return math.factorial(n)
+
+
+def wigner_3j_exact(j1, j2, j3, m1, m2, m3):
+    """Wigner 3j symbol (Racah formula), exact; returns (sign, Fraction of
+    the squared magnitude) so the float is sign * sqrt(value)."""
+    if m1 + m2 + m3 != 0:
+        return 0, Fraction(0)
+    if abs(m1) > j1 or abs(m2) > j2 or abs(m3) > j3:
+        return 0, Fraction(0)
+    if j3 < abs(j1 - j2) or j3 > j1 + j2:
+        return 0, Fraction(0)
+    tri = Fraction(_f(j1 + j2 - j3) * _f(j1 - j2 + j3) * _f(-j1 + j2 + j3), _f(j1 + j2 + j3 + 1))
+    pre = tri * _f(j1 + m1) * _f(j1 - m1) * _f(j2 + m2) * _f(j2 - m2) * _f(j3 + m3) * _f(j3 - m3)
+    s = Fraction(0)
+    kmin = max(0, j2 - j3 - m1, j1 - j3 + m2)
+    kmax = min(j1 + j2 - j3, j1 - m1, j2 + m2)
+    for k in range(kmin, kmax + 1):
+        s += Fraction((-1) ** k, _f(k) * _f(j1 + j2 - j3 - k) * _f(j1 - m1 - k) * _f(j2 + m2 - k)
+                      * _f(j3 - j2 + m1 + k) * _f(j3 - j1 - m2 + k))
+    if s == 0:
+        return 0, Fraction(0)
+    sign = (-1) ** (j1 - j2 - m3) * (1 if s > 0 else -1)
+    return sign, pre * s * s
+
+
+def wigner_3j(j1, j2, j3, m1, m2, m3):
+    sign, sq = wigner_3j_exact(j1, j2, j3, m1, m2, m3)
+    if sign == 0:
+        return 0.0
+    return sign * math.sqrt(float(sq))
+
+
+def get_magnetic_configs_box(ell_1, ell_2, L):
+    """(m1, m2, 0) with m1 >= 0 and m1 + m2 = 0 (``math_evaluator.py:312-331``)."""
+    cfg, tj = [], []
+    for m1 in range(0, ell_1 + 1):
+        for m2 in range(-ell_2, ell_2 + 1):
+            if m1 + m2 == 0:
+                cfg.append((m1, m2, 0))
+                tj.append(wigner_3j(ell_1, ell_2, L, m1, m2, 0))
+    return cfg, tj
+
+
+def get_magnetic_configs_survey(ell_1, ell_2, L):
+    """(m1, m2, M) scanned M = -L.., m1 = -l1.., up to (0,0,0)
+    (``math_evaluator.py:334-355``)."""
+    cfg, tj = [], []
+    for M in range(-L, L + 1):
+        for m1 in range(-ell_1, ell_1 + 1):
+            m2 = -M - m1
+            if -ell_2 <= m2 <= ell_2:
+                cfg.append((m1, m2, M))
+                tj.append(wigner_3j(ell_1, ell_2, L, m1, m2, M))
+                if (m1, m2, M) == (0, 0, 0):
+                    return cfg, tj
+    return cfg, tj
+
+
+# ---------------------------------------------------------------- P_l^m
+def assoc_legendre(ell, m, x):
+    """P_l^m(x) with the Condon-Shortley phase and the negative-m convention
+    P_l^{-m} = (-1)^m (l-m)!/(l+m)! P_l^m (sympy ``assoc_legendre``)."""
+    am = abs(m)
+    if am > ell:
+        return 0.0
+    d = nppoly.polyval(x, _legendre_deriv_coefs(ell, am))
+    v = (-1) ** am * (1.0 - x * x) ** (0.5 * am) * d
+    if m < 0:
+        v *= (-1) ** am * math.factorial(ell - am) / math.factorial(ell + am)
+    return float(v)
+
+
+def get_associated_legendre_coefficients_eff(ell, m, k1, k2, k3_eff, mode="13"):
+    """``math_evaluator.py:418-446``."""
+    res = np.zeros(len(k3_eff))
+    for i, k3 in enumerate(k3_eff):
+        if abs(k1 - k2) <= k3 <= (k1 + k2):
+            if mode == "13":
+                mu = (k3 ** 2 + k1 ** 2 - k2 ** 2) / (2 * k1 * k3)
+            else:
+                mu = (k3 ** 2 + k2 ** 2 - k1 ** 2) / (2 * k2 * k3)
+            res[i] = assoc_legendre(ell, m, mu)
+    res *= (-1) ** (ell + m)
+    return res
+
+
+def get_valid_k3_bins(k1, k2, k_min, k_max, kbin):
+    """``math_evaluator.py:448-452``."""
+    e = np.linspace(2 * k_min, 2 * k_max, kbin * 2 + 1)[:-1]
+    c = 0.5 * (e[1:] + e[:-1])
+    return np.logical_and(c >= abs(k1 - k2), c <= (k1 + k2))
+
+
+def get_q_ells_eff(i, j, k_eff, ell_1, ell_2, L, k3_bins, k3_eff):
+    """q_{l1 l2 L}(k1, k2, k3) at effective k (``math_evaluator.py:592-616``)."""
+    q = np.zeros(k3_bins)
+    lmin = min(ell_1, ell_2)
+    for xx in range(-lmin, lmin + 1):
+        tj = wigner_3j(ell_1, ell_2, L, xx, -xx, 0)
+        a13 = get_associated_legendre_coefficients_eff(ell_1, xx, k_eff[i], k_eff[j], k3_eff, "13")
+        a23 = get_associated_legendre_coefficients_eff(ell_2, -xx, k_eff[i], k_eff[j], k3_eff, "23")
+        q += (-1) ** xx * tj * a13 * a23
+    return q

@@ -1,0 +1,358 @@
+// C ABI of libcnpc_hip.so: context (stream, rocFFT plan cache, scratch) and
+// the thin wrappers that validate arguments and launch the kernels of
+// paint.hip, spectral.hip and contract.hip.  See include/cnpc.h.
+#include <rocfft/rocfft.h>
+
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/cnpc.h"
+#include "cnpc_common.h"
+
+namespace cnpc {
+
+static thread_local std::string g_err;
+void set_error(const std::string& msg) { g_err = msg; }
+
+__global__ void zero_kernel(uint4* p, int64_t n16, unsigned char* tail, int ntail) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = make_uint4(0, 0, 0, 0);
+  if (blockIdx.x == 0 && (int)threadIdx.x < ntail) tail[threadIdx.x] = 0;
+}
+
+__global__ void absmax_kernel(const float* __restrict__ x, int64_t n, double* __restrict__ partial) {
+  float m = 0.f;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    m = fmaxf(m, fabsf(x[i]));
+  for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_down(m, off, 64));
+  __shared__ float red[4];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) partial[blockIdx.x] = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+}
+
+}  // namespace cnpc
+
+using namespace cnpc;
+
+struct FftPlan {
+  rocfft_plan plan = nullptr;
+  rocfft_execution_info info = nullptr;
+  void* work = nullptr;
+  size_t work_bytes = 0;
+};
+
+struct cnpc_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::map<std::tuple<int, int, int, int>, FftPlan> plans;
+  void* scratch = nullptr;
+  size_t scratch_bytes = 0;
+};
+
+#define CHECK_HIP(expr)                                                                        \
+  do {                                                                                         \
+    hipError_t e_ = (expr);                                                                    \
+    if (e_ != hipSuccess) {                                                                    \
+      set_error(std::string(#expr) + ": " + hipGetErrorString(e_));                            \
+      return 2;                                                                                \
+    }                                                                                          \
+  } while (0)
+
+#define REQUIRE(cond, msg)     \
+  do {                         \
+    if (!(cond)) {             \
+      set_error(msg);          \
+      return 1;                \
+    }                          \
+  } while (0)
+
+static int ensure_scratch(cnpc_ctx* ctx, size_t bytes) {
+  if (ctx->scratch_bytes >= bytes) return 0;
+  if (ctx->scratch) {
+    CHECK_HIP(hipStreamSynchronize(ctx->stream));
+    CHECK_HIP(hipFree(ctx->scratch));
+    ctx->scratch = nullptr;
+    ctx->scratch_bytes = 0;
+  }
+  CHECK_HIP(hipMalloc(&ctx->scratch, bytes));
+  ctx->scratch_bytes = bytes;
+  return 0;
+}
+
+static int get_plan(cnpc_ctx* ctx, int n, int batch, int prec, int dir, FftPlan** out) {
+  auto key = std::make_tuple(n, batch, prec, dir);
+  auto it = ctx->plans.find(key);
+  if (it != ctx->plans.end()) {
+    *out = &it->second;
+    return 0;
+  }
+  static std::once_flag once;
+  std::call_once(once, [] { rocfft_setup(); });
+  FftPlan p;
+  size_t lengths[3] = {(size_t)n, (size_t)n, (size_t)n};
+  rocfft_status s = rocfft_plan_create(
+      &p.plan, rocfft_placement_notinplace,
+      dir == 0 ? rocfft_transform_type_real_forward : rocfft_transform_type_real_inverse,
+      prec == 0 ? rocfft_precision_double : rocfft_precision_single, 3, lengths, (size_t)batch, nullptr);
+  REQUIRE(s == rocfft_status_success, "rocfft_plan_create failed (status " + std::to_string((int)s) + ")");
+  s = rocfft_plan_get_work_buffer_size(p.plan, &p.work_bytes);
+  REQUIRE(s == rocfft_status_success, "rocfft_plan_get_work_buffer_size failed");
+  s = rocfft_execution_info_create(&p.info);
+  REQUIRE(s == rocfft_status_success, "rocfft_execution_info_create failed");
+  if (p.work_bytes) {
+    CHECK_HIP(hipMalloc(&p.work, p.work_bytes));
+    s = rocfft_execution_info_set_work_buffer(p.info, p.work, p.work_bytes);
+    REQUIRE(s == rocfft_status_success, "rocfft_execution_info_set_work_buffer failed");
+  }
+  auto res = ctx->plans.emplace(key, p);
+  *out = &res.first->second;
+  return 0;
+}
+
+static void fill_edges(BinEdges& E, const double* edges, int nbins) {
+  std::memset(&E, 0, sizeof(E));
+  E.nbins = nbins;
+  for (int i = 0; i <= nbins; ++i) E.e[i] = edges[i];
+  E.e0 = edges[0];
+  const double de = (edges[nbins] - edges[0]) / nbins;
+  E.inv_de = de > 0 ? 1.0 / de : 0.0;
+}
+
+static void to_spec(PolySpec& d, const cnpc_poly& s) {
+  static_assert(sizeof(PolySpec) == sizeof(cnpc_poly), "PolySpec layout");
+  std::memcpy(&d, &s, sizeof(PolySpec));
+}
+
+extern "C" {
+
+const char* cnpc_last_error(void) { return g_err.c_str(); }
+int cnpc_version(void) { return 1; }
+
+int cnpc_ctx_create(cnpc_ctx** out, int device) {
+  REQUIRE(out != nullptr, "cnpc_ctx_create: null out pointer");
+  int ndev = 0;
+  CHECK_HIP(hipGetDeviceCount(&ndev));
+  REQUIRE(device >= 0 && device < ndev, "cnpc_ctx_create: invalid device " + std::to_string(device));
+  CHECK_HIP(hipSetDevice(device));
+  cnpc_ctx* c = new cnpc_ctx();
+  c->device = device;
+  *out = c;
+  return 0;
+}
+
+int cnpc_ctx_clear_plans(cnpc_ctx* ctx) {
+  REQUIRE(ctx, "null context");
+  CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  for (auto& kv : ctx->plans) {
+    if (kv.second.plan) rocfft_plan_destroy(kv.second.plan);
+    if (kv.second.info) rocfft_execution_info_destroy(kv.second.info);
+    if (kv.second.work) (void)hipFree(kv.second.work);
+  }
+  ctx->plans.clear();
+  return 0;
+}
+
+int cnpc_ctx_destroy(cnpc_ctx* ctx) {
+  if (!ctx) return 0;
+  int rc = cnpc_ctx_clear_plans(ctx);
+  if (ctx->scratch) (void)hipFree(ctx->scratch);
+  delete ctx;
+  return rc;
+}
+
+int cnpc_ctx_set_stream(cnpc_ctx* ctx, void* stream) {
+  REQUIRE(ctx, "null context");
+  ctx->stream = (hipStream_t)stream;
+  return 0;
+}
+
+int cnpc_ctx_reserve(cnpc_ctx* ctx, int64_t bytes) {
+  REQUIRE(ctx && bytes >= 0, "cnpc_ctx_reserve: bad arguments");
+  return ensure_scratch(ctx, (size_t)bytes);
+}
+
+int cnpc_sync(cnpc_ctx* ctx) {
+  REQUIRE(ctx, "null context");
+  CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int cnpc_zero(cnpc_ctx* ctx, void* ptr, int64_t nbytes) {
+  REQUIRE(ctx && (ptr || nbytes == 0) && nbytes >= 0, "cnpc_zero: bad arguments");
+  if (nbytes == 0) return 0;
+  REQUIRE(((uintptr_t)ptr & 15) == 0, "cnpc_zero: pointer must be 16-byte aligned");
+  const int64_t n16 = nbytes / 16;
+  const int ntail = (int)(nbytes - n16 * 16);
+  hipLaunchKernelGGL(zero_kernel, dim3(grid_for(n16 > 0 ? n16 : 1, 256, 1 << 14)), dim3(256), 0, ctx->stream,
+                     (uint4*)ptr, n16, (unsigned char*)ptr + n16 * 16, ntail);
+  CHECK_HIP(hipGetLastError());
+  return 0;
+}
+
+int cnpc_paint(cnpc_ctx* ctx, const double* pos, const double* w, int64_t np, int n, double boxsize, int window,
+               int interlaced, const double* center, int prec, void* mesh0, void* mesh1) {
+  REQUIRE(ctx, "null context");
+  REQUIRE(n >= 2 && n % 2 == 0 && n <= 4096, "cnpc_paint: nmesh must be even and in [2, 4096]");
+  REQUIRE(window >= 1 && window <= 4, "cnpc_paint: window must be 1..4 (ngp, cic, tsc, pcs)");
+  REQUIRE(prec == 0 || prec == 1, "cnpc_paint: prec must be 0 (fp64) or 1 (fp32)");
+  REQUIRE(boxsize > 0, "cnpc_paint: boxsize must be positive");
+  REQUIRE(np >= 0 && (np == 0 || (pos && w)), "cnpc_paint: null particle arrays");
+  REQUIRE(mesh0 && (!interlaced || mesh1), "cnpc_paint: null mesh");
+  const int64_t bytes = (int64_t)n * n * n * (prec == 0 ? 8 : 4);
+  if (int rc = cnpc_zero(ctx, mesh0, bytes)) return rc;
+  if (interlaced)
+    if (int rc = cnpc_zero(ctx, mesh1, bytes)) return rc;
+  CHECK_HIP(launch_paint(pos, w, np, n, boxsize, window, center, interlaced ? 2 : 1, prec, mesh0, mesh1, ctx->stream));
+  return 0;
+}
+
+static int run_fft(cnpc_ctx* ctx, int n, int batch, int prec, int dir, void* in, void* out) {
+  REQUIRE(ctx, "null context");
+  REQUIRE(n >= 2 && n % 2 == 0 && batch >= 1, "fft: bad size");
+  REQUIRE(prec == 0 || prec == 1, "fft: prec must be 0 or 1");
+  REQUIRE(in && out && in != out, "fft: null or aliased buffers (out-of-place only)");
+  FftPlan* p = nullptr;
+  if (int rc = get_plan(ctx, n, batch, prec, dir, &p)) return rc;
+  rocfft_status s = rocfft_execution_info_set_stream(p->info, ctx->stream);
+  REQUIRE(s == rocfft_status_success, "rocfft_execution_info_set_stream failed");
+  void* ib[1] = {in};
+  void* ob[1] = {out};
+  s = rocfft_execute(p->plan, ib, ob, p->info);
+  REQUIRE(s == rocfft_status_success, "rocfft_execute failed (status " + std::to_string((int)s) + ")");
+  return 0;
+}
+
+int cnpc_fft_r2c(cnpc_ctx* ctx, int n, int batch, int prec, void* in, void* out) {
+  return run_fft(ctx, n, batch, prec, 0, in, out);
+}
+int cnpc_fft_c2r(cnpc_ctx* ctx, int n, int batch, int prec, void* in, void* out) {
+  return run_fft(ctx, n, batch, prec, 1, in, out);
+}
+
+int cnpc_prepare_spectrum(cnpc_ctx* ctx, int n, double boxsize, int prec, const void* c1, const void* c2,
+                          int interlaced, int comp, int window, double scale, double sub0, int zero_k0, void* out) {
+  REQUIRE(ctx, "null context");
+  REQUIRE(n >= 2 && n % 2 == 0 && boxsize > 0, "prepare: bad mesh");
+  REQUIRE(prec == 0 || prec == 1, "prepare: prec must be 0 or 1");
+  REQUIRE(comp >= 0 && comp <= 2 && window >= 1 && window <= 4, "prepare: bad compensation / window");
+  REQUIRE(c1 && out && (!interlaced || c2), "prepare: null buffers");
+  PrepareArgs a;
+  a.n = n; a.interlaced = interlaced ? 1 : 0; a.comp = comp; a.p = window;
+  a.kf = 2.0 * M_PI / boxsize; a.H = boxsize / n; a.scale = scale; a.sub0 = sub0;
+  a.zero0 = zero_k0 ? 1 : 0; a.pad = 0;
+  CHECK_HIP(launch_prepare(prec, a, c1, c2, out, ctx->stream));
+  return 0;
+}
+
+int cnpc_bin_spectrum(cnpc_ctx* ctx, int n, double boxsize, int prec, const void* A, const void* B,
+                      const double* edges, int nbins, int npoly, const cnpc_poly* polys, double ab_scale,
+                      int noise_window, double noise_amp, double* out) {
+  REQUIRE(ctx, "null context");
+  REQUIRE(n >= 2 && n % 2 == 0 && boxsize > 0, "bin: bad mesh");
+  REQUIRE(nbins >= 1 && nbins <= CNPC_MAX_BINS, "bin: nbins must be in [1, 256]");
+  REQUIRE(npoly >= 0 && npoly <= CNPC_MAX_POLY, "bin: npoly must be in [0, 9]");
+  REQUIRE(edges && out, "bin: null edges / out");
+  REQUIRE(npoly == 0 || (A && B && polys), "bin: null spectra / polys");
+  REQUIRE(noise_window >= 0 && noise_window <= 4, "bin: bad noise window");
+  for (int i = 0; i < nbins; ++i) REQUIRE(edges[i + 1] > edges[i], "bin: edges must increase");
+  BinEdges E;
+  fill_edges(E, edges, nbins);
+  PolySet P;
+  std::memset(&P, 0, sizeof(P));
+  for (int q = 0; q < npoly; ++q) {
+    REQUIRE(polys[q].deg >= 0 && polys[q].deg < CNPC_ZDEG && polys[q].m >= 0, "bin: bad polynomial");
+    to_spec(P.s[q], polys[q]);
+  }
+  BinArgs a;
+  a.n = n; a.npoly = npoly; a.noise_p = noise_window; a.conj_b = 1;
+  a.kf = 2.0 * M_PI / boxsize; a.ab_scale = ab_scale; a.noise_amp = noise_amp;
+  if (int rc = cnpc_zero(ctx, out, (int64_t)(2 + 2 * npoly) * nbins * 8)) return rc;
+  CHECK_HIP(launch_bin(prec, a, E, P, A, B, out, ctx->stream));
+  return 0;
+}
+
+int cnpc_shell_filter(cnpc_ctx* ctx, int n, double boxsize, int prec, const void* D, const double* edges, int nbins,
+                      int bin0, int nshell, const cnpc_poly* weight, void* out) {
+  REQUIRE(ctx, "null context");
+  REQUIRE(n >= 2 && n % 2 == 0 && boxsize > 0, "shell_filter: bad mesh");
+  REQUIRE(prec == 0 || prec == 1, "shell_filter: prec must be 0 or 1");
+  REQUIRE(nbins >= 1 && nbins <= CNPC_MAX_BINS && edges, "shell_filter: bad edges");
+  REQUIRE(bin0 >= 0 && nshell >= 1 && bin0 + nshell <= nbins, "shell_filter: shell range outside the bins");
+  REQUIRE(D && out && weight, "shell_filter: null buffers");
+  REQUIRE(weight->deg >= 0 && weight->deg < CNPC_ZDEG && weight->m >= 0, "shell_filter: bad weight polynomial");
+  BinEdges E;
+  fill_edges(E, edges, nbins);
+  PolySpec W;
+  to_spec(W, *weight);
+  ShellArgs a;
+  a.n = n; a.bin0 = bin0; a.nshell = nshell; a.pad = 0;
+  a.kf = 2.0 * M_PI / boxsize;
+  a.stride = (int64_t)n * n * (n / 2 + 1);
+  CHECK_HIP(launch_shell_filter(prec, a, E, W, D, out, ctx->stream));
+  return 0;
+}
+
+int cnpc_contract_pairs(cnpc_ctx* ctx, int prec, int64_t ncell, const void* G, const void* A, const void* B,
+                        int npairs, const int32_t* ia, const int32_t* ib, double* out) {
+  REQUIRE(ctx, "null context");
+  REQUIRE(prec == 0 || prec == 1, "contract_pairs: prec must be 0 or 1");
+  REQUIRE(ncell > 0 && npairs >= 0 && G && A && B && (npairs == 0 || (ia && ib && out)),
+          "contract_pairs: bad arguments");
+  if (npairs == 0) return 0;
+  for (int p = 0; p < npairs; ++p) REQUIRE(ia[p] >= 0 && ib[p] >= 0, "contract_pairs: negative shell index");
+  const int nblocks = 2048;
+  if (int rc = ensure_scratch(ctx, (size_t)(nblocks + npairs) * 8)) return rc;
+  double* partial = (double*)ctx->scratch;
+  double* dout = partial + nblocks;
+  const size_t es = prec == 0 ? 8 : 4;
+  for (int p = 0; p < npairs; ++p) {
+    const char* a = (const char*)A + (size_t)ia[p] * ncell * es;
+    const char* b = (const char*)B + (size_t)ib[p] * ncell * es;
+    CHECK_HIP(launch_triple_dot(prec, G, a, b, ncell, partial, nblocks, dout + p, ctx->stream));
+  }
+  CHECK_HIP(hipMemcpyAsync(out, dout, (size_t)npairs * 8, hipMemcpyDeviceToHost, ctx->stream));
+  CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int cnpc_contract_gemm(cnpc_ctx* ctx, int mode, int64_t ncell, const void* G, const void* A, int na, const void* B,
+                       int nb, int same, float sa, float sb, int nblk, double* C) {
+  REQUIRE(ctx, "null context");
+  REQUIRE(mode >= 0 && mode <= 2, "contract_gemm: mode must be 0 (bf16x3), 1 (fp16x3) or 2 (fp64)");
+  REQUIRE(ncell > 0 && na >= 1 && nb >= 1 && G && A && B && C, "contract_gemm: bad arguments");
+  REQUIRE(ncell % 64 == 0, "contract_gemm: ncell must be a multiple of 64");
+  REQUIRE(!same || (A == B && na == nb), "contract_gemm: same requires A == B and na == nb");
+  REQUIRE(sa > 0 && sb > 0, "contract_gemm: prescales must be positive");
+  const size_t es = mode == 2 ? 8 : 4;
+  const uintptr_t mis = (uintptr_t)G | (uintptr_t)A | (uintptr_t)B;
+  REQUIRE((mis & 15) == 0, "contract_gemm: buffers must be 16-byte aligned");
+  (void)es;
+  if (nblk <= 0) nblk = 2048;
+  if (nblk > 65536) nblk = 65536;
+  if (int rc = ensure_scratch(ctx, (size_t)nblk * 1024 * 8)) return rc;
+  CHECK_HIP(launch_gemm(mode, G, A, ncell, na, B, ncell, nb, same ? 1 : 0, ncell, sa, sb, nblk,
+                        (double*)ctx->scratch, C, ctx->stream));
+  return 0;
+}
+
+int cnpc_absmax_f32(cnpc_ctx* ctx, const float* x, int64_t n, double* out) {
+  REQUIRE(ctx && x && out && n > 0, "absmax: bad arguments");
+  const int nblocks = 1024;
+  if (int rc = ensure_scratch(ctx, (size_t)nblocks * 8)) return rc;
+  hipLaunchKernelGGL(absmax_kernel, dim3(nblocks), dim3(256), 0, ctx->stream, x, n, (double*)ctx->scratch);
+  CHECK_HIP(hipGetLastError());
+  std::vector<double> h(nblocks);
+  CHECK_HIP(hipMemcpyAsync(h.data(), ctx->scratch, nblocks * 8, hipMemcpyDeviceToHost, ctx->stream));
+  CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  double m = 0;
+  for (double v : h) m = v > m ? v : m;
+  *out = m;
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,158 @@
+// Shared declarations of the cnpc HIP library (gfx950).
+//
+// Layouts in HBM (all C order, x slowest, z fastest):
+//   real mesh       : N*N*N scalars (double or float)
+//   half spectrum   : N*N*(N/2+1) complex (double2 / float2), the rocFFT
+//                     real-forward output; index (ix*N + iy)*(N/2+1) + iz
+//   shell stack     : nshell consecutive real meshes, shell s at s*N^3
+// Frequencies follow the wrapped convention n = i for i < N/2, i - N for
+// i >= N/2 (so the last z plane of the half spectrum is n_z = -N/2).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+
+#define CNPC_MAX_BINS 256
+#define CNPC_MAX_POLY 9
+#define CNPC_ZDEG 9
+
+namespace cnpc {
+
+void set_error(const std::string& msg);
+
+// Complex polynomial weight evaluated at a unit vector n = (x, y, z):
+//   w(n) = pre * part(P(n)),  part = P (0), Re P (1), Im P (2), conj P (3)
+//   P(n) = (x + i y)^m * sum_j zc[j] * t^j,   t = rx*x + ry*y + rz*z
+// At the zero vector (the k = 0 mode, where khat is set to 0) the weight is
+// the fixed value (k0_re, k0_im).  Spherical harmonics Y_lm (m >= 0, axis z)
+// and Legendre polynomials L_l(r . n) are both of this form.
+struct PolySpec {
+  int m;
+  int deg;
+  int part;
+  int pad;
+  double zc[CNPC_ZDEG];
+  double pre_re, pre_im;
+  double k0_re, k0_im;
+  double rx, ry, rz;
+};
+
+struct BinEdges {
+  int nbins;
+  int pad;
+  double e0;
+  double inv_de;
+  double e[CNPC_MAX_BINS + 1];
+};
+
+__device__ __forceinline__ int wrap_freq(int i, int n) { return i >= (n >> 1) ? i - n : i; }
+
+// |k| exactly as the reference forms it: sqrt((0 + kx^2) + ky^2) + kz^2) with
+// k_d = n_d * (2 pi / L); no fused multiply-add so bin membership is the
+// same bit pattern as numpy's.
+__device__ __forceinline__ double knorm_of(int nx, int ny, int nz, double kf, double* kx, double* ky, double* kz) {
+  double a = __dmul_rn((double)nx, kf);
+  double b = __dmul_rn((double)ny, kf);
+  double c = __dmul_rn((double)nz, kf);
+  *kx = a; *ky = b; *kz = c;
+  double s = __dadd_rn(__dadd_rn(__dmul_rn(a, a), __dmul_rn(b, b)), __dmul_rn(c, c));
+  return __dsqrt_rn(s);
+}
+
+// bin b with e[b] <= k < e[b+1], or -1 (numpy searchsorted side='right' - 1
+// restricted to [e[0], e[nbins]) as in math_evaluator.radial_binning).
+__device__ __forceinline__ int bin_of(const BinEdges& E, double k) {
+  if (!(k >= E.e[0]) || !(k < E.e[E.nbins])) return -1;
+  int b = (int)floor((k - E.e0) * E.inv_de);
+  if (b < 0) b = 0;
+  if (b > E.nbins - 1) b = E.nbins - 1;
+  while (b > 0 && k < E.e[b]) --b;
+  while (b < E.nbins - 1 && k >= E.e[b + 1]) ++b;
+  return b;
+}
+
+__device__ __forceinline__ double2 poly_eval(const PolySpec& s, double x, double y, double z, bool zero_vec) {
+  if (zero_vec) return make_double2(s.k0_re, s.k0_im);
+  double t = s.rx * x + s.ry * y + s.rz * z;
+  double p = s.zc[s.deg];
+  for (int j = s.deg - 1; j >= 0; --j) p = p * t + s.zc[j];
+  double re = p, im = 0.0;
+  for (int j = 0; j < s.m; ++j) {  // multiply by (x + i y)
+    double nr = re * x - im * y;
+    double ni = re * y + im * x;
+    re = nr; im = ni;
+  }
+  if (s.part == 1) {
+    im = 0.0;
+  } else if (s.part == 2) {
+    re = im;
+    im = 0.0;
+  } else if (s.part == 3) {
+    im = -im;
+  }
+  return make_double2(s.pre_re * re - s.pre_im * im, s.pre_re * im + s.pre_im * re);
+}
+
+template <typename T> struct cplx;
+template <> struct cplx<double> { typedef double2 type; };
+template <> struct cplx<float> { typedef float2 type; };
+
+inline unsigned grid_for(int64_t n, int block, int max_blocks = 8192) {
+  int64_t g = (n + block - 1) / block;
+  if (g > max_blocks) g = max_blocks;
+  if (g < 1) g = 1;
+  return (unsigned)g;
+}
+
+
+// ---------------------------------------------------------------- kernel args
+// comp: 0 none, 1 divide by prod_d sinc(w_d/2)^p, 2 divide by prod_d sqrt(C_p(w_d))
+struct PrepareArgs {
+  int n;
+  int interlaced;
+  int comp;
+  int p;
+  double kf;      // 2 pi / L
+  double H;       // L / N
+  double scale;   // multiplies the (combined) spectrum
+  double sub0;    // subtracted from the k = 0 mode after scaling
+  int zero0;      // clear the k = 0 mode
+  int pad;
+};
+
+struct BinArgs {
+  int n;
+  int npoly;
+  int noise_p;        // window order of the bk noise term
+  int conj_b;         // 1: f = a * conj(b);  0: f = a * b
+  double kf;
+  double ab_scale;    // multiplies a*conj(b)
+  double noise_amp;   // f -= noise_amp * C_p/sinc^{2p}
+};
+
+struct PolySet {
+  PolySpec s[CNPC_MAX_POLY];
+};
+
+struct ShellArgs {
+  int n;
+  int bin0;
+  int nshell;
+  int pad;
+  double kf;
+  int64_t stride;  // complex elements between consecutive output shells
+};
+
+hipError_t launch_paint(const double* pos, const double* w, int64_t np, int n, double boxsize, int order,
+                        const double* center, int ncopy, int prec, void* m0, void* m1, hipStream_t st);
+hipError_t launch_prepare(int prec, const PrepareArgs& a, const void* c1, const void* c2, void* out, hipStream_t st);
+hipError_t launch_bin(int prec, const BinArgs& a, const BinEdges& E, const PolySet& P, const void* A, const void* B,
+                      double* out, hipStream_t st);
+hipError_t launch_shell_filter(int prec, const ShellArgs& a, const BinEdges& E, const PolySpec& W, const void* D,
+                               void* out, hipStream_t st);
+hipError_t launch_triple_dot(int prec, const void* G, const void* A, const void* B, int64_t n, double* partial,
+                             int nblocks, double* out, hipStream_t st);
+hipError_t launch_gemm(int mode, const void* G, const void* A, int64_t lda, int na, const void* B, int64_t ldb, int nb,
+                       int same, int64_t K, float sa, float sb, int nblk, double* partial, double* C, hipStream_t st);
+
+}  // namespace cnpc

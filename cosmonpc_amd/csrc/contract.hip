@@ -1,0 +1,320 @@
+// Triple-product contractions  S_ab = sum_x A_a(x) * G(x) * B_b(x).
+//
+// The reference forms each (k1, k2) entry of the Sugiyama bispectrum as one
+// full pass  np.sum(G_00 * F_i * F_j)  over the mesh
+// (clustering_estimator.py:972, :1053).  Three GPU forms live here:
+//
+//  * pairwise  -- one streaming FP64 reduction per (a, b) pair, the
+//                 reference's own loop structure (plain FP64 path);
+//  * gemm_f64  -- all (a, b) at once as a split-K skinny GEMM on FP64 MFMA
+//                 (v_mfma_f64_16x16x4), each shell read once;
+//  * gemm_x3   -- the same GEMM with FP32-stored shells split in registers
+//                 into hi + lo halves (bf16 or fp16) and three MFMA products
+//                 hi*hi + hi*lo + lo*hi accumulated in FP32
+//                 (v_mfma_f32_16x16x32_{bf16,f16}).
+//
+// GEMM tiling: a workgroup of 4 waves owns a contiguous K range; every wave
+// computes the full 32 x 32 output tile as 2 x 2 MFMA blocks over its
+// interleaved K steps; the 4 waves are summed in LDS and each workgroup
+// writes one FP64 32 x 32 partial.  A finalize kernel sums the partials in a
+// fixed order (deterministic).  Rows beyond na / nb re-read the last valid
+// row (same cache lines, no extra HBM traffic) and their outputs are dropped.
+#include "cnpc_common.h"
+
+namespace cnpc {
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+
+// ------------------------------------------------------------- pairwise
+template <typename T>
+__global__ void __launch_bounds__(256) triple_dot_kernel(const T* __restrict__ G, const T* __restrict__ A,
+                                                         const T* __restrict__ B, int64_t n, double* __restrict__ partial) {
+  double acc = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    acc += (double)G[i] * (double)A[i] * (double)B[i];
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+  __shared__ double red[4];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) red[w] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+__global__ void sum_partials_kernel(const double* __restrict__ partial, int nparts, int nout, double* __restrict__ out,
+                                    int accumulate) {
+  // one thread per output; partial layout [nparts][nout]
+  for (int o = blockIdx.x * blockDim.x + threadIdx.x; o < nout; o += gridDim.x * blockDim.x) {
+    double s = 0.0;
+    for (int p = 0; p < nparts; ++p) s += partial[(int64_t)p * nout + o];
+    out[o] = accumulate ? out[o] + s : s;
+  }
+}
+
+hipError_t launch_triple_dot(int prec, const void* G, const void* A, const void* B, int64_t n, double* partial,
+                             int nblocks, double* out, hipStream_t st) {
+  if (prec == 0)
+    hipLaunchKernelGGL(triple_dot_kernel<double>, dim3(nblocks), dim3(256), 0, st, (const double*)G, (const double*)A,
+                       (const double*)B, n, partial);
+  else
+    hipLaunchKernelGGL(triple_dot_kernel<float>, dim3(nblocks), dim3(256), 0, st, (const float*)G, (const float*)A,
+                       (const float*)B, n, partial);
+  hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(64), 0, st, partial, nblocks, 1, out, 0);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------- GEMM common
+struct GemmArgs {
+  const void* G;
+  const void* A;
+  const void* B;
+  int64_t lda, ldb;   // elements between consecutive shells
+  int na, nb;         // valid rows
+  int a0, b0;         // first row of this launch's tile
+  int same;           // B == A (same rows): reuse the A registers
+  int pad;
+  int64_t K;          // cells
+  int64_t kper;       // K elements per workgroup (multiple of the step)
+  float sa, sb;       // power-of-two prescales (fp16 split)
+  double* partial;    // [gridDim.x][32*32]
+};
+
+__device__ __forceinline__ void block_reduce_store(const float (&acc)[2][2][4], double* out) {
+  // acc[bi][bj][j] holds C[16*bi + 4*(lane/16) + j][16*bj + lane%16]
+  __shared__ double red[32 * 32];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int i = threadIdx.x; i < 1024; i += blockDim.x) red[i] = 0.0;
+  __syncthreads();
+  for (int ww = 0; ww < 4; ++ww) {
+    if (w == ww) {
+      for (int bi = 0; bi < 2; ++bi)
+        for (int bj = 0; bj < 2; ++bj)
+          for (int j = 0; j < 4; ++j) {
+            const int r = 16 * bi + 4 * (lane >> 4) + j, c = 16 * bj + (lane & 15);
+            red[r * 32 + c] += (double)acc[bi][bj][j];
+          }
+    }
+    __syncthreads();
+  }
+  for (int i = threadIdx.x; i < 1024; i += blockDim.x) out[i] = red[i];
+}
+
+// ------------------------------------------------------------- gemm_x3
+template <bool F16>
+__device__ __forceinline__ void split8(const float (&v)[8], bf16x8& hi, bf16x8& lo, f16x8& hh, f16x8& ll) {
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    if (F16) {
+      _Float16 h = (_Float16)v[j];
+      hh[j] = h;
+      ll[j] = (_Float16)(v[j] - (float)h);
+    } else {
+      __bf16 h = (__bf16)v[j];
+      hi[j] = h;
+      lo[j] = (__bf16)(v[j] - (float)h);
+    }
+  }
+}
+
+template <bool F16>
+__global__ void __launch_bounds__(256) gemm_x3_kernel(GemmArgs g) {
+  const float* __restrict__ G = (const float*)g.G;
+  const float* __restrict__ A = (const float*)g.A;
+  const float* __restrict__ B = (const float*)g.B;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int rr = lane & 15, kg = lane >> 4;  // MFMA row / k-group of this lane
+  int ra[2], rb[2];
+  for (int h = 0; h < 2; ++h) {
+    ra[h] = min(g.a0 + 16 * h + rr, g.na - 1);
+    rb[h] = min(g.b0 + 16 * h + rr, g.nb - 1);
+  }
+  const float* pa[2] = {A + ra[0] * g.lda, A + ra[1] * g.lda};
+  const float* pb[2] = {B + rb[0] * g.ldb, B + rb[1] * g.ldb};
+  float acc[2][2][4];
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 2; ++j)
+      for (int q = 0; q < 4; ++q) acc[i][j][q] = 0.f;
+
+  const int64_t kbeg = (int64_t)blockIdx.x * g.kper;
+  int64_t kend = kbeg + g.kper;
+  if (kend > g.K) kend = g.K;
+  // each MFMA step consumes 32 k: lane k-group kg takes 8 consecutive floats
+  for (int64_t k0 = kbeg + 32 * w; k0 < kend; k0 += 128) {
+    const int64_t ko = k0 + 8 * kg;
+    float va[2][8], vb[2][8], vg[8];
+    const bool full = (k0 + 32 <= kend);
+    if (full) {
+      for (int h = 0; h < 2; ++h) {
+        const f32x4* s = (const f32x4*)(pa[h] + ko);
+        f32x4 x0 = s[0], x1 = s[1];
+        for (int j = 0; j < 4; ++j) { va[h][j] = x0[j]; va[h][4 + j] = x1[j]; }
+      }
+      const f32x4* sg = (const f32x4*)(G + ko);
+      f32x4 g0 = sg[0], g1 = sg[1];
+      for (int j = 0; j < 4; ++j) { vg[j] = g0[j]; vg[4 + j] = g1[j]; }
+      if (!g.same) {
+        for (int h = 0; h < 2; ++h) {
+          const f32x4* s = (const f32x4*)(pb[h] + ko);
+          f32x4 x0 = s[0], x1 = s[1];
+          for (int j = 0; j < 4; ++j) { vb[h][j] = x0[j]; vb[h][4 + j] = x1[j]; }
+        }
+      }
+    } else {
+      for (int j = 0; j < 8; ++j) {
+        const int64_t kk = ko + j;
+        const bool ok = kk < kend;
+        vg[j] = ok ? G[kk] : 0.f;
+        for (int h = 0; h < 2; ++h) {
+          va[h][j] = ok ? pa[h][kk] : 0.f;
+          if (!g.same) vb[h][j] = ok ? pb[h][kk] : 0.f;
+        }
+      }
+    }
+    if (g.same)
+      for (int h = 0; h < 2; ++h)
+        for (int j = 0; j < 8; ++j) vb[h][j] = va[h][j];
+    bf16x8 ah[2], al[2], bh[2], bl[2];
+    f16x8 fah[2], fal[2], fbh[2], fbl[2];
+    for (int h = 0; h < 2; ++h) {
+      float ta[8], tb[8];
+      for (int j = 0; j < 8; ++j) {
+        ta[j] = va[h][j] * g.sa;
+        tb[j] = vb[h][j] * vg[j] * g.sb;
+      }
+      split8<F16>(ta, ah[h], al[h], fah[h], fal[h]);
+      split8<F16>(tb, bh[h], bl[h], fbh[h], fbl[h]);
+    }
+    for (int i = 0; i < 2; ++i)
+      for (int j = 0; j < 2; ++j) {
+        f32x4 c = {acc[i][j][0], acc[i][j][1], acc[i][j][2], acc[i][j][3]};
+        if (F16) {
+          c = __builtin_amdgcn_mfma_f32_16x16x32_f16(fah[i], fbh[j], c, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_16x16x32_f16(fah[i], fbl[j], c, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_16x16x32_f16(fal[i], fbh[j], c, 0, 0, 0);
+        } else {
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], c, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], c, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], c, 0, 0, 0);
+        }
+        for (int q = 0; q < 4; ++q) acc[i][j][q] = c[q];
+      }
+  }
+  block_reduce_store(acc, g.partial + (int64_t)blockIdx.x * 1024);
+}
+
+// ------------------------------------------------------------- gemm_f64
+__global__ void __launch_bounds__(256) gemm_f64_kernel(GemmArgs g) {
+  const double* __restrict__ G = (const double*)g.G;
+  const double* __restrict__ A = (const double*)g.A;
+  const double* __restrict__ B = (const double*)g.B;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int rr = lane & 15, kg = lane >> 4;
+  int ra[2], rb[2];
+  for (int h = 0; h < 2; ++h) {
+    ra[h] = min(g.a0 + 16 * h + rr, g.na - 1);
+    rb[h] = min(g.b0 + 16 * h + rr, g.nb - 1);
+  }
+  const double* pa[2] = {A + ra[0] * g.lda, A + ra[1] * g.lda};
+  const double* pb[2] = {B + rb[0] * g.ldb, B + rb[1] * g.ldb};
+  f64x4 acc[2][2];
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 2; ++j) acc[i][j] = (f64x4){0.0, 0.0, 0.0, 0.0};
+  const int64_t kbeg = (int64_t)blockIdx.x * g.kper;
+  int64_t kend = kbeg + g.kper;
+  if (kend > g.K) kend = g.K;
+  // one step = 16 k: lane k-group kg holds 4 consecutive doubles, MFMA s uses
+  // element s of every lane (a permutation of k shared by A and B)
+  for (int64_t k0 = kbeg + 16 * w; k0 < kend; k0 += 64) {
+    const int64_t ko = k0 + 4 * kg;
+    double va[2][4], vb[2][4], vg[4];
+    if (k0 + 16 <= kend) {
+      for (int h = 0; h < 2; ++h) {
+        const double2* s = (const double2*)(pa[h] + ko);
+        double2 x0 = s[0], x1 = s[1];
+        va[h][0] = x0.x; va[h][1] = x0.y; va[h][2] = x1.x; va[h][3] = x1.y;
+        if (!g.same) {
+          const double2* t = (const double2*)(pb[h] + ko);
+          double2 y0 = t[0], y1 = t[1];
+          vb[h][0] = y0.x; vb[h][1] = y0.y; vb[h][2] = y1.x; vb[h][3] = y1.y;
+        }
+      }
+      const double2* sg = (const double2*)(G + ko);
+      double2 g0 = sg[0], g1 = sg[1];
+      vg[0] = g0.x; vg[1] = g0.y; vg[2] = g1.x; vg[3] = g1.y;
+    } else {
+      for (int j = 0; j < 4; ++j) {
+        const int64_t kk = ko + j;
+        const bool ok = kk < kend;
+        vg[j] = ok ? G[kk] : 0.0;
+        for (int h = 0; h < 2; ++h) {
+          va[h][j] = ok ? pa[h][kk] : 0.0;
+          if (!g.same) vb[h][j] = ok ? pb[h][kk] : 0.0;
+        }
+      }
+    }
+    if (g.same)
+      for (int h = 0; h < 2; ++h)
+        for (int j = 0; j < 4; ++j) vb[h][j] = va[h][j];
+    for (int s = 0; s < 4; ++s)
+      for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(va[i][s], vb[j][s] * vg[s], acc[i][j], 0, 0, 0);
+  }
+  __shared__ double red[32 * 32];
+  for (int i = threadIdx.x; i < 1024; i += blockDim.x) red[i] = 0.0;
+  __syncthreads();
+  for (int ww = 0; ww < 4; ++ww) {
+    if (w == ww)
+      for (int bi = 0; bi < 2; ++bi)
+        for (int bj = 0; bj < 2; ++bj)
+          for (int j = 0; j < 4; ++j) {
+            const int r = 16 * bi + 4 * (lane >> 4) + j, c = 16 * bj + (lane & 15);
+            red[r * 32 + c] += acc[bi][bj][j];
+          }
+    __syncthreads();
+  }
+  double* out = g.partial + (int64_t)blockIdx.x * 1024;
+  for (int i = threadIdx.x; i < 1024; i += blockDim.x) out[i] = red[i];
+}
+
+// partial [nblk][32][32] -> C[(a0 + r) * ldc + b0 + c] for valid rows
+__global__ void gemm_finalize_kernel(const double* __restrict__ partial, int nblk, int a0, int b0, int na, int nb,
+                                     double inv_scale, double* __restrict__ C, int ldc) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 1024) return;
+  const int r = t >> 5, c = t & 31;
+  if (a0 + r >= na || b0 + c >= nb) return;
+  double s = 0.0;
+  for (int p = 0; p < nblk; ++p) s += partial[(int64_t)p * 1024 + t];
+  C[(int64_t)(a0 + r) * ldc + (b0 + c)] = s * inv_scale;
+}
+
+// mode: 0 bf16x3, 1 f16x3 (FP32 storage); 2 f64 (FP64 storage)
+hipError_t launch_gemm(int mode, const void* G, const void* A, int64_t lda, int na, const void* B, int64_t ldb, int nb,
+                       int same, int64_t K, float sa, float sb, int nblk, double* partial, double* C, hipStream_t st) {
+  const int step = (mode == 2) ? 64 : 128;  // K per workgroup iteration
+  int64_t kper = (K + nblk - 1) / nblk;
+  kper = (kper + step - 1) / step * step;
+  const int nblk_eff = (int)((K + kper - 1) / kper);
+  for (int a0 = 0; a0 < na; a0 += 32) {
+    for (int b0 = 0; b0 < nb; b0 += 32) {
+      GemmArgs g;
+      g.G = G; g.A = A; g.B = B; g.lda = lda; g.ldb = ldb; g.na = na; g.nb = nb; g.a0 = a0; g.b0 = b0;
+      g.same = (same && a0 == b0) ? 1 : 0;
+      g.pad = 0; g.K = K; g.kper = kper; g.sa = sa; g.sb = sb; g.partial = partial;
+      if (mode == 0)
+        hipLaunchKernelGGL(gemm_x3_kernel<false>, dim3(nblk_eff), dim3(256), 0, st, g);
+      else if (mode == 1)
+        hipLaunchKernelGGL(gemm_x3_kernel<true>, dim3(nblk_eff), dim3(256), 0, st, g);
+      else
+        hipLaunchKernelGGL(gemm_f64_kernel, dim3(nblk_eff), dim3(256), 0, st, g);
+      const double inv = 1.0 / ((double)sa * (double)sb);
+      hipLaunchKernelGGL(gemm_finalize_kernel, dim3(4), dim3(256), 0, st, partial, nblk_eff, a0, b0, na, nb, inv, C, nb);
+    }
+  }
+  return hipGetLastError();
+}
+
+}  // namespace cnpc

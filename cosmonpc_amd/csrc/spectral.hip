@@ -1,0 +1,229 @@
+// Half-spectrum kernels: interlace combine + window compensation, radial
+// binning (mode counts, sum |k|, binned cross spectra times polynomial
+// weights), and the fused shell filter + Ylm weight that feeds the shell
+// inverse FFTs.
+//
+// All kernels walk the N*N*(N/2+1) half spectrum of a real mesh.  A mode with
+// 0 < iz < N/2 stands for itself and its mirror -k, so sums over the full
+// grid (which the reference forms on complex128 c2c meshes) become weighted
+// sums with weight 2 for those modes and 1 on the iz = 0 and iz = N/2 planes.
+// Every quantity binned by the reference is either f(-k) = conj(f(k)) or
+// f(-k) = -conj(f(k)); the host picks the real or the imaginary total.
+#include "cnpc_common.h"
+
+namespace cnpc {
+
+// ------------------------------------------------------------- prepare
+// comp: 0 none, 1 divide by prod_d sinc(w_d/2)^p  (CompensateNGP..PCS,
+//       math_evaluator.py:206-248), 2 divide by prod_d sqrt(C_p(w_d))
+//       (Compensate*Shotnoise, math_evaluator.py:251-275).
+
+__device__ __forceinline__ double alias_c(int p, double s) {
+  if (p == 2) return 1.0 - 2.0 / 3 * s;
+  if (p == 3) return 1.0 - s + 2.0 / 15 * s * s;
+  if (p == 4) return 1.0 - 4.0 / 3.0 * s + 2.0 / 5.0 * s * s - 4.0 / 315.0 * s * s * s;
+  return 1.0;
+}
+
+__device__ __forceinline__ double sinc_half(double w) {  // np.sinc(0.5 w / pi)
+  if (w == 0.0) return 1.0;
+  double h = 0.5 * w;
+  return sin(h) / h;
+}
+
+__device__ __forceinline__ double comp_factor(int comp, int p, int nx, int ny, int nz, int n) {
+  if (comp == 0) return 1.0;
+  const double c = 2.0 * M_PI / n;
+  double w[3] = {nx * c, ny * c, nz * c};
+  double f = 1.0;
+  for (int d = 0; d < 3; ++d) {
+    if (comp == 1) {
+      double t = sinc_half(w[d]);
+      double tp = t;
+      for (int i = 1; i < p; ++i) tp *= t;
+      f /= tp;
+    } else if (p > 1) {
+      double s = sin(0.5 * w[d]);
+      f /= sqrt(alias_c(p, s * s));
+    }
+  }
+  return f;
+}
+
+// C_p(w) / sinc(w/2)^{2p} product (Compensate_bk_noise_*, math_evaluator.py:278-309)
+__device__ __forceinline__ double bk_noise_factor(int p, int nx, int ny, int nz, int n) {
+  if (p <= 1) return 1.0;
+  const double c = 2.0 * M_PI / n;
+  double w[3] = {nx * c, ny * c, nz * c};
+  double f = 1.0;
+  for (int d = 0; d < 3; ++d) {
+    double s = sin(0.5 * w[d]);
+    double t = sinc_half(w[d]);
+    double tp = t;
+    for (int i = 1; i < p; ++i) tp *= t;
+    f *= alias_c(p, s * s) / (tp * tp);
+  }
+  return f;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) prepare_kernel(PrepareArgs a, const typename cplx<T>::type* __restrict__ c1,
+                                                      const typename cplx<T>::type* __restrict__ c2,
+                                                      typename cplx<T>::type* __restrict__ out) {
+  const int n = a.n, nzh = n / 2 + 1;
+  const int64_t total = (int64_t)n * n * nzh;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (int64_t)gridDim.x * blockDim.x) {
+    const int iz = (int)(idx % nzh);
+    const int64_t r = idx / nzh;
+    const int iy = (int)(r % n), ix = (int)(r / n);
+    const int nx = wrap_freq(ix, n), ny = wrap_freq(iy, n), nz = wrap_freq(iz, n);
+    double re = (double)c1[idx].x, im = (double)c1[idx].y;
+    if (a.interlaced) {
+      // c = c1/2 + c2/2 * exp(+i/2 sum_d k_d H)   (mesh_generator.py:212-218)
+      const double kH = (double)nx * a.kf * a.H + (double)ny * a.kf * a.H + (double)nz * a.kf * a.H;
+      double sn, cs;
+      sincos(0.5 * kH, &sn, &cs);
+      const double r2 = (double)c2[idx].x, i2 = (double)c2[idx].y;
+      re = re * 0.5 + 0.5 * (r2 * cs - i2 * sn);
+      im = im * 0.5 + 0.5 * (r2 * sn + i2 * cs);
+    }
+    re *= a.scale;
+    im *= a.scale;
+    const bool k0 = (nx == 0 && ny == 0 && nz == 0);
+    if (k0) re -= a.sub0;
+    const double f = comp_factor(a.comp, a.p, nx, ny, nz, n);
+    re *= f;
+    im *= f;
+    if (k0 && a.zero0) { re = 0.0; im = 0.0; }
+    typename cplx<T>::type o;
+    o.x = (T)re;
+    o.y = (T)im;
+    out[idx] = o;
+  }
+}
+
+hipError_t launch_prepare(int prec, const PrepareArgs& a, const void* c1, const void* c2, void* out, hipStream_t st) {
+  const int64_t total = (int64_t)a.n * a.n * (a.n / 2 + 1);
+  dim3 blk(256), grd(grid_for(total, 256, 1 << 16));
+  if (prec == 0)
+    hipLaunchKernelGGL(prepare_kernel<double>, grd, blk, 0, st, a, (const double2*)c1, (const double2*)c2, (double2*)out);
+  else
+    hipLaunchKernelGGL(prepare_kernel<float>, grd, blk, 0, st, a, (const float2*)c1, (const float2*)c2, (float2*)out);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------- binning
+// out layout (FP64, caller-zeroed): [count(nb), ksum(nb), re(npoly*nb), im(npoly*nb)]
+
+
+template <typename T>
+__global__ void __launch_bounds__(256) bin_kernel(BinArgs a, BinEdges E, PolySet P,
+                                                  const typename cplx<T>::type* __restrict__ A,
+                                                  const typename cplx<T>::type* __restrict__ B, double* __restrict__ out) {
+  extern __shared__ double lds[];  // (2 + 2*npoly) * nbins
+  const int nb = E.nbins;
+  const int nacc = (2 + 2 * a.npoly) * nb;
+  for (int i = threadIdx.x; i < nacc; i += blockDim.x) lds[i] = 0.0;
+  __syncthreads();
+  const int n = a.n, nzh = n / 2 + 1;
+  const int64_t total = (int64_t)n * n * nzh;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (int64_t)gridDim.x * blockDim.x) {
+    const int iz = (int)(idx % nzh);
+    const int64_t r = idx / nzh;
+    const int iy = (int)(r % n), ix = (int)(r / n);
+    const int nx = wrap_freq(ix, n), ny = wrap_freq(iy, n), nz = wrap_freq(iz, n);
+    double kx, ky, kz;
+    const double k = knorm_of(nx, ny, nz, a.kf, &kx, &ky, &kz);
+    const int b = bin_of(E, k);
+    if (b < 0) continue;
+    const double wmode = (iz == 0 || iz == n / 2) ? 1.0 : 2.0;
+    atomicAdd(&lds[b], wmode);
+    atomicAdd(&lds[nb + b], wmode * k);
+    if (a.npoly == 0) continue;
+    const double ar = (double)A[idx].x, ai = (double)A[idx].y;
+    const double br = (double)B[idx].x, bi = (double)B[idx].y;
+    double fr, fi;
+    if (a.conj_b) { fr = ar * br + ai * bi; fi = ai * br - ar * bi; }
+    else { fr = ar * br - ai * bi; fi = ar * bi + ai * br; }
+    fr *= a.ab_scale;
+    fi *= a.ab_scale;
+    if (a.noise_p > 0 || a.noise_amp != 0.0) fr -= a.noise_amp * bk_noise_factor(a.noise_p, nx, ny, nz, n);
+    const bool zv = (k == 0.0);
+    const double inv = zv ? 0.0 : 1.0 / k;
+    for (int q = 0; q < a.npoly; ++q) {
+      const double2 w = poly_eval(P.s[q], kx * inv, ky * inv, kz * inv, zv);
+      atomicAdd(&lds[(2 + q) * nb + b], wmode * (fr * w.x - fi * w.y));
+      atomicAdd(&lds[(2 + a.npoly + q) * nb + b], wmode * (fr * w.y + fi * w.x));
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < nacc; i += blockDim.x) {
+    const double v = lds[i];
+    if (v != 0.0) unsafeAtomicAdd(&out[i], v);
+  }
+}
+
+hipError_t launch_bin(int prec, const BinArgs& a, const BinEdges& E, const PolySet& P, const void* A, const void* B,
+                      double* out, hipStream_t st) {
+  const int64_t total = (int64_t)a.n * a.n * (a.n / 2 + 1);
+  const size_t lds = (size_t)(2 + 2 * a.npoly) * E.nbins * sizeof(double);
+  dim3 blk(256), grd(grid_for(total, 256, 2048));
+  if (prec == 0)
+    hipLaunchKernelGGL(bin_kernel<double>, grd, blk, lds, st, a, E, P, (const double2*)A, (const double2*)B, out);
+  else
+    hipLaunchKernelGGL(bin_kernel<float>, grd, blk, lds, st, a, E, P, (const float2*)A, (const float2*)B, out);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------- shell filter
+// out[s] = w(khat) * delta(k) * [bin(k) == bin0 + s], s = 0..nshell-1, each a
+// full half spectrum (zeros outside the shell) ready for a c2r transform:
+// the filtered-field product of clustering_estimator.py:934-970 with the
+// Y_lm weight of :898-902 folded in.
+
+template <typename T>
+__global__ void __launch_bounds__(256) shell_filter_kernel(ShellArgs a, BinEdges E, PolySpec W,
+                                                           const typename cplx<T>::type* __restrict__ D,
+                                                           typename cplx<T>::type* __restrict__ out) {
+  const int n = a.n, nzh = n / 2 + 1;
+  const int64_t total = (int64_t)n * n * nzh;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (int64_t)gridDim.x * blockDim.x) {
+    const int iz = (int)(idx % nzh);
+    const int64_t r = idx / nzh;
+    const int iy = (int)(r % n), ix = (int)(r / n);
+    const int nx = wrap_freq(ix, n), ny = wrap_freq(iy, n), nz = wrap_freq(iz, n);
+    double kx, ky, kz;
+    const double k = knorm_of(nx, ny, nz, a.kf, &kx, &ky, &kz);
+    const int b = bin_of(E, k) - a.bin0;
+    typename cplx<T>::type v;
+    v.x = (T)0;
+    v.y = (T)0;
+    if (b >= 0 && b < a.nshell) {
+      const bool zv = (k == 0.0);
+      const double inv = zv ? 0.0 : 1.0 / k;
+      const double2 w = poly_eval(W, kx * inv, ky * inv, kz * inv, zv);
+      const double dr = (double)D[idx].x, di = (double)D[idx].y;
+      v.x = (T)(dr * w.x - di * w.y);
+      v.y = (T)(dr * w.y + di * w.x);
+    }
+    for (int s = 0; s < a.nshell; ++s) {
+      typename cplx<T>::type o;
+      o.x = (s == b) ? v.x : (T)0;
+      o.y = (s == b) ? v.y : (T)0;
+      out[s * a.stride + idx] = o;
+    }
+  }
+}
+
+hipError_t launch_shell_filter(int prec, const ShellArgs& a, const BinEdges& E, const PolySpec& W, const void* D,
+                               void* out, hipStream_t st) {
+  const int64_t total = (int64_t)a.n * a.n * (a.n / 2 + 1);
+  dim3 blk(256), grd(grid_for(total, 256, 1 << 16));
+  if (prec == 0)
+    hipLaunchKernelGGL(shell_filter_kernel<double>, grd, blk, 0, st, a, E, W, (const double2*)D, (double2*)out);
+  else
+    hipLaunchKernelGGL(shell_filter_kernel<float>, grd, blk, 0, st, a, E, W, (const float2*)D, (float2*)out);
+  return hipGetLastError();
+}
+
+}  // namespace cnpc

@@ -1,0 +1,192 @@
+"""Device-side handle: one cnpc context per (device, stream), torch tensors
+for every buffer.  Thin typed wrappers over the C ABI (include/cnpc.h); every
+call validates shapes and dtypes on the host before anything is launched.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import check, cnpc_poly
+
+WINDOW = {"ngp": 1, "cic": 2, "tsc": 3, "pcs": 4}
+PREC = {"fp64": 0, "fp32": 1}
+REAL_DTYPE = {0: torch.float64, 1: torch.float32}
+CPLX_DTYPE = {0: torch.complex128, 1: torch.complex64}
+GEMM_MODE = {"bf16x3": 0, "fp16x3": 1, "gemm_fp64": 2}
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else ctypes.c_void_p(0)
+
+
+class GpuContext:
+    """Owns a ``cnpc_ctx`` bound to torch's current stream on ``device``."""
+
+    def __init__(self, device=None):
+        if not torch.cuda.is_available():
+            raise _lib.CnpcError("no GPU visible to torch: the MI355X path has no CPU fallback")
+        self.lib = _lib.load()
+        self.device = torch.device("cuda", torch.cuda.current_device() if device is None else device)
+        torch.cuda.set_device(self.device)
+        h = ctypes.c_void_p()
+        check(self.lib.cnpc_ctx_create(ctypes.byref(h), self.device.index), "cnpc_ctx_create")
+        self.h = h
+        self.bind_stream()
+
+    def bind_stream(self, stream=None):
+        s = stream if stream is not None else torch.cuda.current_stream(self.device)
+        check(self.lib.cnpc_ctx_set_stream(self.h, ctypes.c_void_p(s.cuda_stream)), "cnpc_ctx_set_stream")
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            self.lib.cnpc_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---------------------------------------------------------------- buffers
+    def mesh(self, n, prec, batch=None):
+        shape = (n, n, n) if batch is None else (batch, n, n, n)
+        return torch.empty(shape, dtype=REAL_DTYPE[prec], device=self.device)
+
+    def spectrum(self, n, prec, batch=None):
+        shape = (n, n, n // 2 + 1) if batch is None else (batch, n, n, n // 2 + 1)
+        return torch.empty(shape, dtype=CPLX_DTYPE[prec], device=self.device)
+
+    @staticmethod
+    def _check_mesh(t, n, prec, batch=None):
+        shape = (n, n, n) if batch is None else (batch, n, n, n)
+        if t.dtype != REAL_DTYPE[prec] or tuple(t.shape) != shape or not t.is_contiguous() or not t.is_cuda:
+            raise ValueError(f"expected contiguous device mesh {shape} {REAL_DTYPE[prec]}, got {tuple(t.shape)} {t.dtype}")
+
+    @staticmethod
+    def _check_spec(t, n, prec, batch=None):
+        shape = (n, n, n // 2 + 1) if batch is None else (batch, n, n, n // 2 + 1)
+        if t.dtype != CPLX_DTYPE[prec] or tuple(t.shape) != shape or not t.is_contiguous() or not t.is_cuda:
+            raise ValueError(f"expected contiguous device spectrum {shape} {CPLX_DTYPE[prec]}, got {tuple(t.shape)} {t.dtype}")
+
+    # ---------------------------------------------------------------- kernels
+    def paint(self, pos, w, n, boxsize, sampler, interlaced, prec, mesh0, mesh1=None, boxcenter=None):
+        if pos.dtype != torch.float64 or w.dtype != torch.float64:
+            raise ValueError("positions and weights must be float64")
+        if pos.dim() != 2 or pos.shape[1] != 3 or w.shape != (pos.shape[0],):
+            raise ValueError("positions must be (Np, 3) and weights (Np,)")
+        if not (pos.is_contiguous() and w.is_contiguous() and pos.is_cuda and w.is_cuda):
+            raise ValueError("positions / weights must be contiguous device tensors")
+        self._check_mesh(mesh0, n, prec)
+        if interlaced:
+            self._check_mesh(mesh1, n, prec)
+        center = None
+        if boxcenter is not None:
+            center = (ctypes.c_double * 3)(*[float(c) for c in boxcenter])
+        check(self.lib.cnpc_paint(self.h, _ptr(pos), _ptr(w), pos.shape[0], n, float(boxsize), WINDOW[sampler],
+                                  int(bool(interlaced)), center, prec, _ptr(mesh0), _ptr(mesh1)), "cnpc_paint")
+
+    def r2c(self, mesh, out, n, prec, batch=None):
+        self._check_mesh(mesh, n, prec, batch)
+        self._check_spec(out, n, prec, batch)
+        check(self.lib.cnpc_fft_r2c(self.h, n, batch or 1, prec, _ptr(mesh), _ptr(out)), "cnpc_fft_r2c")
+
+    def c2r(self, spec, out, n, prec, batch=None):
+        """Inverse FFT; ``spec`` is overwritten."""
+        self._check_spec(spec, n, prec, batch)
+        self._check_mesh(out, n, prec, batch)
+        check(self.lib.cnpc_fft_c2r(self.h, n, batch or 1, prec, _ptr(spec), _ptr(out)), "cnpc_fft_c2r")
+
+    def prepare(self, c1, c2, out, n, boxsize, prec, interlaced, comp, sampler, scale, sub0=0.0, zero_k0=False):
+        self._check_spec(c1, n, prec)
+        self._check_spec(out, n, prec)
+        if interlaced:
+            self._check_spec(c2, n, prec)
+        check(self.lib.cnpc_prepare_spectrum(self.h, n, float(boxsize), prec, _ptr(c1), _ptr(c2 if interlaced else None),
+                                             int(bool(interlaced)), int(comp), WINDOW[sampler], float(scale),
+                                             float(sub0), int(bool(zero_k0)), _ptr(out)), "cnpc_prepare_spectrum")
+
+    def bin(self, a, b, n, boxsize, prec, edges, polys=(), ab_scale=1.0, noise_window=0, noise_amp=0.0):
+        """Returns host arrays (count, ksum, re[npoly, nb], im[npoly, nb])."""
+        edges = np.ascontiguousarray(edges, dtype=np.float64)
+        nb = len(edges) - 1
+        npoly = len(polys)
+        if npoly:
+            self._check_spec(a, n, prec)
+            self._check_spec(b, n, prec)
+        out = torch.empty((2 + 2 * npoly) * nb, dtype=torch.float64, device=self.device)
+        parr = (cnpc_poly * max(npoly, 1))(*polys) if npoly else None
+        check(self.lib.cnpc_bin_spectrum(self.h, n, float(boxsize), prec, _ptr(a if npoly else None),
+                                         _ptr(b if npoly else None), edges.ctypes.data_as(ctypes.c_void_p), nb, npoly,
+                                         ctypes.cast(parr, ctypes.c_void_p) if parr is not None else None,
+                                         float(ab_scale), int(noise_window), float(noise_amp), _ptr(out)),
+              "cnpc_bin_spectrum")
+        h = out.cpu().numpy()
+        return h[:nb].copy(), h[nb:2 * nb].copy(), h[2 * nb:(2 + npoly) * nb].reshape(npoly, nb), \
+            h[(2 + npoly) * nb:].reshape(npoly, nb)
+
+    def shell_filter(self, delta, n, boxsize, prec, edges, bin0, nshell, poly, out):
+        edges = np.ascontiguousarray(edges, dtype=np.float64)
+        self._check_spec(delta, n, prec)
+        if out.dim() != 4 or out.shape[0] < nshell:
+            raise ValueError("shell output must hold nshell spectra")
+        self._check_spec(out[:nshell], n, prec, nshell)
+        check(self.lib.cnpc_shell_filter(self.h, n, float(boxsize), prec, _ptr(delta),
+                                         edges.ctypes.data_as(ctypes.c_void_p), len(edges) - 1, int(bin0), int(nshell),
+                                         ctypes.byref(poly), _ptr(out)), "cnpc_shell_filter")
+
+    def contract_pairs(self, prec, G, A, B, ia, ib):
+        """sum_x G A[ia] B[ib] per pair, host float64 array."""
+        ncell = G.numel()
+        if A.dim() != 2 or B.dim() != 2 or A.shape[1] != ncell or B.shape[1] != ncell:
+            raise ValueError("A / B must be (nshell, ncell) stacks matching G")
+        ia = np.ascontiguousarray(ia, dtype=np.int32)
+        ib = np.ascontiguousarray(ib, dtype=np.int32)
+        if len(ia) != len(ib) or (len(ia) and (ia.max() >= A.shape[0] or ib.max() >= B.shape[0])):
+            raise ValueError("pair index out of range")
+        out = np.zeros(len(ia), dtype=np.float64)
+        check(self.lib.cnpc_contract_pairs(self.h, prec, ncell, _ptr(G), _ptr(A), _ptr(B), len(ia),
+                                           ia.ctypes.data_as(ctypes.c_void_p), ib.ctypes.data_as(ctypes.c_void_p),
+                                           out.ctypes.data_as(ctypes.c_void_p)), "cnpc_contract_pairs")
+        return out
+
+    def contract_gemm(self, mode, G, A, B=None, sa=1.0, sb=1.0, nblk=0):
+        """C[a, b] = sum_x A[a] G B[b] (device float64 tensor)."""
+        same = B is None
+        if same:
+            B = A
+        ncell = G.numel()
+        if A.dim() != 2 or B.dim() != 2 or A.shape[1] != ncell or B.shape[1] != ncell:
+            raise ValueError("A / B must be (nshell, ncell) stacks matching G")
+        want = torch.float64 if GEMM_MODE[mode] == 2 else torch.float32
+        for t in (G, A, B):
+            if t.dtype != want or not t.is_contiguous():
+                raise ValueError(f"{mode} needs contiguous {want} operands")
+        C = torch.empty((A.shape[0], B.shape[0]), dtype=torch.float64, device=self.device)
+        check(self.lib.cnpc_contract_gemm(self.h, GEMM_MODE[mode], ncell, _ptr(G), _ptr(A), A.shape[0], _ptr(B),
+                                          B.shape[0], int(same), float(sa), float(sb), int(nblk), _ptr(C)),
+              "cnpc_contract_gemm")
+        return C
+
+    def absmax(self, x):
+        out = ctypes.c_double()
+        check(self.lib.cnpc_absmax_f32(self.h, _ptr(x), x.numel(), ctypes.byref(out)), "cnpc_absmax_f32")
+        return out.value
+
+    def sync(self):
+        check(self.lib.cnpc_sync(self.h), "cnpc_sync")
+
+
+_CTX = {}
+
+
+def get_context(device=None):
+    """Process-wide context per device (created lazily)."""
+    idx = torch.cuda.current_device() if device is None else device
+    if idx not in _CTX:
+        _CTX[idx] = GpuContext(idx)
+    ctx = _CTX[idx]
+    ctx.bind_stream()
+    return ctx

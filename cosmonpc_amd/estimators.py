@@ -1,0 +1,400 @@
+"""Box-geometry estimators on the GPU.
+
+``calculate_power_spectrum_box`` mirrors ``clustering_estimator.py:566-746``
+and ``calculate_bk_sugi_box`` mirrors ``clustering_estimator.py:749-1555``:
+same arguments, same result keys, same normalisations.  The numeric work runs
+in libcnpc_hip.so; the host keeps only O(k_bins^2) bookkeeping (symmetry
+fill, magnetic-config sums, analytic S3 and normalisation).
+
+GPU-only options (read from ``stat_attrs``; the reference ignores them):
+  precision   : "fp64" (default) | "fp32"  -- carried by the painted fields
+  contraction : "gemm_fp64" (fp64 default) | "pairwise" | "bf16x3"
+                (fp32 default) | "fp16x3"
+"""
+import logging
+import math
+import time
+
+import numpy as np
+import torch
+
+from . import angular as A
+from .device import PREC, get_context
+
+LAST_TIMINGS = {}
+
+
+def _sync_time(ctx):
+    torch.cuda.synchronize(ctx.device)
+    return time.perf_counter()
+
+
+# ----------------------------------------------------------------------- pk
+def calculate_power_spectrum_box(rfield_a, rfield_b, correlation_mode, stat_attrs, comm=None, **kwargs):
+    """P_ell(k) of a periodic box (``clustering_estimator.py:566-746``)."""
+    ctx = get_context()
+    poles = stat_attrs["poles"]
+    boxsize, nmesh = np.array(stat_attrs["boxsize"]), np.array(stat_attrs["nmesh"])
+    k_min, k_max, k_bins = stat_attrs["k_min"], stat_attrs["k_max"], stat_attrs["k_bins"]
+    sampler, interlaced = stat_attrs["sampler"], stat_attrs["interlaced"]
+    rsd = np.array(stat_attrs["rsd"], dtype=float)
+    NZ_a = stat_attrs["NZ_a"]
+    P_shot = 1.0 / NZ_a if correlation_mode == "auto" else 0.0
+    NZ_b = stat_attrs["NZ_b"] if correlation_mode == "cross" else None
+    k_edge = np.linspace(k_min, k_max, k_bins + 1)
+    n, L, prec = rfield_a.n, rfield_a.boxsize, rfield_a.prec
+
+    comp = 1 if interlaced else 2  # sinc^p when interlaced, else sqrt alias sums
+    # the zero mode of field a is cleared (clustering_estimator.py:634-635)
+    da = rfield_a.delta_k(ctx, comp, zero_k0=True)
+    db = rfield_b.delta_k(ctx, comp) if correlation_mode == "cross" else da
+
+    even = [p for p in poles if not p & 1]
+    polys = [A.make_poly(0, [1.0], A.PART_FULL, 1.0, 1.0) if p == 0 else A.legendre_poly(p, rsd) for p in even]
+    count, ksum, re, _ = ctx.bin(da, db, n, L, prec, k_edge, polys)
+    k_num = count
+    with np.errstate(invalid="ignore", divide="ignore"):
+        results = {"k_eff": ksum / k_num, "k_num": k_num}
+    for pole in poles:
+        if pole & 1:
+            res = np.zeros(k_bins).astype("float128")
+        else:
+            with np.errstate(invalid="ignore", divide="ignore"):
+                res = re[even.index(pole)] / k_num
+            if correlation_mode == "auto":
+                res *= boxsize.prod() ** (-1) * (nmesh.prod() ** 2) / NZ_a ** 2
+            else:
+                res *= boxsize.prod() ** (-1) * nmesh.prod() ** 2 / (NZ_a * NZ_b)
+            if pole == 0:
+                res -= P_shot
+            else:
+                res *= 2 * pole + 1
+        results.update({f"P{pole}": res, "P_shot": P_shot})
+    return results
+
+
+# ---------------------------------------------------------------- bispectrum
+class ShellBuilder:
+    """Builds real shell fields  R = c2r(w(khat) * delta * 1[k in bin])  for a
+    run of bins, batching the shell filter and the inverse FFTs."""
+
+    def __init__(self, ctx, n, boxsize, prec, edges, batch=4):
+        self.ctx, self.n, self.L, self.prec = ctx, n, boxsize, prec
+        self.edges = np.asarray(edges, dtype=np.float64)
+        self.batch = batch
+        self.work = ctx.spectrum(n, prec, batch)
+
+    def build(self, delta, poly, bins, out):
+        """out[r] = shell of bins[r] (bins must be consecutive)."""
+        n = self.n
+        b0 = bins[0]
+        nb = len(bins)
+        assert list(bins) == list(range(b0, b0 + nb))
+        done = 0
+        while done < nb:
+            c = min(self.batch, nb - done)
+            self.ctx.shell_filter(delta, n, self.L, self.prec, self.edges, b0 + done, c, poly, self.work)
+            self.ctx.c2r(self.work[:c], out[done:done + c].view(c, n, n, n), n, self.prec, c)
+            done += c
+
+
+def _gemm_mode(contraction, prec):
+    if contraction in ("bf16x3", "fp16x3"):
+        if prec != PREC["fp32"]:
+            raise ValueError(f"contraction '{contraction}' needs precision 'fp32'")
+    elif contraction == "gemm_fp64":
+        if prec != PREC["fp64"]:
+            raise ValueError("contraction 'gemm_fp64' needs precision 'fp64'")
+    elif contraction != "pairwise":
+        raise ValueError("contraction must be one of 'pairwise', 'gemm_fp64', 'bf16x3', 'fp16x3'")
+    return contraction
+
+
+def _pow2_scale(amax, target=2.0 ** 14):
+    if not amax > 0 or not math.isfinite(amax):
+        return 1.0
+    return 2.0 ** math.floor(math.log2(target / amax))
+
+
+def _contract(ctx, mode, prec, G, R1, R2, pairs):
+    """Real contraction sum_x G R1[a] R2[b]: full matrix (GEMM modes) or the
+    listed (a, b) pairs (pairwise).  R2 None means R2 is R1."""
+    if mode == "pairwise":
+        ia = np.array([p[0] for p in pairs], dtype=np.int32)
+        ib = np.array([p[1] for p in pairs], dtype=np.int32)
+        vals = ctx.contract_pairs(prec, G, R1, R1 if R2 is None else R2, ia, ib)
+        nr = R1.shape[0]
+        nc = R1.shape[0] if R2 is None else R2.shape[0]
+        C = np.zeros((nr, nc))
+        C[ia, ib] = vals
+        return C
+    if mode == "fp16x3":
+        sa = _pow2_scale(ctx.absmax(R1))
+        sg = ctx.absmax(G)
+        sb = _pow2_scale(sg * ctx.absmax(R1 if R2 is None else R2))
+        return ctx.contract_gemm(mode, G, R1, R2, sa, sb).cpu().numpy()
+    return ctx.contract_gemm(mode, G, R1, R2).cpu().numpy()
+
+
+def _fit_block(k_bins, bytes_per_bin, ctx, reserve=0.25):
+    """Largest block of bins whose shells fit in the free HBM (keeping a
+    fraction in reserve for FFT work areas and spectra)."""
+    free, total = torch.cuda.mem_get_info(ctx.device)
+    budget = free - reserve * total
+    return int(max(1, min(k_bins, budget // max(bytes_per_bin, 1))))
+
+
+def _bk_signal(ctx, stat_attrs, ca, cb, G, kinfo, mcfg, tjv, ell_1, ell_2, tracer_type, correlation_mode, mode,
+               block_size, dvm):
+    """total_res of the signal loop (``clustering_estimator.py:867-1061``)."""
+    n, L, prec, k_edge, k_bins = kinfo
+    N3 = n ** 3
+    builder = ShellBuilder(ctx, n, L, prec, k_edge)
+    same_tracer = correlation_mode == "auto" or tracer_type == "aab"
+    same_ell = ell_1 == ell_2
+    if dvm == "full":
+        total = np.zeros((k_bins, k_bins), dtype=np.complex128)
+    else:
+        total = np.zeros(k_bins, dtype=np.complex128)
+    for ss, (m1, m2, M) in enumerate(mcfg):
+        sym_eq = same_tracer and same_ell and m1 == m2
+        sym_cj = same_tracer and same_ell and m1 == -m2
+        use_sym = sym_eq or sym_cj
+        comps1 = A.shell_components(ell_1, m1)
+        src2 = ca if tracer_type in ("aaa", "aab") else cb
+        if use_sym:
+            sign = (-1) ** (m1 + ell_1)
+            # F' = F (equal) or (-1)^(m+l) conj(F): same real shells, new phases
+            comps2 = [(p, ph if sym_eq else sign * np.conj(ph)) for p, ph in comps1]
+        else:
+            comps2 = A.shell_components(ell_2, m2)
+        nc1, nc2 = len(comps1), len(comps2)
+        # resident shells per block: nc1 * bs (+ nc2 * bs off the diagonal)
+        shell_bytes = N3 * (8 if prec == 0 else 4)
+        bs_fit = _fit_block(k_bins, shell_bytes * (nc1 + nc2), ctx)
+        if use_sym and _fit_block(k_bins, shell_bytes * nc1, ctx) == k_bins:
+            bs_fit = k_bins  # one diagonal block: leg 2 reuses leg 1's shells
+        if dvm == "full":
+            sub = np.zeros((k_bins, k_bins), dtype=np.complex128)
+            bs = min(bs_fit, k_bins if block_size == "full" else int(block_size))
+        else:
+            sub = np.zeros(k_bins, dtype=np.complex128)
+            bs = bs_fit
+        for bi in range(0, k_bins, bs):
+            ib = list(range(bi, min(k_bins, bi + bs)))
+            R1 = ctx.mesh(n, prec, nc1 * len(ib)).view(nc1 * len(ib), N3)
+            for c, (poly, _) in enumerate(comps1):
+                builder.build(ca, poly, ib, R1[c * len(ib):(c + 1) * len(ib)])
+            jstarts = [bi] if dvm == "diagonal" else list(range(bi if use_sym else 0, k_bins, bs))
+            for bj in jstarts:
+                jb = list(range(bj, min(k_bins, bj + bs)))
+                if bj == bi and use_sym:
+                    R2 = None
+                else:
+                    R2 = ctx.mesh(n, prec, nc2 * len(jb)).view(nc2 * len(jb), N3)
+                    for c, (poly, _) in enumerate(comps2):
+                        if use_sym:
+                            builder.build(ca, comps1[c][0], jb, R2[c * len(jb):(c + 1) * len(jb)])
+                        else:
+                            builder.build(src2, poly, jb, R2[c * len(jb):(c + 1) * len(jb)])
+                # pairs this block contributes (reference loop order)
+                want = []
+                for i in ib:
+                    if dvm == "diagonal":
+                        js = [i]
+                    elif use_sym and bj == bi:
+                        js = [j for j in jb if j >= i]
+                    else:
+                        js = jb
+                    for j in js:
+                        want.append((i, j))
+                rows2 = len(ib) if R2 is None else len(jb)
+                pairs = [(c1 * len(ib) + (i - bi), c2 * rows2 + (j - bj)) for (i, j) in want
+                         for c1 in range(nc1) for c2 in range(nc2)]
+                C = _contract(ctx, mode, prec, G, R1, R2, pairs)
+                for (i, j) in want:
+                    s = 0j
+                    for c1 in range(nc1):
+                        for c2 in range(nc2):
+                            s += comps1[c1][1] * comps2[c2][1] * C[c1 * len(ib) + (i - bi), c2 * rows2 + (j - bj)]
+                    if dvm == "full":
+                        sub[i, j] = tjv[ss] * s
+                    else:
+                        sub[i] = tjv[ss] * s
+                del R2
+            del R1
+        if dvm == "full" and use_sym:
+            iu = np.triu_indices(k_bins, 1)
+            sub[iu[1], iu[0]] = sub[iu] if sym_eq else np.conj(sub[iu])
+        total += sub if (m1, m2, M) == (0, 0, 0) else 2 * np.real(sub)
+    return total
+
+
+def calculate_bk_sugi_box(rfield_a, rfield_b, rfield_c, correlation_mode, stat_attrs, comm=None, **kwargs):
+    """Sugiyama bispectrum multipole of a periodic box
+    (``clustering_estimator.py:749-1555``)."""
+    ctx = get_context()
+    dvm = stat_attrs.get("data_vector_mode", "diagonal")
+    sn_mode = stat_attrs.get("shotnoise_mode", "ana")
+    all_eff = kwargs.get("shotnoise_ana_all_eff", stat_attrs.get("shotnoise_ana_all_eff", True))
+    block_size = stat_attrs.get("block_size", 1 if dvm == "diagonal" else "full")
+    ell_1, ell_2, L = stat_attrs["angu_config"]
+    boxsize = np.array(stat_attrs["boxsize"])
+    nmesh = np.array(stat_attrs["nmesh"])
+    k_min, k_max, k_bins = stat_attrs["k_min"], stat_attrs["k_max"], stat_attrs["k_bins"]
+    sampler = stat_attrs["sampler"]
+    tracer_type = stat_attrs["tracer_type"]
+    vol_per_cell = boxsize.prod() / nmesh.prod()
+    n, Lbox, prec = rfield_a.n, rfield_a.boxsize, rfield_a.prec
+    contraction = _gemm_mode(stat_attrs.get("contraction", "gemm_fp64" if prec == 0 else "bf16x3"), prec)
+
+    N_ells = (2 * ell_1 + 1) * (2 * ell_2 + 1) * (2 * L + 1)
+    H_ells = A.wigner_3j(ell_1, ell_2, L, 0, 0, 0)
+    mcfg, tjv = A.get_magnetic_configs_box(ell_1, ell_2, L)
+
+    NZ_a, N_gal_a = stat_attrs["NZ_a"], stat_attrs["N_gal_a"]
+    if correlation_mode == "cross":
+        NZ_b, N_gal_b = stat_attrs["NZ_b"], stat_attrs["N_gal_b"]
+        if tracer_type == "abc":
+            NZ_c, N_gal_c = stat_attrs["NZ_c"], stat_attrs["N_gal_c"]
+    k_edge = np.linspace(k_min, k_max, k_bins + 1)
+
+    if dvm == "full":
+        if block_size == "full":
+            pass
+        elif not (isinstance(block_size, int) and 1 <= block_size <= k_bins):
+            raise ValueError("For full mode, block_size must be 1, 'full', or an integer in [1, k_bins].")
+    elif dvm != "diagonal":
+        raise ValueError("data_vector_mode must be either 'diagonal' or 'full'")
+
+    t0 = _sync_time(ctx)
+    # delta_k = sinc^p-compensated r2c(n/dV - NZ)   (clustering_estimator.py:808-816)
+    ca = rfield_a.delta_k(ctx, 1, scale=1.0 / vol_per_cell, sub0=NZ_a)
+    cb = rfield_b.delta_k(ctx, 1, scale=1.0 / vol_per_cell, sub0=NZ_b) if correlation_mode == "cross" else None
+    cc = (rfield_c.delta_k(ctx, 1, scale=1.0 / vol_per_cell, sub0=NZ_c)
+          if (correlation_mode == "cross" and tracer_type == "abc") else None)
+    # G_00 = Re c2r(delta_k) of the third leg (:844-851)
+    gsrc = ca if correlation_mode == "auto" else (cc if tracer_type == "abc" else cb)
+    tmp = gsrc.clone()
+    G = ctx.mesh(n, prec)
+    ctx.c2r(tmp, G, n, prec)
+    del tmp
+    count, ksum, _, _ = ctx.bin(None, None, n, Lbox, prec, k_edge)
+    k_num = count
+    with np.errstate(invalid="ignore", divide="ignore"):
+        k_eff = ksum / k_num
+    t1 = _sync_time(ctx)
+    total_res = _bk_signal(ctx, stat_attrs, ca, cb, G.view(-1), (n, Lbox, prec, k_edge, k_bins), mcfg, tjv,
+                           ell_1, ell_2, tracer_type, correlation_mode, contraction, block_size, dvm)
+    t2 = _sync_time(ctx)
+    LAST_TIMINGS.update({"prepare_s": t1 - t0, "signal_s": t2 - t1, "contraction": contraction})
+    del G
+
+    # ---------------------------------------------------------- shot noise
+    k_center = 0.5 * (k_edge[1:] + k_edge[:-1])
+    k3_edge = np.linspace(2 * k_min, 2 * k_max, k_bins * 2 + 1)[:-1]
+    k3_center = 0.5 * (k3_edge[1:] + k3_edge[:-1])
+    k3_bins = len(k3_center)
+    k_num_k3, ksum3, _, _ = ctx.bin(None, None, n, Lbox, prec, k3_edge)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        k3_eff = ksum3 / k_num_k3
+    p = {"ngp": 1, "cic": 2, "tsc": 3, "pcs": 4}[sampler]
+
+    SN3_fft = None
+    if tracer_type == "abc":
+        SN0 = SN1 = SN2 = SN3 = 0.0
+    else:
+        SN0 = N_gal_a if (correlation_mode == "auto" and [ell_1, ell_2, L] == [0, 0, 0]) else 0.0
+        # P = V^2 a conj(b) - SN0 C_p/sinc^2p, times y_L0 if L != 0, on k3 bins (:1405-1434)
+        ypoly = A.ylm_poly(L, 0, True) if L != 0 else A.make_poly(0, [1.0], A.PART_FULL, 1.0, 1.0)
+        bsrc = ca if correlation_mode == "auto" else cb
+        _, _, re3, _ = ctx.bin(ca, bsrc, n, Lbox, prec, k3_edge, [ypoly], ab_scale=boxsize.prod() ** 2,
+                               noise_window=p, noise_amp=SN0 if correlation_mode == "auto" else 0.0)
+        total_sum = re3[0].astype(np.complex128)
+        SN1, SN2 = 0.0, 0.0
+        if ell_1 == L and ell_2 == 0:
+            with np.errstate(invalid="ignore", divide="ignore"):
+                if correlation_mode == "auto":
+                    SN1 = (total_sum / k_num_k3)[: len(k_center)] * (2 * L + 1)
+                    if ell_1 == 0:
+                        SN2 = SN1.copy()
+                elif tracer_type == "abb":
+                    SN1 = (total_sum / k_num_k3)[: len(k_center)] * (2 * L + 1)
+        if dvm == "full":
+            S1 = np.zeros((k_bins, k_bins), dtype="c16")
+            S2 = np.zeros((k_bins, k_bins), dtype="c16")
+            if np.ndim(SN1) == 0:
+                S1[:] = SN1
+            else:
+                S1[:] = np.asarray(SN1)[:, None]
+            if np.ndim(SN2) == 0:
+                S2[:] = SN2
+            else:
+                S2[:] = np.asarray(SN2)[None, :]
+            SN1, SN2 = S1, S2
+
+        def s3_ana():
+            if tracer_type == "abb":
+                return 0.0
+            if not all_eff:
+                raise NotImplementedError("shotnoise_ana_all_eff=False is not supported")
+            with np.errstate(invalid="ignore", divide="ignore"):
+                weighted_k_num = k_num_k3 / k3_center
+            if dvm == "diagonal":
+                s3 = np.zeros(k_bins, dtype="complex128")
+                for i in range(k_bins):
+                    q = A.get_q_ells_eff(i, i, k_eff, ell_1, ell_2, L, k3_bins, k3_eff)
+                    numer = np.sum(q * total_sum / k3_center)
+                    denom = np.sum(weighted_k_num[: 2 * i + 1])
+                    if denom > 0:
+                        s3[i] = numer / denom
+            else:
+                s3 = np.zeros((k_bins, k_bins), dtype="complex128")
+                for i in range(k_bins):
+                    for j in range(k_bins):
+                        q = A.get_q_ells_eff(i, j, k_eff, ell_1, ell_2, L, k3_bins, k3_eff)
+                        valid = A.get_valid_k3_bins(k_center[i], k_center[j], k_min, k_max, k_bins)
+                        numer = np.sum(q * total_sum / k3_center)
+                        denom = np.sum(weighted_k_num * valid)
+                        if denom > 0:
+                            s3[i, j] = numer / denom
+            return s3 * (H_ells * N_ells)
+
+        if sn_mode == "ana":
+            SN3 = s3_ana()
+        elif sn_mode in ("fft", "both"):
+            raise NotImplementedError("shotnoise_mode 'fft' is not available yet on the GPU path")
+        else:
+            raise ValueError("shotnoise_mode must be either 'ana', 'fft', or 'both'.")
+
+    # ---------------------------------------------------------- normalisation
+    V = boxsize.prod()
+    if correlation_mode == "auto":
+        I_norm = (N_gal_a ** 3) / V ** 2
+    elif tracer_type == "aab":
+        I_norm = (N_gal_a ** 2 * N_gal_b) / V ** 2
+    elif tracer_type == "abb":
+        I_norm = (N_gal_a * N_gal_b ** 2) / V ** 2
+    else:
+        I_norm = (N_gal_a * N_gal_b * N_gal_c) / V ** 2
+    total_shot_noise = SN0 + SN1 + SN2 + SN3
+    total_res = total_res * (N_ells * H_ells / I_norm)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        if dvm == "full":
+            total_res *= V ** 2 / np.outer(k_num[:k_bins], k_num[:k_bins])
+        else:
+            total_res *= V ** 2 / (k_num[: len(total_res)]) ** 2
+    total_res *= vol_per_cell
+    total_shot_noise = total_shot_noise * (1 / I_norm)
+    sn_terms = {"SN0": SN0 / I_norm, "SN1": SN1 / I_norm, "SN2": SN2 / I_norm, "SN3": SN3 / I_norm}
+    if sn_mode == "both":
+        sn_terms["SN3_FFT"] = SN3_fft / I_norm
+    return {
+        "B_sugi": total_res - total_shot_noise,
+        "SN_terms": sn_terms,
+        "I_norm": I_norm,
+        "Shot_noise": total_shot_noise,
+        "Bk_raw": total_res,
+        "k_eff": k_eff,
+        "nmodes": k_num,
+    }

@@ -1,0 +1,129 @@
+"""``run_stats`` / ``run_task``: the drop-in boundary (``task_executor.py:11-343``).
+
+Same config dict, same validation and errors, same output file names and the
+same pickled ``.npy`` result dict (results merged with ``stat_attrs``).  The
+numeric work runs on one MI355X through libcnpc_hip.so; there are no MPI
+ranks (the reference's rank-0 logic is the only rank here).
+"""
+import gc
+import logging
+import os
+import time
+
+import numpy as np
+
+from .estimators import calculate_bk_sugi_box, calculate_power_spectrum_box
+from .mesh import get_mesh_box
+from .param_helper import (catalog_check, validate_config, validate_poles, validate_sugi_poles, validate_tracer)
+
+
+def _save(path, res):
+    np.save(path, res, allow_pickle=True)
+    logging.info(f"result saved to {path}")
+
+
+def run_task(statistic, correlation_mode, geometry, catalogs, **kwargs):
+    """``task_executor.py:11-328``.  ``kwargs`` may carry ``arrays`` (catalog
+    arrays instead of files) and the GPU options ``precision`` /
+    ``contraction``."""
+    logging.basicConfig(level=logging.INFO, format="%(asctime)s - %(module)s - %(levelname)s - %(message)s")
+    logging.info(f"Running {correlation_mode} {statistic} task with catalogs:{catalogs},\n")
+    arrays = kwargs.pop("arrays", None)
+    precision = kwargs.get("precision", "fp64")
+    k_max, nmesh, boxsize = kwargs["k_max"], kwargs["nmesh"], kwargs["boxsize"]
+    nyquist_freq = np.pi * nmesh[0] / boxsize[0]
+    logging.info(f"Nyquist frequency: {nyquist_freq}")
+    t0 = time.time()
+
+    if statistic == "pk":
+        if k_max >= nyquist_freq:
+            raise ValueError(
+                f"k_max {k_max} is larger than the Nyquist frequency {nyquist_freq}. Please choose a smaller k_max.")
+        validate_poles(kwargs["poles"])
+        if geometry == "survey-like":
+            raise NotImplementedError("survey-like geometry is not available on the GPU path yet")
+        poles = kwargs["poles"]
+        if all(p % 2 == 1 for p in poles):
+            raise ValueError(
+                f"All poles in {poles} are odd. Please choose at least one even pole, since odd poles are "
+                "defaultly set to 0 for periodic boxes.")
+        stat_attrs, fa, fb = get_mesh_box(
+            catalogs, correlation_mode, nmesh=kwargs["nmesh"], geometry=geometry, boxsize=kwargs["boxsize"],
+            sampler=kwargs["sampler"], interlaced=kwargs["interlaced"], column_names=kwargs["column_names"],
+            statistic=statistic, apply_rsd=kwargs["apply_rsd"], para_cosmo=kwargs["cosmology"],
+            redshift_box=kwargs["redshift_box"], los=kwargs["rsd"], precision=precision, arrays=arrays)
+        stat_attrs.update(kwargs)
+        stat_attrs["nyquist_freq"] = nyquist_freq
+        t1 = time.time()
+        logging.info(f"Time to create (FKP) overdensity field(s): {t1 - t0:.2f} seconds")
+        pk_res = calculate_power_spectrum_box(fa, fb, correlation_mode, stat_attrs, **kwargs)
+        del fa, fb
+        gc.collect()
+        pk_res.update(stat_attrs)
+        for key in ("z_range", "scheme"):
+            pk_res.pop(key, None)
+        out_dir = kwargs.get("output_dir")
+        name_a = os.path.splitext(os.path.basename(catalogs["data_a"]))[0]
+        os.makedirs(out_dir, exist_ok=True)
+        if correlation_mode == "auto":
+            path = os.path.join(out_dir, f"pk_res_{correlation_mode}_{name_a}.npy")
+        else:
+            name_b = os.path.splitext(os.path.basename(catalogs["data_b"]))[0]
+            path = os.path.join(out_dir, f"pk_res_{correlation_mode}_{name_a}_{name_b}.npy")
+        _save(path, pk_res)
+        logging.info(f"Time to compute power spectrum: {time.time() - t1:.2f} seconds")
+        return pk_res
+
+    if statistic == "bk_sugi":
+        tracer_type = kwargs["tracer_type"]
+        angu_config = kwargs["angu_config"]
+        dvm = kwargs.get("data_vector_mode", "diagonal")
+        sn_mode = kwargs.get("shotnoise_mode", "ana")
+        if dvm not in ("diagonal", "full"):
+            raise ValueError("data_vector_mode must be either 'diagonal' or 'full'")
+        if sn_mode not in ("ana", "fft", "both"):
+            raise ValueError("shotnoise_mode must be either 'ana', 'fft', or 'both'")
+        kwargs.setdefault("block_size", 1 if dvm == "diagonal" else "full")
+        validate_tracer(tracer_type, correlation_mode)
+        if 2 * k_max >= nyquist_freq:
+            raise ValueError(
+                f"2 * k_max = {2 * k_max} is larger than the Nyquist frequency {nyquist_freq}. "
+                "Please choose a smaller k_max.")
+        validate_sugi_poles(angu_config, geometry)
+        if geometry != "box-like":
+            raise NotImplementedError("survey-like geometry is not available on the GPU path yet")
+        stat_attrs, fa, fb, fc = get_mesh_box(
+            catalogs, correlation_mode, statistic=statistic, nmesh=kwargs["nmesh"], geometry=geometry,
+            boxsize=kwargs["boxsize"], sampler=kwargs["sampler"], interlaced=kwargs["interlaced"],
+            column_names=kwargs["column_names"], tracer_type=tracer_type, apply_rsd=kwargs["apply_rsd"],
+            para_cosmo=kwargs["cosmology"], redshift_box=kwargs["redshift_box"], los=kwargs["rsd"],
+            precision=precision, arrays=arrays)
+        stat_attrs.update(kwargs)
+        stat_attrs["nyquist_freq"] = nyquist_freq
+        t1 = time.time()
+        logging.info(f"Time to create overdensity field(s): {t1 - t0:.2f} seconds")
+        bk_res = calculate_bk_sugi_box(fa, fb, fc, correlation_mode, stat_attrs, **kwargs)
+        bk_res.update(stat_attrs)
+        for key in ("z_range", "comp_weight_plan", "scheme", "alpha_scheme", "normalization_scheme"):
+            bk_res.pop(key, None)
+        out_dir = kwargs.get("output_dir")
+        os.makedirs(out_dir, exist_ok=True)
+        name_a = os.path.splitext(os.path.basename(catalogs["data_a"]))[0]
+        parent = os.path.basename(os.path.dirname(catalogs["data_a"]))
+        if kwargs.get("use_parent_dir", True):
+            name_a = parent + "_" + name_a
+        l1, l2, L = angu_config
+        path = os.path.join(out_dir,
+                            f"bk_sugi_res_{dvm}_{l1}{l2}{L}_{correlation_mode}_{tracer_type}_{name_a}.npy")
+        _save(path, bk_res)
+        logging.info(f"Time to compute bispectrum: {time.time() - t1:.2f} seconds")
+        return bk_res
+    raise ValueError(f"unsupported statistic {statistic!r}")
+
+
+def run_stats(config):
+    """``task_executor.py:330-343``: validate, create output_dir, run."""
+    validate_config(config)
+    catalog_check(config["catalogs"], config["geometry"], config["correlation_mode"], config["statistic"])
+    os.makedirs(config["output_dir"], exist_ok=True)
+    return run_task(**config)

@@ -1,0 +1,211 @@
+"""GPU parity: the HIP path (through libcnpc_hip.so) against the FP64 oracle
+on identical seeded inputs.
+
+Tolerances: FP64 path 1e-10 relative per bin (relative to max|B| for
+bispectrum entries, which cross zero); FP32 + split-bf16 path 1e-4 relative to
+max|B| per multipole.
+"""
+import numpy as np
+import pytest
+
+from tests.helpers import bk_attrs, catalog, pk_attrs, rel_err, rel_to_max, weights
+
+pytestmark = pytest.mark.gpu
+
+WINDOWS = ["ngp", "cic", "tsc", "pcs"]
+
+
+def _oracle():
+    from oracle.cnpc_ref import angular, box, conventions
+
+    return conventions, angular, box
+
+
+def _log(msg):
+    print(msg, flush=True)
+
+
+@pytest.mark.parametrize("sampler", WINDOWS)
+@pytest.mark.parametrize("prec", [0, 1])
+def test_paint_matches_oracle(gpu_ctx, sampler, prec):
+    import torch
+
+    C, _, _ = _oracle()
+    _log(f"paint {sampler} prec={prec}")
+    n, L = 32, 500.0
+    pos = catalog(5000, L, 3, ngrid=32)
+    w = weights(len(pos), seed=4)
+    ctx = gpu_ctx
+    m0, m1 = ctx.mesh(n, prec), ctx.mesh(n, prec)
+    ctx.paint(torch.from_numpy(pos.copy()).cuda(), torch.from_numpy(w).cuda(), n, L, sampler, True, prec, m0, m1)
+    ref0 = C.paint(pos, w, n, L, sampler).real
+    ref1 = C.paint(pos, w, n, L, sampler, shift=0.5).real
+    tol = 1e-12 if prec == 0 else 2e-6
+    assert np.max(np.abs(m0.cpu().numpy() - ref0)) <= tol * np.max(ref0)
+    assert np.max(np.abs(m1.cpu().numpy() - ref1)) <= tol * np.max(ref1)
+    assert abs(m0.sum().item() - w.sum()) <= 1e-9 * w.sum() * (1 if prec == 0 else 1e4)
+
+
+def test_paint_wraps_and_empty(gpu_ctx):
+    import torch
+
+    C, _, _ = _oracle()
+    n, L = 16, 100.0
+    pos = np.array([[0.0, 0.0, 0.0], [99.99, 99.99, 99.99], [50.0, 0.01, 99.0]])
+    w = np.array([1.0, 2.0, 3.0])
+    ctx = gpu_ctx
+    m0, m1 = ctx.mesh(n, 0), ctx.mesh(n, 0)
+    ctx.paint(torch.from_numpy(pos).cuda(), torch.from_numpy(w).cuda(), n, L, "pcs", True, 0, m0, m1)
+    np.testing.assert_allclose(m0.cpu().numpy(), C.paint(pos, w, n, L, "pcs").real, atol=1e-14)
+    np.testing.assert_allclose(m1.cpu().numpy(), C.paint(pos, w, n, L, "pcs", shift=0.5).real, atol=1e-14)
+    e = torch.zeros((0, 3), dtype=torch.float64, device="cuda")
+    ctx.paint(e, torch.zeros(0, dtype=torch.float64, device="cuda"), n, L, "cic", False, 0, m0)
+    assert m0.abs().max().item() == 0.0
+
+
+def test_fft_roundtrip_and_normalisation(gpu_ctx):
+    import torch
+
+    n = 32
+    rng = np.random.default_rng(0)
+    x = rng.standard_normal((n, n, n))
+    ctx = gpu_ctx
+    X = torch.from_numpy(x).cuda()
+    S = ctx.spectrum(n, 0)
+    ctx.r2c(X, S, n, 0)
+    ref = np.fft.rfftn(x)
+    assert np.max(np.abs(S.cpu().numpy() - ref)) <= 1e-12 * np.max(np.abs(ref))
+    Y = ctx.mesh(n, 0)
+    ctx.c2r(S, Y, n, 0)
+    np.testing.assert_allclose(Y.cpu().numpy() / n ** 3, x, atol=1e-12)
+
+
+@pytest.mark.parametrize("sampler,interlaced", [("cic", True), ("tsc", False), ("pcs", True), ("ngp", False)])
+def test_pk_box_matches_oracle(gpu_ctx, sampler, interlaced):
+    from cosmonpc_amd.estimators import calculate_power_spectrum_box
+    from cosmonpc_amd.mesh import paint_field
+
+    _, _, B = _oracle()
+    _log(f"pk {sampler} interlaced={interlaced}")
+    n, L = 64, 1000.0
+    pos = catalog(40000, L, 5, ngrid=64)
+    w = np.ones(len(pos))
+    attrs, fa, _, _ = B.get_mesh_box({"a": (pos, w)}, [n] * 3, [L] * 3, sampler, interlaced)
+    sa = dict(attrs)
+    sa.update(pk_attrs(n, L, 0.005, 0.185, 18, [0, 1, 2, 4], sampler, interlaced, rsd=(0, 0.6, 0.8)))
+    ref = B.calculate_power_spectrum_box(fa, None, "auto", sa)
+    f = paint_field(gpu_ctx, pos, w, n, L, sampler, interlaced, 0)
+    got = calculate_power_spectrum_box(f, None, "auto", sa)
+    np.testing.assert_array_equal(got["k_num"], ref["k_num"])
+    assert rel_err(got["k_eff"], ref["k_eff"]) < 1e-12
+    for p in (0, 2, 4):
+        assert rel_err(got[f"P{p}"], ref[f"P{p}"]) < 1e-10, p
+    assert got["P1"].dtype == np.float128 and not np.any(got["P1"])
+
+
+def test_pk_box_cross_matches_oracle(gpu_ctx):
+    from cosmonpc_amd.estimators import calculate_power_spectrum_box
+    from cosmonpc_amd.mesh import paint_field
+
+    _, _, B = _oracle()
+    n, L = 48, 800.0
+    pa = catalog(20000, L, 6, ngrid=48)
+    pb = catalog(15000, L, 7, ngrid=48)
+    wa, wb = weights(len(pa), 1), weights(len(pb), 2)
+    attrs, fa, fb, _ = B.get_mesh_box({"a": (pa, wa), "b": (pb, wb)}, [n] * 3, [L] * 3, "tsc", True,
+                                      correlation_mode="cross")
+    sa = dict(attrs)
+    sa.update(pk_attrs(n, L, 0.0, 0.18, 12, [0, 2], "tsc", True))
+    ref = B.calculate_power_spectrum_box(fa, fb, "cross", sa)
+    ga = paint_field(gpu_ctx, pa, wa, n, L, "tsc", True, 0)
+    gb = paint_field(gpu_ctx, pb, wb, n, L, "tsc", True, 0)
+    got = calculate_power_spectrum_box(ga, gb, "cross", sa)
+    np.testing.assert_array_equal(got["k_num"], ref["k_num"])
+    for p in (0, 2):
+        assert rel_to_max(got[f"P{p}"], ref[f"P{p}"]) < 1e-10
+
+
+def _bk_case(gpu_ctx, angu, dvm, contraction, prec=0, sampler="tsc", interlaced=True, n=32, L=600.0,
+             k=(0.02, 0.14, 8), npart=30000, seed=11, block_size=None):
+    from cosmonpc_amd.estimators import calculate_bk_sugi_box
+    from cosmonpc_amd.mesh import paint_field
+
+    _, _, B = _oracle()
+    _log(f"bk {angu} {dvm} {contraction} prec={prec} {sampler} n={n}")
+    pos = catalog(npart, L, seed, ngrid=n)
+    w = np.ones(len(pos))
+    attrs, fa, _, _ = B.get_mesh_box({"a": (pos, w)}, [n] * 3, [L] * 3, sampler, interlaced)
+    extra = {} if block_size is None else {"block_size": block_size}
+    sa = dict(attrs)
+    sa.update(bk_attrs(n, L, k[0], k[1], k[2], angu, dvm, sampler, interlaced, contraction=contraction, **extra))
+    ref = B.calculate_bk_sugi_box(fa, None, None, "auto", sa)
+    f = paint_field(gpu_ctx, pos, w, n, L, sampler, interlaced, prec)
+    got = calculate_bk_sugi_box(f, None, None, "auto", sa)
+    return got, ref
+
+
+@pytest.mark.parametrize("angu", [(0, 0, 0), (2, 0, 2), (1, 1, 0), (2, 2, 2)])
+@pytest.mark.parametrize("contraction", ["gemm_fp64", "pairwise"])
+def test_bk_box_full_fp64_matches_oracle(gpu_ctx, angu, contraction):
+    got, ref = _bk_case(gpu_ctx, angu, "full", contraction)
+    np.testing.assert_array_equal(got["nmodes"], ref["nmodes"])
+    assert rel_to_max(got["Bk_raw"], ref["Bk_raw"]) < 1e-10
+    assert rel_to_max(got["B_sugi"], ref["B_sugi"]) < 1e-10
+    for key in ("SN0", "SN1", "SN2", "SN3"):
+        a, b = np.asarray(got["SN_terms"][key]), np.asarray(ref["SN_terms"][key])
+        if np.max(np.abs(b)) > 0:
+            assert rel_to_max(a, b) < 1e-10, key
+        else:
+            assert np.max(np.abs(a)) == 0, key
+
+
+@pytest.mark.parametrize("angu", [(0, 0, 0), (2, 0, 2), (2, 2, 0)])
+def test_bk_box_diagonal_fp64_matches_oracle(gpu_ctx, angu):
+    got, ref = _bk_case(gpu_ctx, angu, "diagonal", "gemm_fp64", sampler="pcs")
+    assert rel_to_max(got["B_sugi"], ref["B_sugi"]) < 1e-10
+    assert rel_to_max(got["Bk_raw"], ref["Bk_raw"]) < 1e-10
+
+
+def test_bk_box_blocked_matches_full(gpu_ctx):
+    got, ref = _bk_case(gpu_ctx, (0, 0, 0), "full", "pairwise", block_size=3)
+    assert rel_to_max(got["Bk_raw"], ref["Bk_raw"]) < 1e-10
+
+
+@pytest.mark.parametrize("angu", [(0, 0, 0), (2, 0, 2)])
+@pytest.mark.parametrize("contraction", ["bf16x3", "fp16x3"])
+def test_bk_box_fp32_split_error(gpu_ctx, angu, contraction):
+    got, ref = _bk_case(gpu_ctx, angu, "full", contraction, prec=1, n=48, L=800.0, k=(0.02, 0.18, 10),
+                        npart=60000)
+    err = rel_to_max(got["Bk_raw"], ref["Bk_raw"])
+    _log(f"{contraction} {angu} error rel max|B| = {err:.3e}")
+    assert err < 1e-4
+
+
+def test_gemm_modes_agree_with_pairwise(gpu_ctx):
+    """GEMM tiles (incl. rows beyond 32 and ragged K) against the pairwise sum."""
+    import torch
+
+    rng = np.random.default_rng(5)
+    ncell = 64 * 1000 + 64
+    na, nb = 37, 5
+    A = rng.standard_normal((na, ncell))
+    B = rng.standard_normal((nb, ncell))
+    G = rng.standard_normal(ncell)
+    ref = (A * G) @ B.T
+    ctx = gpu_ctx
+    Ad, Bd, Gd = (torch.from_numpy(x).cuda() for x in (A, B, G))
+    C64 = ctx.contract_gemm("gemm_fp64", Gd, Ad, Bd, nblk=7).cpu().numpy()
+    assert np.max(np.abs(C64 - ref)) < 1e-10 * np.max(np.abs(ref))
+    ia, ib = np.meshgrid(np.arange(na), np.arange(nb), indexing="ij")
+    P = ctx.contract_pairs(0, Gd, Ad, Bd, ia.ravel(), ib.ravel()).reshape(na, nb)
+    assert np.max(np.abs(P - ref)) < 1e-10 * np.max(np.abs(ref))
+    Cs = ctx.contract_gemm("gemm_fp64", Gd, Ad, None, nblk=13).cpu().numpy()
+    refs = (A * G) @ A.T
+    assert np.max(np.abs(Cs - refs)) < 1e-10 * np.max(np.abs(refs))
+    A32, B32, G32 = (torch.from_numpy(x.astype(np.float32)).cuda() for x in (A, B, G))
+    ref32 = (A.astype(np.float32).astype(np.float64) * G.astype(np.float32)) @ B.astype(np.float32).T
+    for mode in ("bf16x3", "fp16x3"):
+        s = 1.0 if mode == "bf16x3" else 2.0 ** 10
+        C = ctx.contract_gemm(mode, G32, A32, B32, sa=s, sb=s, nblk=9).cpu().numpy()
+        err = np.max(np.abs(C - ref32)) / np.sqrt(ncell)
+        assert err < 1e-3, (mode, err)

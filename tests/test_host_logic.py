@@ -1,0 +1,123 @@
+"""Host logic of the run_stats boundary: validation errors (types and
+messages of the reference's param_helper.py / task_executor.py), config
+normalisation, and failures raised before any GPU work."""
+import copy
+
+import numpy as np
+import pytest
+
+from cosmonpc_amd import param_helper as P
+
+
+def _base(**kw):
+    c = {
+        "statistic": "pk", "correlation_mode": "auto", "geometry": "box-like",
+        "catalogs": {"data_a": "a.npy", "randoms_a": None, "data_b": None, "randoms_b": None},
+        "column_names": ["x", "y", "z"], "rsd": [0, 0, 1], "apply_rsd": False,
+        "nmesh": [128, 128, 128], "boxsize": [1000.0, 1000.0, 1000.0], "sampler": "cic", "interlaced": True,
+        "k_min": 0.005, "k_max": 0.305, "k_bins": 30, "poles": [0, 2, 4], "compensation": True,
+        "cosmology": {"h": 0.6766, "Omega0": 0.30966}, "redshift_box": 0.5, "output_dir": "/tmp/cnpc_out",
+    }
+    c.update(kw)
+    return c
+
+
+def test_validate_config_aliases_and_defaults():
+    c = _base(**{"shotnoise-mode": "fft", "use_fast_mode": False})
+    P.validate_config(c)
+    assert c["shotnoise_mode"] == "fft" and "shotnoise-mode" not in c
+    assert c["high_order_mode"] == "default" and "use_fast_mode" not in c
+    c = _base()
+    P.validate_config(c)
+    assert c["shotnoise_mode"] == "ana" and c["high_order_mode"] == "default"
+    c = _base(high_order_mode="fast", use_fast_mode=True, geometry="survey-like", poles=[0, 2, 4])
+    P.validate_config(c)
+    assert "use_fast_mode" not in c
+
+
+@pytest.mark.parametrize("kw,exc,msg", [
+    ({"sampler": "xyz"}, AssertionError, "sampler should be 'ngp', 'cic', 'pcs', or 'tsc'"),
+    ({"nmesh": [64, 64]}, AssertionError, "nmesh must be a list or tuple of three elements"),
+    ({"nmesh": [64, 64, 32]}, AssertionError, "All elements in nmesh must be equal"),
+    ({"nmesh": [64.0, 64, 64]}, AssertionError, "All elements in nmesh must be positive ints"),
+    ({"boxsize": [1000.0, -1.0, 1000.0]}, AttributeError, "__name__"),
+    ({"shotnoise_mode": "x"}, ValueError, "shotnoise_mode must be either 'ana', 'fft', or 'both'"),
+    ({"high_order_mode": "x"}, ValueError, "high_order_mode must be one of 'default', 'fast', or 'compare'"),
+    ({"high_order_mode": "fast"}, ValueError, "only supported for survey-like pk measurements"),
+    ({"interlaced": 1}, TypeError, "Config field 'interlaced' must be a boolean (True/False), got int: 1"),
+])
+def test_validate_config_errors(kw, exc, msg):
+    c = _base(**kw)
+    with pytest.raises(exc) as e:
+        P.validate_config(c)
+    assert msg in str(e.value)
+
+
+def test_validate_poles_and_sugi_poles():
+    P.validate_poles([0, 2, 4])
+    with pytest.raises(ValueError, match="non-negative integers"):
+        P.validate_poles([0, -2])
+    with pytest.raises(ValueError, match="duplicate"):
+        P.validate_poles([0, 0])
+    with pytest.raises(ValueError, match="ascending"):
+        P.validate_poles([2, 0])
+    P.validate_sugi_poles([2, 0, 2], "box-like")
+    for cfg, msg in (([0, 2, 2], "ell1 must be greater"), ([1, 0, 0], "must be even"), ([1, 1, 1], "must be even"),
+                     ([2, 0, 4], "triangle condition")):
+        with pytest.raises(ValueError, match=msg):
+            P.validate_sugi_poles(cfg, "box-like")
+    with pytest.raises(ValueError, match="geometry must be"):
+        P.validate_sugi_poles([0, 0, 0], "sphere")
+
+
+def test_validate_tracer_and_catalog_check():
+    P.validate_tracer("aaa", "auto")
+    with pytest.raises(AssertionError, match="For auto-correlation"):
+        P.validate_tracer("aab", "auto")
+    with pytest.raises(AssertionError, match="tracer_type must be one of"):
+        P.validate_tracer("xyz", "cross")
+    with pytest.raises(AssertionError, match="data_b catalog must be provided"):
+        P.catalog_check({"data_a": "a", "data_b": None}, "box-like", "cross", "pk")
+    with pytest.raises(AssertionError, match="randoms_a catalog must be provided for survey-like auto"):
+        P.catalog_check({"data_a": "a", "randoms_a": None}, "survey-like", "auto", "pk")
+    with pytest.raises(AssertionError, match="data_c catalog must be provided"):
+        P.catalog_check({"data_a": "a", "data_b": "b"}, "box-like", "cross", "bk_sugi", tracer_type="abc")
+
+
+def test_run_task_errors_before_gpu_work():
+    from cosmonpc_amd.task_executor import run_task
+
+    c = _base(k_max=0.3)
+    c["nmesh"] = [32, 32, 32]  # k_Nyq = 0.1005
+    with pytest.raises(ValueError, match="is larger than the Nyquist frequency"):
+        run_task(**copy.deepcopy(c))
+    c = _base(poles=[1, 3])
+    with pytest.raises(ValueError, match="All poles in"):
+        run_task(**c)
+    c = _base(statistic="bk_sugi", tracer_type="aaa", angu_config=[0, 0, 0], k_max=0.2)
+    c["nmesh"] = [64, 64, 64]  # 2 k_max = 0.4 >= k_Nyq = 0.201
+    with pytest.raises(ValueError, match="2 \\* k_max"):
+        run_task(**c)
+    c = _base(statistic="bk_sugi", tracer_type="aaa", angu_config=[0, 0, 0], k_max=0.05,
+              data_vector_mode="both")
+    with pytest.raises(ValueError, match="data_vector_mode must be either"):
+        run_task(**c)
+    c = _base(statistic="bk_sugi", tracer_type="aab", angu_config=[0, 0, 0], k_max=0.05)
+    with pytest.raises(AssertionError, match="For auto-correlation"):
+        run_task(**c)
+
+
+def test_catalog_reader_box_npy(tmp_path):
+    from cosmonpc_amd import catalog, synth
+
+    pos = np.random.default_rng(0).uniform(0, 10, (50, 3))
+    w = np.linspace(1, 2, 50)
+    f = tmp_path / "cat.npy"
+    cols = synth.write_npy_catalog(str(f), pos, w)
+    p, ww = catalog.read_box_catalog(str(f), cols)
+    np.testing.assert_array_equal(p, pos)
+    np.testing.assert_array_equal(ww, w)
+    p, ww = catalog.read_box_catalog(str(f), ["x", "y", "z", "other"])
+    assert np.all(ww == 1.0)
+    with pytest.raises(ValueError, match="Unsupported data file type: txt"):
+        catalog.read_box_catalog("cat.txt", cols)

@@ -1,0 +1,171 @@
+"""Pins the oracle's angular functions and the product's host tables against
+scipy / sympy library routines and identities."""
+import math
+
+import numpy as np
+import pytest
+import scipy.special as sps
+
+from oracle.cnpc_ref import angular as OA
+
+
+def _unit_vectors(n=64, seed=0):
+    rng = np.random.default_rng(seed)
+    v = rng.standard_normal((n, 3))
+    v /= np.linalg.norm(v, axis=1)[:, None]
+    v[0] = (0, 0, 1)
+    v[1] = (0, 0, -1)
+    v[2] = (1, 0, 0)
+    return v
+
+
+@pytest.mark.parametrize("l", range(0, 5))
+def test_oracle_ylm_matches_scipy(l):
+    v = _unit_vectors()
+    theta = np.arccos(np.clip(v[:, 2], -1, 1))
+    phi = np.arctan2(v[:, 1], v[:, 0])
+    for m in range(-l, l + 1):
+        got = OA.get_Ylm(l, m, Racah_normalized=True)(v[:, 0], v[:, 1], v[:, 2])
+        exp = sps.sph_harm_y(l, m, theta, phi) * np.sqrt(4 * np.pi / (2 * l + 1))
+        np.testing.assert_allclose(got, exp, atol=1e-12)
+        got_n = OA.get_Ylm(l, m, Racah_normalized=False)(v[:, 0], v[:, 1], v[:, 2])
+        np.testing.assert_allclose(got_n, sps.sph_harm_y(l, m, theta, phi), atol=1e-12)
+
+
+@pytest.mark.parametrize("l", range(0, 5))
+def test_addition_theorem(l):
+    v = _unit_vectors(seed=l)
+    s = sum(np.abs(OA.get_Ylm(l, m, True)(v[:, 0], v[:, 1], v[:, 2])) ** 2 for m in range(-l, l + 1))
+    np.testing.assert_allclose(s, 1.0, atol=1e-12)
+
+
+@pytest.mark.parametrize("l", range(0, 9))
+@pytest.mark.parametrize("racah", [True, False])
+def test_product_ylm_tables_match_oracle(l, racah):
+    from cosmonpc_amd import angular as A
+
+    v = _unit_vectors(seed=10 + l)
+    for m in range(-l, l + 1):
+        if l > 4 and abs(m) > 0:
+            continue  # the sympy construction is slow for high l, m != 0
+        ref = OA.get_Ylm(l, m, racah)
+        got = A.ylm_value(l, m, v[:, 0], v[:, 1], v[:, 2], racah)
+        np.testing.assert_allclose(got, ref(v[:, 0], v[:, 1], v[:, 2]), atol=1e-12)
+        # value at the zero vector (the k = 0 mode) follows the sympy polynomial
+        z0 = complex(ref(np.zeros(1), np.zeros(1), np.zeros(1))[0])
+        assert abs(A.ylm_k0(l, m, racah) - z0) < 1e-14, (l, m)
+
+
+def _emulate_poly(p, x, y, z):
+    """Host restatement of the device polynomial evaluation (poly_eval)."""
+    t = p.rx * x + p.ry * y + p.rz * z
+    val = np.polynomial.polynomial.polyval(t, [p.zc[i] for i in range(p.deg + 1)]) * (x + 1j * y) ** p.m
+    if p.part == 1:
+        val = val.real + 0j
+    elif p.part == 2:
+        val = val.imag + 0j
+    elif p.part == 3:
+        val = np.conj(val)
+    out = complex(p.pre_re, p.pre_im) * val
+    zero = (x == 0) & (y == 0) & (z == 0)
+    return np.where(zero, complex(p.k0_re, p.k0_im), out)
+
+
+@pytest.mark.parametrize("l", range(0, 5))
+def test_poly_specs_and_shell_components(l):
+    from cosmonpc_amd import angular as A
+
+    v = _unit_vectors(seed=20 + l)
+    x, y, z = v[:, 0], v[:, 1], v[:, 2]
+    for m in range(-l, l + 1):
+        ref = A.ylm_value(l, m, x, y, z)
+        np.testing.assert_allclose(_emulate_poly(A.ylm_poly(l, m), x, y, z), ref, atol=1e-13)
+        np.testing.assert_allclose(_emulate_poly(A.ylm_poly(l, m, part=A.PART_RE), x, y, z), ref.real, atol=1e-13)
+        np.testing.assert_allclose(_emulate_poly(A.ylm_poly(l, m, part=A.PART_IM), x, y, z), ref.imag, atol=1e-13)
+        comps = A.shell_components(l, m)
+        total = sum(ph * _emulate_poly(p, x, y, z) for p, ph in comps)
+        np.testing.assert_allclose(total, ref, atol=1e-13)
+        for p, _ in comps:  # each weight times a Hermitian field stays Hermitian
+            np.testing.assert_allclose(_emulate_poly(p, -x, -y, -z), np.conj(_emulate_poly(p, x, y, z)), atol=1e-13)
+
+
+@pytest.mark.parametrize("ell", range(0, 9))
+def test_legendre_poly(ell):
+    from cosmonpc_amd import angular as A
+
+    r = np.array([0.0, 0.6, 0.8])
+    v = _unit_vectors(seed=ell)
+    mu = v @ r
+    exp = sps.eval_legendre(ell, mu)
+    np.testing.assert_allclose(_emulate_poly(A.legendre_poly(ell, r), v[:, 0], v[:, 1], v[:, 2]).real, exp,
+                               atol=1e-12)
+    np.testing.assert_allclose(OA.get_legendre(ell, *r)(v[:, 0], v[:, 1], v[:, 2]), exp, atol=1e-12)
+
+
+def test_wigner_3j_racah_vs_sympy_and_orthogonality():
+    from cosmonpc_amd import angular as A
+
+    for j1 in range(0, 5):
+        for j2 in range(0, 5):
+            for j3 in range(abs(j1 - j2), j1 + j2 + 1):
+                tot = 0.0
+                for m1 in range(-j1, j1 + 1):
+                    for m2 in range(-j2, j2 + 1):
+                        m3 = -m1 - m2
+                        if abs(m3) > j3:
+                            continue
+                        a = A.wigner_3j(j1, j2, j3, m1, m2, m3)
+                        if (j1 + j2 + j3 + m1) % 3 == 0:  # spot-check against sympy
+                            assert a == pytest.approx(OA.wigner_3j(j1, j2, j3, m1, m2, m3), abs=1e-14)
+                        tot += a * a
+                # sum over m1, m2 at fixed m3 is 1/(2 j3 + 1); over all m3 it is 1
+                assert tot == pytest.approx(1.0, abs=1e-13)
+
+
+def test_assoc_legendre_vs_sympy():
+    from cosmonpc_amd import angular as A
+
+    for ell in range(0, 5):
+        for m in range(-ell, ell + 1):
+            for x in (-1.0, -0.73, 0.0, 0.31, 0.99, 1.0):
+                assert A.assoc_legendre(ell, m, x) == pytest.approx(OA.assoc_legendre(ell, m, x), abs=1e-13)
+
+
+def test_magnetic_configs_and_q_ells():
+    from cosmonpc_amd import angular as A
+
+    for cfg in ([0, 0, 0], [2, 0, 2], [1, 1, 0], [2, 2, 2], [3, 1, 2], [4, 2, 2]):
+        c1, t1 = A.get_magnetic_configs_box(*cfg)
+        c2, t2 = OA.get_magnetic_configs_box(*cfg)
+        assert c1 == c2 and np.allclose(t1, t2, atol=1e-15)
+        c1, t1 = A.get_magnetic_configs_survey(*cfg)
+        c2, t2 = OA.get_magnetic_configs_survey(*cfg)
+        assert c1 == c2 and np.allclose(t1, t2, atol=1e-15)
+    k_eff = np.array([0.015, 0.025, 0.036, 0.044])
+    k3_eff = np.array([0.031, 0.05, 0.072, 0.089, 0.11, 0.13, 0.149])
+    for cfg in ([0, 0, 0], [2, 0, 2], [1, 1, 0], [2, 2, 2]):
+        for i in range(4):
+            for j in range(4):
+                a = A.get_q_ells_eff(i, j, k_eff, *cfg, 7, k3_eff)
+                b = OA.get_q_ells_eff(i, j, k_eff, *cfg, 7, k3_eff)
+                np.testing.assert_allclose(a, b, atol=1e-13)
+    # closed form: q_000 = 1 inside the triangle, 0 outside
+    q = A.get_q_ells_eff(0, 3, k_eff, 0, 0, 0, 7, k3_eff)
+    inside = (k3_eff >= abs(k_eff[0] - k_eff[3])) & (k3_eff <= k_eff[0] + k_eff[3])
+    np.testing.assert_allclose(q, inside.astype(float), atol=1e-15)
+
+
+def test_compensations_closed_form():
+    w = [np.linspace(-np.pi, np.pi, 9, endpoint=False).reshape(-1, 1, 1), np.zeros((1, 1, 1)),
+         np.zeros((1, 1, 1))]
+    v = np.ones((9, 1, 1), dtype=complex)
+    s = OA.compensate_sinc(w, v, "tsc")
+    t = np.sinc(w[0] / (2 * np.pi)) ** 3
+    np.testing.assert_allclose(s, 1 / t, rtol=1e-14)
+    c = OA.compensate_shotnoise(w, v, "cic")
+    np.testing.assert_allclose(c, 1 / np.sqrt(1 - 2 / 3 * np.sin(w[0] / 2) ** 2), rtol=1e-14)
+    # the alias sum C_p(w) = sum_m sinc^{2p}((w + 2 pi m)/2pi) (Jing 2005) for CIC
+    m = np.arange(-2000, 2001)
+    ww = 0.7
+    direct = np.sum(np.sinc((ww + 2 * np.pi * m) / (2 * np.pi)) ** 4)
+    assert direct == pytest.approx(1 - 2 / 3 * math.sin(ww / 2) ** 2, rel=1e-9)
