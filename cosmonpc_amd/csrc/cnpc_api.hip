@@ -276,6 +276,20 @@ int cnpc_bin_spectrum(cnpc_ctx* ctx, int n, double boxsize, int prec, const void
   return 0;
 }
 
+int cnpc_cross_spectrum(cnpc_ctx* ctx, int n, int prec, const void* A, const void* B, double ab_scale,
+                        int noise_window, double noise_amp, void* out) {
+  REQUIRE(ctx, "null context");
+  REQUIRE(n >= 2 && n % 2 == 0, "cross_spectrum: bad mesh");
+  REQUIRE(prec == 0 || prec == 1, "cross_spectrum: prec must be 0 or 1");
+  REQUIRE(A && B && out, "cross_spectrum: null buffers");
+  REQUIRE(noise_window >= 1 && noise_window <= 4, "cross_spectrum: noise window must be 1..4");
+  BinArgs a;
+  a.n = n; a.npoly = 0; a.noise_p = noise_window; a.conj_b = 1;
+  a.kf = 0.0; a.ab_scale = ab_scale; a.noise_amp = noise_amp;
+  CHECK_HIP(launch_cross_spectrum(prec, a, A, B, out, ctx->stream));
+  return 0;
+}
+
 int cnpc_shell_filter(cnpc_ctx* ctx, int n, double boxsize, int prec, const void* D, const double* edges, int nbins,
                       int bin0, int nshell, const cnpc_poly* weight, void* out) {
   REQUIRE(ctx, "null context");
@@ -283,7 +297,7 @@ int cnpc_shell_filter(cnpc_ctx* ctx, int n, double boxsize, int prec, const void
   REQUIRE(prec == 0 || prec == 1, "shell_filter: prec must be 0 or 1");
   REQUIRE(nbins >= 1 && nbins <= CNPC_MAX_BINS && edges, "shell_filter: bad edges");
   REQUIRE(bin0 >= 0 && nshell >= 1 && bin0 + nshell <= nbins, "shell_filter: shell range outside the bins");
-  REQUIRE(D && out && weight, "shell_filter: null buffers");
+  REQUIRE(out && weight, "shell_filter: null buffers");
   REQUIRE(weight->deg >= 0 && weight->deg < CNPC_ZDEG && weight->m >= 0, "shell_filter: bad weight polynomial");
   BinEdges E;
   fill_edges(E, edges, nbins);

@@ -202,7 +202,9 @@ __global__ void __launch_bounds__(256) shell_filter_kernel(ShellArgs a, BinEdges
       const bool zv = (k == 0.0);
       const double inv = zv ? 0.0 : 1.0 / k;
       const double2 w = poly_eval(W, kx * inv, ky * inv, kz * inv, zv);
-      const double dr = (double)D[idx].x, di = (double)D[idx].y;
+      // a null spectrum filters the weight alone (the S3 shells of
+      // clustering_estimator.py:1213-1217)
+      const double dr = D ? (double)D[idx].x : 1.0, di = D ? (double)D[idx].y : 0.0;
       v.x = (T)(dr * w.x - di * w.y);
       v.y = (T)(dr * w.y + di * w.x);
     }
@@ -213,6 +215,44 @@ __global__ void __launch_bounds__(256) shell_filter_kernel(ShellArgs a, BinEdges
       out[s * a.stride + idx] = o;
     }
   }
+}
+
+// ------------------------------------------------------------- cross spectrum
+// out = ab_scale * a conj(b) - noise_amp * C_p/sinc^{2p}(k): the S3 source
+// spectrum of clustering_estimator.py:1197-1211 (before its c2r).
+template <typename T>
+__global__ void __launch_bounds__(256) cross_spectrum_kernel(BinArgs a, const typename cplx<T>::type* __restrict__ A,
+                                                             const typename cplx<T>::type* __restrict__ B,
+                                                             typename cplx<T>::type* __restrict__ out) {
+  const int n = a.n, nzh = n / 2 + 1;
+  const int64_t total = (int64_t)n * n * nzh;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (int64_t)gridDim.x * blockDim.x) {
+    const int iz = (int)(idx % nzh);
+    const int64_t r = idx / nzh;
+    const int iy = (int)(r % n), ix = (int)(r / n);
+    const int nx = wrap_freq(ix, n), ny = wrap_freq(iy, n), nz = wrap_freq(iz, n);
+    const double ar = (double)A[idx].x, ai = (double)A[idx].y;
+    const double br = (double)B[idx].x, bi = (double)B[idx].y;
+    double fr = (ar * br + ai * bi) * a.ab_scale;
+    double fi = (ai * br - ar * bi) * a.ab_scale;
+    if (a.noise_amp != 0.0) fr -= a.noise_amp * bk_noise_factor(a.noise_p, nx, ny, nz, n);
+    typename cplx<T>::type o;
+    o.x = (T)fr;
+    o.y = (T)fi;
+    out[idx] = o;
+  }
+}
+
+hipError_t launch_cross_spectrum(int prec, const BinArgs& a, const void* A, const void* B, void* out, hipStream_t st) {
+  const int64_t total = (int64_t)a.n * a.n * (a.n / 2 + 1);
+  dim3 blk(256), grd(grid_for(total, 256, 1 << 16));
+  if (prec == 0)
+    hipLaunchKernelGGL(cross_spectrum_kernel<double>, grd, blk, 0, st, a, (const double2*)A, (const double2*)B,
+                       (double2*)out);
+  else
+    hipLaunchKernelGGL(cross_spectrum_kernel<float>, grd, blk, 0, st, a, (const float2*)A, (const float2*)B,
+                       (float2*)out);
+  return hipGetLastError();
 }
 
 hipError_t launch_shell_filter(int prec, const ShellArgs& a, const BinEdges& E, const PolySpec& W, const void* D,

@@ -127,9 +127,17 @@ class GpuContext:
         return h[:nb].copy(), h[nb:2 * nb].copy(), h[2 * nb:(2 + npoly) * nb].reshape(npoly, nb), \
             h[(2 + npoly) * nb:].reshape(npoly, nb)
 
+    def cross_spectrum(self, a, b, out, n, prec, ab_scale, noise_window, noise_amp):
+        for t in (a, b, out):
+            self._check_spec(t, n, prec)
+        check(self.lib.cnpc_cross_spectrum(self.h, n, prec, _ptr(a), _ptr(b), float(ab_scale), int(noise_window),
+                                           float(noise_amp), _ptr(out)), "cnpc_cross_spectrum")
+
     def shell_filter(self, delta, n, boxsize, prec, edges, bin0, nshell, poly, out):
+        """``delta`` None filters the weight alone."""
         edges = np.ascontiguousarray(edges, dtype=np.float64)
-        self._check_spec(delta, n, prec)
+        if delta is not None:
+            self._check_spec(delta, n, prec)
         if out.dim() != 4 or out.shape[0] < nshell:
             raise ValueError("shell output must hold nshell spectra")
         self._check_spec(out[:nshell], n, prec, nshell)

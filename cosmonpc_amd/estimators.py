@@ -144,30 +144,41 @@ def _fit_block(k_bins, bytes_per_bin, ctx, reserve=0.25):
     return int(max(1, min(k_bins, budget // max(bytes_per_bin, 1))))
 
 
-def _bk_signal(ctx, stat_attrs, ca, cb, G, kinfo, mcfg, tjv, ell_1, ell_2, tracer_type, correlation_mode, mode,
-               block_size, dvm):
-    """total_res of the signal loop (``clustering_estimator.py:867-1061``)."""
+def _weight_components(ell, m, conj_w):
+    """Real shell components of Y_lm (or of conj(Y_lm) = (-1)^m Y_l,-m)."""
+    if not conj_w:
+        return A.shell_components(ell, m)
+    return [(p, (-1) ** m * ph) for p, ph in A.shell_components(ell, -m)]
+
+
+def _bk_signal(ctx, src1, src2, G, kinfo, mcfg, tjv, ell_1, ell_2, sym_allowed, mode, block_size, dvm,
+               conj_w=False):
+    """Sum over magnetic configs of 3j * sum_x G F_i F'_j with the reference's
+    symmetry fill and 2 Re accumulation (``clustering_estimator.py:867-1061``;
+    with ``conj_w`` and null sources, the S3 loop of ``:1219-1376``).
+
+    F_i = c2r(w1(khat) src1 mask_i), F'_j = c2r(w2(khat) src2 mask_j) with
+    w = Y (or conj Y); src None means a unit spectrum."""
     n, L, prec, k_edge, k_bins = kinfo
     N3 = n ** 3
     builder = ShellBuilder(ctx, n, L, prec, k_edge)
-    same_tracer = correlation_mode == "auto" or tracer_type == "aab"
     same_ell = ell_1 == ell_2
     if dvm == "full":
         total = np.zeros((k_bins, k_bins), dtype=np.complex128)
     else:
         total = np.zeros(k_bins, dtype=np.complex128)
+    ca = src1
     for ss, (m1, m2, M) in enumerate(mcfg):
-        sym_eq = same_tracer and same_ell and m1 == m2
-        sym_cj = same_tracer and same_ell and m1 == -m2
+        sym_eq = sym_allowed and same_ell and m1 == m2
+        sym_cj = sym_allowed and same_ell and m1 == -m2
         use_sym = sym_eq or sym_cj
-        comps1 = A.shell_components(ell_1, m1)
-        src2 = ca if tracer_type in ("aaa", "aab") else cb
+        comps1 = _weight_components(ell_1, m1, conj_w)
         if use_sym:
             sign = (-1) ** (m1 + ell_1)
             # F' = F (equal) or (-1)^(m+l) conj(F): same real shells, new phases
             comps2 = [(p, ph if sym_eq else sign * np.conj(ph)) for p, ph in comps1]
         else:
-            comps2 = A.shell_components(ell_2, m2)
+            comps2 = _weight_components(ell_2, m2, conj_w)
         nc1, nc2 = len(comps1), len(comps2)
         # resident shells per block: nc1 * bs (+ nc2 * bs off the diagonal)
         shell_bytes = N3 * (8 if prec == 0 else 4)
@@ -230,6 +241,28 @@ def _bk_signal(ctx, stat_attrs, ca, cb, G, kinfo, mcfg, tjv, ell_1, ell_2, trace
     return total
 
 
+def bk_box_g00(ctx, delta, n, prec):
+    """G_00 = Re c2r(delta_k) (``clustering_estimator.py:844-851``) as a flat
+    real mesh; ``delta`` is left untouched."""
+    tmp = delta.clone()
+    G = ctx.mesh(n, prec)
+    ctx.c2r(tmp, G, n, prec)
+    del tmp
+    return G.view(-1)
+
+
+def bk_box_signal_auto(ctx, delta, G, n, boxsize, prec, k_edge, angu_config, contraction, dvm="full",
+                       block_size="full"):
+    """Signal stage of an auto-correlation box bispectrum: from the
+    compensated delta_k and G_00 resident in HBM to ``total_res``
+    (``clustering_estimator.py:867-1061``) for one angular configuration."""
+    ell_1, ell_2, L = angu_config
+    mcfg, tjv = A.get_magnetic_configs_box(ell_1, ell_2, L)
+    k_bins = len(k_edge) - 1
+    return _bk_signal(ctx, delta, delta, G, (n, boxsize, prec, np.asarray(k_edge), k_bins), mcfg, tjv, ell_1,
+                      ell_2, True, _gemm_mode(contraction, prec), block_size, dvm)
+
+
 def calculate_bk_sugi_box(rfield_a, rfield_b, rfield_c, correlation_mode, stat_attrs, comm=None, **kwargs):
     """Sugiyama bispectrum multipole of a periodic box
     (``clustering_estimator.py:749-1555``)."""
@@ -275,17 +308,18 @@ def calculate_bk_sugi_box(rfield_a, rfield_b, rfield_c, correlation_mode, stat_a
           if (correlation_mode == "cross" and tracer_type == "abc") else None)
     # G_00 = Re c2r(delta_k) of the third leg (:844-851)
     gsrc = ca if correlation_mode == "auto" else (cc if tracer_type == "abc" else cb)
-    tmp = gsrc.clone()
-    G = ctx.mesh(n, prec)
-    ctx.c2r(tmp, G, n, prec)
-    del tmp
+    G = bk_box_g00(ctx, gsrc, n, prec)
     count, ksum, _, _ = ctx.bin(None, None, n, Lbox, prec, k_edge)
     k_num = count
     with np.errstate(invalid="ignore", divide="ignore"):
         k_eff = ksum / k_num
     t1 = _sync_time(ctx)
-    total_res = _bk_signal(ctx, stat_attrs, ca, cb, G.view(-1), (n, Lbox, prec, k_edge, k_bins), mcfg, tjv,
-                           ell_1, ell_2, tracer_type, correlation_mode, contraction, block_size, dvm)
+    # leg 2 shells come from tracer a for aaa/aab, else b (:899-902)
+    src2 = ca if tracer_type in ("aaa", "aab") else cb
+    same_tracer = correlation_mode == "auto" or tracer_type == "aab"
+    kinfo = (n, Lbox, prec, k_edge, k_bins)
+    total_res = _bk_signal(ctx, ca, src2, G.view(-1), kinfo, mcfg, tjv, ell_1, ell_2, same_tracer, contraction,
+                           block_size, dvm)
     t2 = _sync_time(ctx)
     LAST_TIMINGS.update({"prepare_s": t1 - t0, "signal_s": t2 - t1, "contraction": contraction})
     del G
@@ -360,10 +394,33 @@ def calculate_bk_sugi_box(rfield_a, rfield_b, rfield_c, correlation_mode, stat_a
                             s3[i, j] = numer / denom
             return s3 * (H_ells * N_ells)
 
+        def s3_fft():
+            """S3 from shells of conj(Y) * mask contracted with the source
+            field c2r(V^2 a conj(b) - SN0 C_p/sinc^2p)  (:1182-1384)."""
+            if tracer_type == "abb":
+                return np.zeros((k_bins, k_bins) if dvm == "full" else k_bins, dtype="complex128")
+            spec = ctx.spectrum(n, prec)
+            ctx.cross_spectrum(ca, ca if correlation_mode == "auto" else cb, spec, n, prec, boxsize.prod() ** 2, p,
+                               SN0 if correlation_mode == "auto" else 0.0)
+            S = ctx.mesh(n, prec)
+            ctx.c2r(spec, S, n, prec)
+            del spec
+            tot = _bk_signal(ctx, None, None, S.view(-1), kinfo, mcfg, tjv, ell_1, ell_2, True, contraction,
+                             block_size, dvm, conj_w=True)
+            del S
+            kk = k_num.astype(np.float64)
+            with np.errstate(invalid="ignore", divide="ignore"):
+                norm = np.outer(kk, kk) if dvm == "full" else kk ** 2
+                tot = np.where(norm > 0, tot / np.where(norm > 0, norm, 1.0), 0.0)
+            return tot * (H_ells * N_ells) * (1 / nmesh.prod())
+
         if sn_mode == "ana":
             SN3 = s3_ana()
-        elif sn_mode in ("fft", "both"):
-            raise NotImplementedError("shotnoise_mode 'fft' is not available yet on the GPU path")
+        elif sn_mode == "fft":
+            SN3 = s3_fft()
+        elif sn_mode == "both":
+            SN3 = s3_ana()
+            SN3_fft = s3_fft()
         else:
             raise ValueError("shotnoise_mode must be either 'ana', 'fft', or 'both'.")
 

@@ -95,10 +95,17 @@ int cnpc_bin_spectrum(cnpc_ctx* ctx, int nmesh, double boxsize, int prec, const 
                       const double* edges, int nbins, int npoly, const cnpc_poly* polys, double ab_scale,
                       int noise_window, double noise_amp, double* out);
 
+/* out = ab_scale * a(k) conj(b(k)) - noise_amp * C_p/sinc^{2p}(k) on the half
+ * spectrum (noise_window p = 1..4).  Replaces the S3 source field of
+ * clustering_estimator.py:1197-1211 (before its c2r); out may alias a. */
+int cnpc_cross_spectrum(cnpc_ctx* ctx, int nmesh, int prec, const void* a, const void* b, double ab_scale,
+                        int noise_window, double noise_amp, void* out);
+
 /* Shell filter: for s in [0, nshell): out[s] = w(khat) * delta(k) *
  * [edges[bin0+s] <= |k| < edges[bin0+s+1]] (full half spectra, stride
- * N*N*(N/2+1) complex).  Replaces (ylm_weighted_cfield * mask_i) of
- * clustering_estimator.py:898-902, :934-970. */
+ * N*N*(N/2+1) complex); delta_k may be NULL (weight alone).  Replaces
+ * (ylm_weighted_cfield * mask_i) of clustering_estimator.py:898-902,
+ * :934-970 and the S3 shells conj(Y) * mask of :1213-1217. */
 int cnpc_shell_filter(cnpc_ctx* ctx, int nmesh, double boxsize, int prec, const void* delta_k, const double* edges,
                       int nbins, int bin0, int nshell, const cnpc_poly* weight, void* out);
 

@@ -166,6 +166,26 @@ def test_bk_box_diagonal_fp64_matches_oracle(gpu_ctx, angu):
     assert rel_to_max(got["Bk_raw"], ref["Bk_raw"]) < 1e-10
 
 
+@pytest.mark.parametrize("angu,dvm", [((0, 0, 0), "full"), ((2, 0, 2), "full"), ((1, 1, 0), "diagonal"),
+                                      ((2, 2, 2), "full")])
+def test_bk_box_s3_fft_matches_oracle(gpu_ctx, angu, dvm):
+    from cosmonpc_amd.estimators import calculate_bk_sugi_box
+    from cosmonpc_amd.mesh import paint_field
+
+    _, _, B = _oracle()
+    n, L = 32, 600.0
+    pos = catalog(30000, L, 11, ngrid=n)
+    w = np.ones(len(pos))
+    attrs, fa, _, _ = B.get_mesh_box({"a": (pos, w)}, [n] * 3, [L] * 3, "cic", True)
+    sa = dict(attrs)
+    sa.update(bk_attrs(n, L, 0.02, 0.14, 6, angu, dvm, "cic", True, sn="both", contraction="gemm_fp64"))
+    ref = B.calculate_bk_sugi_box(fa, None, None, "auto", sa)
+    got = calculate_bk_sugi_box(paint_field(gpu_ctx, pos, w, n, L, "cic", True, 0), None, None, "auto", sa)
+    for key in ("SN3", "SN3_FFT"):
+        assert rel_to_max(got["SN_terms"][key], ref["SN_terms"][key]) < 1e-10, key
+    assert rel_to_max(got["B_sugi"], ref["B_sugi"]) < 1e-10
+
+
 def test_bk_box_blocked_matches_full(gpu_ctx):
     got, ref = _bk_case(gpu_ctx, (0, 0, 0), "full", "pairwise", block_size=3)
     assert rel_to_max(got["Bk_raw"], ref["Bk_raw"]) < 1e-10
