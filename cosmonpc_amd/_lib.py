@@ -50,6 +50,7 @@ SIGNATURES = {
     "cnpc_prepare_spectrum": [_P, _I, _D, _I, _P, _P, _I, _I, _I, _D, _D, _I, _P],
     "cnpc_bin_spectrum": [_P, _I, _D, _I, _P, _P, _P, _I, _I, _P, _D, _I, _D, _P],
     "cnpc_cross_spectrum": [_P, _I, _I, _P, _P, _D, _I, _D, _P],
+    "cnpc_resample_spectrum": [_P, _I, _I, _I, _P, _I, _P],
     "cnpc_shell_filter": [_P, _I, _D, _I, _P, _P, _I, _I, _I, _P, _P],
     "cnpc_contract_pairs": [_P, _I, _L, _P, _P, _P, _I, _P, _P, _P],
     "cnpc_contract_gemm": [_P, _I, _L, _P, _P, _I, _P, _I, _I, _F, _F, _I, _P],

@@ -290,6 +290,16 @@ int cnpc_cross_spectrum(cnpc_ctx* ctx, int n, int prec, const void* A, const voi
   return 0;
 }
 
+int cnpc_resample_spectrum(cnpc_ctx* ctx, int n, int m, int prec, const void* in, int nmax, void* out) {
+  REQUIRE(ctx, "null context");
+  REQUIRE(n >= 2 && n % 2 == 0 && m >= 2 && m % 2 == 0, "resample: meshes must be even");
+  REQUIRE(prec == 0 || prec == 1, "resample: prec must be 0 or 1");
+  REQUIRE(nmax >= 0 && nmax < m / 2 && nmax < n / 2, "resample: nmax must be below both Nyquist indices");
+  REQUIRE(in && out && in != out, "resample: null or aliased buffers");
+  CHECK_HIP(launch_resample(prec, n, m, nmax, in, out, ctx->stream));
+  return 0;
+}
+
 int cnpc_shell_filter(cnpc_ctx* ctx, int n, double boxsize, int prec, const void* D, const double* edges, int nbins,
                       int bin0, int nshell, const cnpc_poly* weight, void* out) {
   REQUIRE(ctx, "null context");

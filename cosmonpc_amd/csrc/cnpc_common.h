@@ -151,6 +151,7 @@ hipError_t launch_bin(int prec, const BinArgs& a, const BinEdges& E, const PolyS
 hipError_t launch_shell_filter(int prec, const ShellArgs& a, const BinEdges& E, const PolySpec& W, const void* D,
                                void* out, hipStream_t st);
 hipError_t launch_cross_spectrum(int prec, const BinArgs& a, const void* A, const void* B, void* out, hipStream_t st);
+hipError_t launch_resample(int prec, int n, int m, int nmax, const void* in, void* out, hipStream_t st);
 hipError_t launch_triple_dot(int prec, const void* G, const void* A, const void* B, int64_t n, double* partial,
                              int nblocks, double* out, hipStream_t st);
 hipError_t launch_gemm(int mode, const void* G, const void* A, int64_t lda, int na, const void* B, int64_t ldb, int nb,

@@ -270,7 +270,8 @@ __global__ void __launch_bounds__(256) gemm_f64_kernel(GemmArgs g) {
       for (int bi = 0; bi < 2; ++bi)
         for (int bj = 0; bj < 2; ++bj)
           for (int j = 0; j < 4; ++j) {
-            const int r = 16 * bi + 4 * (lane >> 4) + j, c = 16 * bj + (lane & 15);
+            // v_mfma_f64_16x16x4: element j of lane l is D[l/16 + 4 j][l%16]
+            const int r = 16 * bi + (lane >> 4) + 4 * j, c = 16 * bj + (lane & 15);
             red[r * 32 + c] += acc[bi][bj][j];
           }
     __syncthreads();

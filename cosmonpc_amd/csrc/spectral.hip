@@ -217,6 +217,42 @@ __global__ void __launch_bounds__(256) shell_filter_kernel(ShellArgs a, BinEdges
   }
 }
 
+// ------------------------------------------------------------- resample
+// Copies the modes with |n_x|, |n_y|, n_z <= nmax of an N-grid half spectrum
+// into an M-grid half spectrum (zeros elsewhere).  With nmax < M/2 every
+// copied mode keeps its physical k, so an M-point c2r evaluates the same
+// band-limited field on the coarser M-grid.
+template <typename T>
+__global__ void __launch_bounds__(256) resample_kernel(int n, int m, int nmax, const typename cplx<T>::type* __restrict__ in,
+                                                       typename cplx<T>::type* __restrict__ out) {
+  const int mzh = m / 2 + 1, nzh = n / 2 + 1;
+  const int64_t total = (int64_t)m * m * mzh;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (int64_t)gridDim.x * blockDim.x) {
+    const int iz = (int)(idx % mzh);
+    const int64_t r = idx / mzh;
+    const int iy = (int)(r % m), ix = (int)(r / m);
+    const int nx = wrap_freq(ix, m), ny = wrap_freq(iy, m), nz = wrap_freq(iz, m);
+    typename cplx<T>::type v;
+    v.x = (T)0;
+    v.y = (T)0;
+    if (nx <= nmax && nx >= -nmax && ny <= nmax && ny >= -nmax && nz >= 0 && nz <= nmax) {
+      const int jx = nx < 0 ? nx + n : nx, jy = ny < 0 ? ny + n : ny;
+      v = in[((int64_t)jx * n + jy) * nzh + nz];
+    }
+    out[idx] = v;
+  }
+}
+
+hipError_t launch_resample(int prec, int n, int m, int nmax, const void* in, void* out, hipStream_t st) {
+  const int64_t total = (int64_t)m * m * (m / 2 + 1);
+  dim3 blk(256), grd(grid_for(total, 256, 1 << 16));
+  if (prec == 0)
+    hipLaunchKernelGGL(resample_kernel<double>, grd, blk, 0, st, n, m, nmax, (const double2*)in, (double2*)out);
+  else
+    hipLaunchKernelGGL(resample_kernel<float>, grd, blk, 0, st, n, m, nmax, (const float2*)in, (float2*)out);
+  return hipGetLastError();
+}
+
 // ------------------------------------------------------------- cross spectrum
 // out = ab_scale * a conj(b) - noise_amp * C_p/sinc^{2p}(k): the S3 source
 // spectrum of clustering_estimator.py:1197-1211 (before its c2r).

@@ -133,6 +133,16 @@ class GpuContext:
         check(self.lib.cnpc_cross_spectrum(self.h, n, prec, _ptr(a), _ptr(b), float(ab_scale), int(noise_window),
                                            float(noise_amp), _ptr(out)), "cnpc_cross_spectrum")
 
+    def resample(self, spec, n, m, prec, nmax, out=None):
+        """m-grid half spectrum holding the |n_d| <= nmax modes of ``spec``."""
+        self._check_spec(spec, n, prec)
+        if out is None:
+            out = self.spectrum(m, prec)
+        self._check_spec(out, m, prec)
+        check(self.lib.cnpc_resample_spectrum(self.h, n, m, prec, _ptr(spec), int(nmax), _ptr(out)),
+              "cnpc_resample_spectrum")
+        return out
+
     def shell_filter(self, delta, n, boxsize, prec, edges, bin0, nshell, poly, out):
         """``delta`` None filters the weight alone."""
         edges = np.ascontiguousarray(edges, dtype=np.float64)

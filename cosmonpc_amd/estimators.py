@@ -241,6 +241,50 @@ def _bk_signal(ctx, src1, src2, G, kinfo, mcfg, tjv, ell_1, ell_2, sym_allowed, 
     return total
 
 
+def _nice_size(m):
+    """Smallest m' >= m with m' % 4 == 0 and prime factors in {2, 3, 5, 7}."""
+    while True:
+        if m % 4 == 0:
+            r = m
+            for p in (2, 3, 5, 7):
+                while r % p == 0:
+                    r //= p
+            if r == 1:
+                return m
+        m += 1
+
+
+def contraction_grid(n, boxsize, k_max, contraction, shell_grid="reduced"):
+    """Grid for the shell contraction.  Shells hold |n_d| <= R = ceil(k_max/kf)
+    and only G modes with |n_d| <= 2R meet them in a triangle, so
+    sum_x G F_i F_j over the n-grid equals (n/m)^3 times the same sum over any
+    m-grid with m >= 4R + 2 (no aliasing of n1 + n2 + n3 = 0).  Returns
+    (m, nmax) or (n, None) when the reduced grid is not smaller."""
+    if shell_grid == "full" or contraction == "pairwise":
+        return n, None
+    kf = 2 * math.pi / float(boxsize)
+    R = int(math.ceil(k_max / kf))
+    m = _nice_size(4 * R + 2)
+    if m >= n:
+        return n, None
+    return m, 2 * R
+
+
+def reduce_spectra(ctx, specs, n, prec, m, nmax):
+    """Resample distinct spectra onto the m-grid (None passes through)."""
+    out, seen = [], {}
+    for s in specs:
+        if s is None:
+            out.append(None)
+        elif id(s) in seen:
+            out.append(seen[id(s)])
+        else:
+            r = ctx.resample(s, n, m, prec, nmax)
+            seen[id(s)] = r
+            out.append(r)
+    return out
+
+
 def bk_box_g00(ctx, delta, n, prec):
     """G_00 = Re c2r(delta_k) (``clustering_estimator.py:844-851``) as a flat
     real mesh; ``delta`` is left untouched."""
@@ -308,7 +352,6 @@ def calculate_bk_sugi_box(rfield_a, rfield_b, rfield_c, correlation_mode, stat_a
           if (correlation_mode == "cross" and tracer_type == "abc") else None)
     # G_00 = Re c2r(delta_k) of the third leg (:844-851)
     gsrc = ca if correlation_mode == "auto" else (cc if tracer_type == "abc" else cb)
-    G = bk_box_g00(ctx, gsrc, n, prec)
     count, ksum, _, _ = ctx.bin(None, None, n, Lbox, prec, k_edge)
     k_num = count
     with np.errstate(invalid="ignore", divide="ignore"):
@@ -317,12 +360,20 @@ def calculate_bk_sugi_box(rfield_a, rfield_b, rfield_c, correlation_mode, stat_a
     # leg 2 shells come from tracer a for aaa/aab, else b (:899-902)
     src2 = ca if tracer_type in ("aaa", "aab") else cb
     same_tracer = correlation_mode == "auto" or tracer_type == "aab"
-    kinfo = (n, Lbox, prec, k_edge, k_bins)
-    total_res = _bk_signal(ctx, ca, src2, G.view(-1), kinfo, mcfg, tjv, ell_1, ell_2, same_tracer, contraction,
-                           block_size, dvm)
+    # the contraction runs on the reduced grid m (exact up to rounding)
+    m, nmax = contraction_grid(n, Lbox, k_max, contraction, stat_attrs.get("shell_grid", "reduced"))
+    if m != n:
+        ca_m, src2_m, gsrc_m = reduce_spectra(ctx, [ca, src2, gsrc], n, prec, m, nmax)
+    else:
+        ca_m, src2_m, gsrc_m = ca, src2, gsrc
+    grid_factor = (n / m) ** 3
+    G = bk_box_g00(ctx, gsrc_m, m, prec)
+    kinfo = (m, Lbox, prec, k_edge, k_bins)
+    total_res = grid_factor * _bk_signal(ctx, ca_m, src2_m, G, kinfo, mcfg, tjv, ell_1, ell_2, same_tracer,
+                                         contraction, block_size, dvm)
     t2 = _sync_time(ctx)
-    LAST_TIMINGS.update({"prepare_s": t1 - t0, "signal_s": t2 - t1, "contraction": contraction})
-    del G
+    LAST_TIMINGS.update({"prepare_s": t1 - t0, "signal_s": t2 - t1, "contraction": contraction, "grid": m})
+    del G, ca_m, src2_m, gsrc_m
 
     # ---------------------------------------------------------- shot noise
     k_center = 0.5 * (k_edge[1:] + k_edge[:-1])
@@ -402,11 +453,13 @@ def calculate_bk_sugi_box(rfield_a, rfield_b, rfield_c, correlation_mode, stat_a
             spec = ctx.spectrum(n, prec)
             ctx.cross_spectrum(ca, ca if correlation_mode == "auto" else cb, spec, n, prec, boxsize.prod() ** 2, p,
                                SN0 if correlation_mode == "auto" else 0.0)
-            S = ctx.mesh(n, prec)
-            ctx.c2r(spec, S, n, prec)
+            if m != n:
+                spec = ctx.resample(spec, n, m, prec, nmax)
+            S = ctx.mesh(m, prec)
+            ctx.c2r(spec, S, m, prec)
             del spec
-            tot = _bk_signal(ctx, None, None, S.view(-1), kinfo, mcfg, tjv, ell_1, ell_2, True, contraction,
-                             block_size, dvm, conj_w=True)
+            tot = grid_factor * _bk_signal(ctx, None, None, S.view(-1), kinfo, mcfg, tjv, ell_1, ell_2, True,
+                                           contraction, block_size, dvm, conj_w=True)
             del S
             kk = k_num.astype(np.float64)
             with np.errstate(invalid="ignore", divide="ignore"):

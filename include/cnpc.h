@@ -101,6 +101,14 @@ int cnpc_bin_spectrum(cnpc_ctx* ctx, int nmesh, double boxsize, int prec, const 
 int cnpc_cross_spectrum(cnpc_ctx* ctx, int nmesh, int prec, const void* a, const void* b, double ab_scale,
                         int noise_window, double noise_amp, void* out);
 
+/* Copy the modes |n_x|, |n_y|, n_z <= nmax (nmax < m/2, n/2) of an n-grid half
+ * spectrum into an m-grid half spectrum, zero elsewhere.  A band-limited
+ * triple product sum_x G F_i F_j is the same on any grid m >= 4R+2 when
+ * the shells F have |n_d| <= R and G is cut at 2R, up to the factor (n/m)^3,
+ * so the bispectrum contraction of clustering_estimator.py:972 can run on
+ * the smaller grid.  No reference counterpart (pmesh keeps one grid). */
+int cnpc_resample_spectrum(cnpc_ctx* ctx, int n, int m, int prec, const void* in, int nmax, void* out);
+
 /* Shell filter: for s in [0, nshell): out[s] = w(khat) * delta(k) *
  * [edges[bin0+s] <= |k| < edges[bin0+s+1]] (full half spectra, stride
  * N*N*(N/2+1) complex); delta_k may be NULL (weight alone).  Replaces

@@ -125,8 +125,11 @@ def test_pk_box_cross_matches_oracle(gpu_ctx):
         assert rel_to_max(got[f"P{p}"], ref[f"P{p}"]) < 1e-10
 
 
-def _bk_case(gpu_ctx, angu, dvm, contraction, prec=0, sampler="tsc", interlaced=True, n=32, L=600.0,
-             k=(0.02, 0.14, 8), npart=30000, seed=11, block_size=None):
+def _bk_case(gpu_ctx, angu, dvm, contraction, prec=0, sampler="tsc", interlaced=True, n=32, L=300.0,
+             k=(0.03, 0.15, 6), npart=30000, seed=11, block_size=None):
+    # run_task requires 2 k_max < k_Nyq (task_executor.py:192-196); the
+    # half-spectrum path relies on it, so every case respects it
+    assert 2 * k[1] < np.pi * n / L
     from cosmonpc_amd.estimators import calculate_bk_sugi_box
     from cosmonpc_amd.mesh import paint_field
 
@@ -173,16 +176,46 @@ def test_bk_box_s3_fft_matches_oracle(gpu_ctx, angu, dvm):
     from cosmonpc_amd.mesh import paint_field
 
     _, _, B = _oracle()
-    n, L = 32, 600.0
+    n, L = 32, 300.0
     pos = catalog(30000, L, 11, ngrid=n)
     w = np.ones(len(pos))
     attrs, fa, _, _ = B.get_mesh_box({"a": (pos, w)}, [n] * 3, [L] * 3, "cic", True)
     sa = dict(attrs)
-    sa.update(bk_attrs(n, L, 0.02, 0.14, 6, angu, dvm, "cic", True, sn="both", contraction="gemm_fp64"))
+    sa.update(bk_attrs(n, L, 0.03, 0.15, 6, angu, dvm, "cic", True, sn="both", contraction="gemm_fp64"))
     ref = B.calculate_bk_sugi_box(fa, None, None, "auto", sa)
     got = calculate_bk_sugi_box(paint_field(gpu_ctx, pos, w, n, L, "cic", True, 0), None, None, "auto", sa)
     for key in ("SN3", "SN3_FFT"):
         assert rel_to_max(got["SN_terms"][key], ref["SN_terms"][key]) < 1e-10, key
+    assert rel_to_max(got["B_sugi"], ref["B_sugi"]) < 1e-10
+
+
+@pytest.mark.parametrize("angu", [(0, 0, 0), (2, 0, 2), (2, 2, 0)])
+def test_bk_box_reduced_grid_matches_oracle(gpu_ctx, angu):
+    """Contraction on the reduced grid (64^3 -> 36^3 here) is the same sum."""
+    from cosmonpc_amd.estimators import LAST_TIMINGS, contraction_grid
+
+    assert contraction_grid(64, 1000.0, 0.05, "gemm_fp64")[0] == 36
+    got, ref = _bk_case(gpu_ctx, angu, "full", "gemm_fp64", n=64, L=1000.0, k=(0.01, 0.05, 4), npart=40000,
+                        sampler="cic")
+    assert LAST_TIMINGS["grid"] == 36
+    assert rel_to_max(got["Bk_raw"], ref["Bk_raw"]) < 1e-10
+    assert rel_to_max(got["B_sugi"], ref["B_sugi"]) < 1e-10
+
+
+def test_bk_box_reduced_grid_s3_fft(gpu_ctx):
+    from cosmonpc_amd.estimators import calculate_bk_sugi_box
+    from cosmonpc_amd.mesh import paint_field
+
+    _, _, B = _oracle()
+    n, L = 64, 1000.0
+    pos = catalog(40000, L, 12, ngrid=n)
+    w = np.ones(len(pos))
+    attrs, fa, _, _ = B.get_mesh_box({"a": (pos, w)}, [n] * 3, [L] * 3, "tsc", True)
+    sa = dict(attrs)
+    sa.update(bk_attrs(n, L, 0.01, 0.05, 4, (0, 0, 0), "full", "tsc", True, sn="fft", contraction="gemm_fp64"))
+    ref = B.calculate_bk_sugi_box(fa, None, None, "auto", sa)
+    got = calculate_bk_sugi_box(paint_field(gpu_ctx, pos, w, n, L, "tsc", True, 0), None, None, "auto", sa)
+    assert rel_to_max(got["SN_terms"]["SN3"], ref["SN_terms"]["SN3"]) < 1e-10
     assert rel_to_max(got["B_sugi"], ref["B_sugi"]) < 1e-10
 
 
@@ -194,7 +227,7 @@ def test_bk_box_blocked_matches_full(gpu_ctx):
 @pytest.mark.parametrize("angu", [(0, 0, 0), (2, 0, 2)])
 @pytest.mark.parametrize("contraction", ["bf16x3", "fp16x3"])
 def test_bk_box_fp32_split_error(gpu_ctx, angu, contraction):
-    got, ref = _bk_case(gpu_ctx, angu, "full", contraction, prec=1, n=48, L=800.0, k=(0.02, 0.18, 10),
+    got, ref = _bk_case(gpu_ctx, angu, "full", contraction, prec=1, n=48, L=800.0, k=(0.01, 0.09, 8),
                         npart=60000)
     err = rel_to_max(got["Bk_raw"], ref["Bk_raw"])
     _log(f"{contraction} {angu} error rel max|B| = {err:.3e}")
