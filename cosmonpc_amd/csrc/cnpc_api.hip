@@ -49,9 +49,12 @@ struct FftPlan {
 struct cnpc_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
-  std::map<std::tuple<int, int, int, int>, FftPlan> plans;
-  void* scratch = nullptr;
+  // key: (n, batch, prec, direction, dims)
+  std::map<std::tuple<int, int, int, int, int>, FftPlan> plans;
+  void* scratch = nullptr;  // reduction partials
   size_t scratch_bytes = 0;
+  void* synth = nullptr;    // shell-synthesis cube, bins and twiddles
+  size_t synth_bytes = 0;
 };
 
 #define CHECK_HIP(expr)                                                                        \
@@ -84,8 +87,23 @@ static int ensure_scratch(cnpc_ctx* ctx, size_t bytes) {
   return 0;
 }
 
-static int get_plan(cnpc_ctx* ctx, int n, int batch, int prec, int dir, FftPlan** out) {
-  auto key = std::make_tuple(n, batch, prec, dir);
+static int ensure_synth(cnpc_ctx* ctx, size_t bytes) {
+  if (ctx->synth_bytes >= bytes) return 0;
+  if (ctx->synth) {
+    CHECK_HIP(hipStreamSynchronize(ctx->stream));
+    CHECK_HIP(hipFree(ctx->synth));
+    ctx->synth = nullptr;
+    ctx->synth_bytes = 0;
+  }
+  CHECK_HIP(hipMalloc(&ctx->synth, bytes));
+  ctx->synth_bytes = bytes;
+  return 0;
+}
+
+// dims = 3: N^3 transforms; dims = 2: N^2 planes (the (y, z) pass of the
+// shell synthesis).  Real transforms, default packed layouts.
+static int get_plan(cnpc_ctx* ctx, int n, int batch, int prec, int dir, FftPlan** out, int dims = 3) {
+  auto key = std::make_tuple(n, batch, prec, dir, dims);
   auto it = ctx->plans.find(key);
   if (it != ctx->plans.end()) {
     *out = &it->second;
@@ -98,7 +116,7 @@ static int get_plan(cnpc_ctx* ctx, int n, int batch, int prec, int dir, FftPlan*
   rocfft_status s = rocfft_plan_create(
       &p.plan, rocfft_placement_notinplace,
       dir == 0 ? rocfft_transform_type_real_forward : rocfft_transform_type_real_inverse,
-      prec == 0 ? rocfft_precision_double : rocfft_precision_single, 3, lengths, (size_t)batch, nullptr);
+      prec == 0 ? rocfft_precision_double : rocfft_precision_single, (size_t)dims, lengths, (size_t)batch, nullptr);
   REQUIRE(s == rocfft_status_success, "rocfft_plan_create failed (status " + std::to_string((int)s) + ")");
   s = rocfft_plan_get_work_buffer_size(p.plan, &p.work_bytes);
   REQUIRE(s == rocfft_status_success, "rocfft_plan_get_work_buffer_size failed");
@@ -161,6 +179,7 @@ int cnpc_ctx_destroy(cnpc_ctx* ctx) {
   if (!ctx) return 0;
   int rc = cnpc_ctx_clear_plans(ctx);
   if (ctx->scratch) (void)hipFree(ctx->scratch);
+  if (ctx->synth) (void)hipFree(ctx->synth);
   delete ctx;
   return rc;
 }
@@ -211,13 +230,13 @@ int cnpc_paint(cnpc_ctx* ctx, const double* pos, const double* w, int64_t np, in
   return 0;
 }
 
-static int run_fft(cnpc_ctx* ctx, int n, int batch, int prec, int dir, void* in, void* out) {
+static int run_fft(cnpc_ctx* ctx, int n, int batch, int prec, int dir, void* in, void* out, int dims = 3) {
   REQUIRE(ctx, "null context");
   REQUIRE(n >= 2 && n % 2 == 0 && batch >= 1, "fft: bad size");
   REQUIRE(prec == 0 || prec == 1, "fft: prec must be 0 or 1");
   REQUIRE(in && out && in != out, "fft: null or aliased buffers (out-of-place only)");
   FftPlan* p = nullptr;
-  if (int rc = get_plan(ctx, n, batch, prec, dir, &p)) return rc;
+  if (int rc = get_plan(ctx, n, batch, prec, dir, &p, dims)) return rc;
   rocfft_status s = rocfft_execution_info_set_stream(p->info, ctx->stream);
   REQUIRE(s == rocfft_status_success, "rocfft_execution_info_set_stream failed");
   void* ib[1] = {in};
@@ -288,6 +307,40 @@ int cnpc_cross_spectrum(cnpc_ctx* ctx, int n, int prec, const void* A, const voi
   a.kf = 0.0; a.ab_scale = ab_scale; a.noise_amp = noise_amp;
   CHECK_HIP(launch_cross_spectrum(prec, a, A, B, out, ctx->stream));
   return 0;
+}
+
+int cnpc_shell_synth(cnpc_ctx* ctx, int n, double boxsize, int prec, const void* D, const double* edges, int nbins,
+                     int bin0, int nshell, const cnpc_poly* weight, void* work, void* out) {
+  REQUIRE(ctx, "null context");
+  REQUIRE(n >= 2 && n % 2 == 0 && boxsize > 0, "shell_synth: bad mesh");
+  REQUIRE(prec == 0 || prec == 1, "shell_synth: prec must be 0 or 1");
+  REQUIRE(nbins >= 1 && nbins <= 127 && edges, "shell_synth: nbins must be in [1, 127]");
+  REQUIRE(bin0 >= 0 && nshell >= 1 && bin0 + nshell <= nbins, "shell_synth: shell range outside the bins");
+  REQUIRE(work && out && weight && work != out, "shell_synth: null or aliased buffers");
+  REQUIRE(weight->deg >= 0 && weight->deg < CNPC_ZDEG && weight->m >= 0 && weight->part >= 0 && weight->part <= 3,
+          "shell_synth: bad weight polynomial");
+  for (int i = 0; i < nbins; ++i) REQUIRE(edges[i + 1] > edges[i] && edges[i] >= 0, "shell_synth: bad edges");
+  const double kf = 2.0 * M_PI / boxsize;
+  // every mode with |k| < edges[nbins] has |n_d| <= R
+  const int R = (int)std::ceil(edges[nbins] / kf);
+  REQUIRE(R < n / 2, "shell_synth: k_max reaches the Nyquist index (needs edges[nbins] < pi n / L)");
+  int zb = 32;
+  while (zb > 8 && xsynth_lds_bytes(prec, n, R, nshell, zb) > 64 * 1024) zb /= 2;
+  REQUIRE(xsynth_lds_bytes(prec, n, R, nshell, zb) <= 64 * 1024, "shell_synth: k_max too large for the LDS tile");
+  const size_t cs = prec == 0 ? 16 : 8;
+  const size_t P = 2 * (size_t)R + 1, cube = P * (R + 1) * P;
+  const size_t vbytes = (cube * cs + 255) / 256 * 256, bbytes = (cube + 255) / 256 * 256;
+  if (int rc = ensure_synth(ctx, vbytes + bbytes + (size_t)n * cs)) return rc;
+  BinEdges E;
+  fill_edges(E, edges, nbins);
+  PolySpec W;
+  to_spec(W, *weight);
+  SynthArgs a;
+  a.n = n; a.R = R; a.bin0 = bin0; a.nshell = nshell; a.kf = kf;
+  char* base = (char*)ctx->synth;
+  CHECK_HIP(launch_synth(prec, a, E, W, D, base, (signed char*)(base + vbytes), base + vbytes + bbytes, work, zb,
+                         ctx->stream));
+  return run_fft(ctx, n, nshell * n, prec, 1, work, out, 2);
 }
 
 int cnpc_resample_spectrum(cnpc_ctx* ctx, int n, int m, int prec, const void* in, int nmax, void* out) {

@@ -143,6 +143,18 @@ struct ShellArgs {
   int64_t stride;  // complex elements between consecutive output shells
 };
 
+struct SynthArgs {
+  int n;        // grid
+  int R;        // |n_d| <= R holds every shell mode
+  int bin0;     // first shell
+  int nshell;   // shells in this call
+  double kf;
+};
+
+size_t xsynth_lds_bytes(int prec, int n, int R, int nshell, int zb);
+hipError_t launch_synth(int prec, const SynthArgs& a, const BinEdges& E, const PolySpec& W, const void* D, void* V,
+                        signed char* bins, void* tw, void* out, int zb, hipStream_t st);
+
 hipError_t launch_paint(const double* pos, const double* w, int64_t np, int n, double boxsize, int order,
                         const double* center, int ncopy, int prec, void* m0, void* m1, hipStream_t st);
 hipError_t launch_prepare(int prec, const PrepareArgs& a, const void* c1, const void* c2, void* out, hipStream_t st);

@@ -280,16 +280,19 @@ __global__ void __launch_bounds__(256) gemm_f64_kernel(GemmArgs g) {
   for (int i = threadIdx.x; i < 1024; i += blockDim.x) out[i] = red[i];
 }
 
-// partial [nblk][32][32] -> C[(a0 + r) * ldc + b0 + c] for valid rows
-__global__ void gemm_finalize_kernel(const double* __restrict__ partial, int nblk, int a0, int b0, int na, int nb,
-                                     double inv_scale, double* __restrict__ C, int ldc) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= 1024) return;
+// partial [nblk][32][32] -> C[(a0 + r) * ldc + b0 + c] for valid rows; one
+// wave per output entry, lanes stride over the partials, fixed-order
+// butterfly reduction (deterministic)
+__global__ void __launch_bounds__(64) gemm_finalize_kernel(const double* __restrict__ partial, int nblk, int a0, int b0,
+                                                           int na, int nb, double inv_scale, double* __restrict__ C,
+                                                           int ldc) {
+  const int t = blockIdx.x;  // 0..1023
   const int r = t >> 5, c = t & 31;
   if (a0 + r >= na || b0 + c >= nb) return;
   double s = 0.0;
-  for (int p = 0; p < nblk; ++p) s += partial[(int64_t)p * 1024 + t];
-  C[(int64_t)(a0 + r) * ldc + (b0 + c)] = s * inv_scale;
+  for (int p = threadIdx.x; p < nblk; p += 64) s += partial[(int64_t)p * 1024 + t];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  if (threadIdx.x == 0) C[(int64_t)(a0 + r) * ldc + (b0 + c)] = s * inv_scale;
 }
 
 // mode: 0 bf16x3, 1 f16x3 (FP32 storage); 2 f64 (FP64 storage)
@@ -312,7 +315,8 @@ hipError_t launch_gemm(int mode, const void* G, const void* A, int64_t lda, int 
       else
         hipLaunchKernelGGL(gemm_f64_kernel, dim3(nblk_eff), dim3(256), 0, st, g);
       const double inv = 1.0 / ((double)sa * (double)sb);
-      hipLaunchKernelGGL(gemm_finalize_kernel, dim3(4), dim3(256), 0, st, partial, nblk_eff, a0, b0, na, nb, inv, C, nb);
+      hipLaunchKernelGGL(gemm_finalize_kernel, dim3(1024), dim3(64), 0, st, partial, nblk_eff, a0, b0, na, nb, inv, C,
+                         nb);
     }
   }
   return hipGetLastError();

@@ -143,6 +143,18 @@ class GpuContext:
               "cnpc_resample_spectrum")
         return out
 
+    def shell_synth(self, delta, n, boxsize, prec, edges, bin0, nshell, poly, work, out):
+        """Real shells out[s] (nshell, n, n, n) by the pruned synthesis;
+        ``work`` holds nshell half-spectrum planes stacks."""
+        edges = np.ascontiguousarray(edges, dtype=np.float64)
+        if delta is not None:
+            self._check_spec(delta, n, prec)
+        self._check_spec(work[:nshell], n, prec, nshell)
+        self._check_mesh(out, n, prec, nshell)
+        check(self.lib.cnpc_shell_synth(self.h, n, float(boxsize), prec, _ptr(delta),
+                                        edges.ctypes.data_as(ctypes.c_void_p), len(edges) - 1, int(bin0), int(nshell),
+                                        ctypes.byref(poly), _ptr(work), _ptr(out)), "cnpc_shell_synth")
+
     def shell_filter(self, delta, n, boxsize, prec, edges, bin0, nshell, poly, out):
         """``delta`` None filters the weight alone."""
         edges = np.ascontiguousarray(edges, dtype=np.float64)

@@ -78,10 +78,17 @@ class ShellBuilder:
     """Builds real shell fields  R = c2r(w(khat) * delta * 1[k in bin])  for a
     run of bins, batching the shell filter and the inverse FFTs."""
 
-    def __init__(self, ctx, n, boxsize, prec, edges, batch=4):
+    def __init__(self, ctx, n, boxsize, prec, edges, batch=4, method="synth"):
         self.ctx, self.n, self.L, self.prec = ctx, n, boxsize, prec
         self.edges = np.asarray(edges, dtype=np.float64)
         self.batch = batch
+        if method not in ("synth", "filter"):
+            raise ValueError("shell_method must be 'synth' or 'filter'")
+        # the pruned synthesis needs |n_d| <= R < n/2 and bins that fit int8
+        R = math.ceil(self.edges[-1] / (2 * math.pi / boxsize))
+        if method == "synth" and (len(self.edges) - 1 > 127 or R >= n // 2):
+            method = "filter"
+        self.method = method
         self.work = ctx.spectrum(n, prec, batch)
 
     def build(self, delta, poly, bins, out):
@@ -93,8 +100,12 @@ class ShellBuilder:
         done = 0
         while done < nb:
             c = min(self.batch, nb - done)
-            self.ctx.shell_filter(delta, n, self.L, self.prec, self.edges, b0 + done, c, poly, self.work)
-            self.ctx.c2r(self.work[:c], out[done:done + c].view(c, n, n, n), n, self.prec, c)
+            dst = out[done:done + c].view(c, n, n, n)
+            if self.method == "synth":
+                self.ctx.shell_synth(delta, n, self.L, self.prec, self.edges, b0 + done, c, poly, self.work, dst)
+            else:
+                self.ctx.shell_filter(delta, n, self.L, self.prec, self.edges, b0 + done, c, poly, self.work)
+                self.ctx.c2r(self.work[:c], dst, n, self.prec, c)
             done += c
 
 
@@ -152,7 +163,7 @@ def _weight_components(ell, m, conj_w):
 
 
 def _bk_signal(ctx, src1, src2, G, kinfo, mcfg, tjv, ell_1, ell_2, sym_allowed, mode, block_size, dvm,
-               conj_w=False):
+               conj_w=False, shell_method="synth"):
     """Sum over magnetic configs of 3j * sum_x G F_i F'_j with the reference's
     symmetry fill and 2 Re accumulation (``clustering_estimator.py:867-1061``;
     with ``conj_w`` and null sources, the S3 loop of ``:1219-1376``).
@@ -161,7 +172,7 @@ def _bk_signal(ctx, src1, src2, G, kinfo, mcfg, tjv, ell_1, ell_2, sym_allowed, 
     w = Y (or conj Y); src None means a unit spectrum."""
     n, L, prec, k_edge, k_bins = kinfo
     N3 = n ** 3
-    builder = ShellBuilder(ctx, n, L, prec, k_edge)
+    builder = ShellBuilder(ctx, n, L, prec, k_edge, method=shell_method)
     same_ell = ell_1 == ell_2
     if dvm == "full":
         total = np.zeros((k_bins, k_bins), dtype=np.complex128)
@@ -324,6 +335,7 @@ def calculate_bk_sugi_box(rfield_a, rfield_b, rfield_c, correlation_mode, stat_a
     vol_per_cell = boxsize.prod() / nmesh.prod()
     n, Lbox, prec = rfield_a.n, rfield_a.boxsize, rfield_a.prec
     contraction = _gemm_mode(stat_attrs.get("contraction", "gemm_fp64" if prec == 0 else "bf16x3"), prec)
+    shell_method = stat_attrs.get("shell_method", "synth")
 
     N_ells = (2 * ell_1 + 1) * (2 * ell_2 + 1) * (2 * L + 1)
     H_ells = A.wigner_3j(ell_1, ell_2, L, 0, 0, 0)
@@ -370,7 +382,7 @@ def calculate_bk_sugi_box(rfield_a, rfield_b, rfield_c, correlation_mode, stat_a
     G = bk_box_g00(ctx, gsrc_m, m, prec)
     kinfo = (m, Lbox, prec, k_edge, k_bins)
     total_res = grid_factor * _bk_signal(ctx, ca_m, src2_m, G, kinfo, mcfg, tjv, ell_1, ell_2, same_tracer,
-                                         contraction, block_size, dvm)
+                                         contraction, block_size, dvm, shell_method=shell_method)
     t2 = _sync_time(ctx)
     LAST_TIMINGS.update({"prepare_s": t1 - t0, "signal_s": t2 - t1, "contraction": contraction, "grid": m})
     del G, ca_m, src2_m, gsrc_m
@@ -459,7 +471,8 @@ def calculate_bk_sugi_box(rfield_a, rfield_b, rfield_c, correlation_mode, stat_a
             ctx.c2r(spec, S, m, prec)
             del spec
             tot = grid_factor * _bk_signal(ctx, None, None, S.view(-1), kinfo, mcfg, tjv, ell_1, ell_2, True,
-                                           contraction, block_size, dvm, conj_w=True)
+                                           contraction, block_size, dvm, conj_w=True,
+                                           shell_method=shell_method)
             del S
             kk = k_num.astype(np.float64)
             with np.errstate(invalid="ignore", divide="ignore"):

@@ -117,6 +117,16 @@ int cnpc_resample_spectrum(cnpc_ctx* ctx, int n, int m, int prec, const void* in
 int cnpc_shell_filter(cnpc_ctx* ctx, int nmesh, double boxsize, int prec, const void* delta_k, const double* edges,
                       int nbins, int bin0, int nshell, const cnpc_poly* weight, void* out);
 
+/* Pruned shell synthesis: out[s] = c2r(w(khat) delta(k) 1[k in bin bin0+s])
+ * for s < nshell, the same real shells as cnpc_shell_filter followed by
+ * cnpc_fft_c2r, computed as a sparse x-direction DFT over the modes inside
+ * |n_d| <= R = ceil(edges[nbins] L / 2 pi) followed by a batched 2-D c2r
+ * over (y, z).  work: nshell*n*n*(n/2+1) complex; out: nshell*n^3 real;
+ * delta_k may be NULL (weight alone); nbins <= 127.  Replaces the shell IFFTs
+ * c2r(ylm_weighted_cfield * mask_i) of clustering_estimator.py:934-970. */
+int cnpc_shell_synth(cnpc_ctx* ctx, int nmesh, double boxsize, int prec, const void* delta_k, const double* edges,
+                     int nbins, int bin0, int nshell, const cnpc_poly* weight, void* work, void* out);
+
 /* Pairwise plain contraction: out[p] = sum_x G A_{ia[p]} B_{ib[p]}, one full
  * streaming pass per pair (the np.sum(G_00 * F_i * F_j) of
  * clustering_estimator.py:972, :1053).  Shell s of A is at A + s*ncell. out is

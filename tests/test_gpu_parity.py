@@ -126,7 +126,7 @@ def test_pk_box_cross_matches_oracle(gpu_ctx):
 
 
 def _bk_case(gpu_ctx, angu, dvm, contraction, prec=0, sampler="tsc", interlaced=True, n=32, L=300.0,
-             k=(0.03, 0.15, 6), npart=30000, seed=11, block_size=None):
+             k=(0.03, 0.15, 6), npart=30000, seed=11, block_size=None, shell_method="synth"):
     # run_task requires 2 k_max < k_Nyq (task_executor.py:192-196); the
     # half-spectrum path relies on it, so every case respects it
     assert 2 * k[1] < np.pi * n / L
@@ -134,11 +134,13 @@ def _bk_case(gpu_ctx, angu, dvm, contraction, prec=0, sampler="tsc", interlaced=
     from cosmonpc_amd.mesh import paint_field
 
     _, _, B = _oracle()
-    _log(f"bk {angu} {dvm} {contraction} prec={prec} {sampler} n={n}")
+    _log(f"bk {angu} {dvm} {contraction} prec={prec} {sampler} n={n} {shell_method}")
     pos = catalog(npart, L, seed, ngrid=n)
     w = np.ones(len(pos))
     attrs, fa, _, _ = B.get_mesh_box({"a": (pos, w)}, [n] * 3, [L] * 3, sampler, interlaced)
-    extra = {} if block_size is None else {"block_size": block_size}
+    extra = {"shell_method": shell_method}
+    if block_size is not None:
+        extra["block_size"] = block_size
     sa = dict(attrs)
     sa.update(bk_attrs(n, L, k[0], k[1], k[2], angu, dvm, sampler, interlaced, contraction=contraction, **extra))
     ref = B.calculate_bk_sugi_box(fa, None, None, "auto", sa)
@@ -217,6 +219,41 @@ def test_bk_box_reduced_grid_s3_fft(gpu_ctx):
     got = calculate_bk_sugi_box(paint_field(gpu_ctx, pos, w, n, L, "tsc", True, 0), None, None, "auto", sa)
     assert rel_to_max(got["SN_terms"]["SN3"], ref["SN_terms"]["SN3"]) < 1e-10
     assert rel_to_max(got["B_sugi"], ref["B_sugi"]) < 1e-10
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+@pytest.mark.parametrize("n,L,kmax", [(32, 300.0, 0.15), (40, 500.0, 0.2), (64, 1000.0, 0.12)])
+def test_shell_synth_matches_filter_c2r(gpu_ctx, prec, n, L, kmax):
+    """Pruned synthesis (sparse x-DFT + 2-D c2r) == shell filter + 3-D c2r,
+    for real, imaginary and conjugated Y_lm weights and a null spectrum."""
+    import torch
+
+    from cosmonpc_amd import angular as A
+
+    ctx = gpu_ctx
+    rng = np.random.default_rng(n)
+    x = torch.from_numpy(rng.standard_normal((n, n, n)).astype(np.float64 if prec == 0 else np.float32)).cuda()
+    D = ctx.spectrum(n, prec)
+    ctx.r2c(x, D, n, prec)
+    edges = np.linspace(0.01, kmax, 7)
+    nb = 6
+    work = ctx.spectrum(n, prec, nb)
+    for ell, m, part in [(0, 0, A.PART_RE), (2, 1, A.PART_IM), (3, 0, A.PART_RE), (4, -3, A.PART_RE)]:
+        poly = A.ylm_poly(ell, m, True, part, factor=(1j if ell % 2 else 1.0))
+        for src in (D, None):
+            a = ctx.mesh(n, prec, nb)
+            b = ctx.mesh(n, prec, nb)
+            ctx.shell_synth(src, n, L, prec, edges, 0, nb, poly, work, a)
+            ctx.shell_filter(src, n, L, prec, edges, 0, nb, poly, work)
+            ctx.c2r(work, b, n, prec, nb)
+            ah, bh = a.cpu().numpy(), b.cpu().numpy()
+            tol = 1e-12 if prec == 0 else 2e-5
+            assert np.max(np.abs(ah - bh)) <= tol * np.max(np.abs(bh)), (ell, m, src is None)
+
+
+def test_bk_box_filter_method_matches_oracle(gpu_ctx):
+    got, ref = _bk_case(gpu_ctx, (2, 0, 2), "full", "gemm_fp64", shell_method="filter")
+    assert rel_to_max(got["Bk_raw"], ref["Bk_raw"]) < 1e-10
 
 
 def test_bk_box_blocked_matches_full(gpu_ctx):
