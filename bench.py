@@ -138,6 +138,7 @@ def main():
         del f64
         res = {}
         for mode in ("pairwise", "gemm_fp64"):
+            signal_step(d64, 0, mode)  # warm-up: FFT plans and work areas
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             res[mode] = signal_step(d64, 0, mode)

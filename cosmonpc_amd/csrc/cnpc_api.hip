@@ -327,20 +327,34 @@ int cnpc_shell_synth(cnpc_ctx* ctx, int n, double boxsize, int prec, const void*
   int zb = 32;
   while (zb > 8 && xsynth_lds_bytes(prec, n, R, nshell, zb) > 64 * 1024) zb /= 2;
   REQUIRE(xsynth_lds_bytes(prec, n, R, nshell, zb) <= 64 * 1024, "shell_synth: k_max too large for the LDS tile");
+  REQUIRE(synth_lines_per_block(prec, n) >= 1, "shell_synth: mesh too large for the LDS line buffers");
+  SynthArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.n = n; a.R = R; a.bin0 = bin0; a.nshell = nshell; a.kf = kf;
+  {  // mixed-radix factorisation of n into 4, 2, 3, 5, 7
+    int m = n, k = 0;
+    while (m % 4 == 0 && k < CNPC_MAX_RADICES) { a.rad[k++] = 4; m /= 4; }
+    for (int r : {2, 3, 5, 7})
+      while (m % r == 0 && k < CNPC_MAX_RADICES) { a.rad[k++] = r; m /= r; }
+    REQUIRE(m == 1, "shell_synth: nmesh must factor into 2, 3, 5 and 7");
+    a.nrad = k;
+  }
   const size_t cs = prec == 0 ? 16 : 8;
-  const size_t P = 2 * (size_t)R + 1, cube = P * (R + 1) * P;
+  const size_t P = 2 * (size_t)R + 1, Q = (size_t)R + 1, cube = P * Q * P;
+  // work holds U (nshell n P Q) and T (nshell n Q n); its size is nshell
+  // half spectra, nshell n n (n/2+1)
+  REQUIRE(P * Q + Q * (size_t)n <= (size_t)n * (n / 2 + 1),
+          "shell_synth: k_max too large for the work area (needs k_max below a quarter of the grid Nyquist)");
   const size_t vbytes = (cube * cs + 255) / 256 * 256, bbytes = (cube + 255) / 256 * 256;
   if (int rc = ensure_synth(ctx, vbytes + bbytes + (size_t)n * cs)) return rc;
   BinEdges E;
   fill_edges(E, edges, nbins);
   PolySpec W;
   to_spec(W, *weight);
-  SynthArgs a;
-  a.n = n; a.R = R; a.bin0 = bin0; a.nshell = nshell; a.kf = kf;
   char* base = (char*)ctx->synth;
-  CHECK_HIP(launch_synth(prec, a, E, W, D, base, (signed char*)(base + vbytes), base + vbytes + bbytes, work, zb,
+  CHECK_HIP(launch_synth(prec, a, E, W, D, base, (signed char*)(base + vbytes), base + vbytes + bbytes, work, out, zb,
                          ctx->stream));
-  return run_fft(ctx, n, nshell * n, prec, 1, work, out, 2);
+  return 0;
 }
 
 int cnpc_resample_spectrum(cnpc_ctx* ctx, int n, int m, int prec, const void* in, int nmax, void* out) {

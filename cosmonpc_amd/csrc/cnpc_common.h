@@ -143,17 +143,22 @@ struct ShellArgs {
   int64_t stride;  // complex elements between consecutive output shells
 };
 
+#define CNPC_MAX_RADICES 24
+
 struct SynthArgs {
   int n;        // grid
   int R;        // |n_d| <= R holds every shell mode
   int bin0;     // first shell
   int nshell;   // shells in this call
   double kf;
+  int nrad;     // mixed-radix factorisation of n (radices 2, 3, 4, 5, 7)
+  int rad[CNPC_MAX_RADICES];
 };
 
 size_t xsynth_lds_bytes(int prec, int n, int R, int nshell, int zb);
+int synth_lines_per_block(int prec, int n);
 hipError_t launch_synth(int prec, const SynthArgs& a, const BinEdges& E, const PolySpec& W, const void* D, void* V,
-                        signed char* bins, void* tw, void* out, int zb, hipStream_t st);
+                        signed char* bins, void* tw, void* work, void* out, int zb, hipStream_t st);
 
 hipError_t launch_paint(const double* pos, const double* w, int64_t np, int n, double boxsize, int order,
                         const double* center, int ncopy, int prec, void* m0, void* m1, hipStream_t st);

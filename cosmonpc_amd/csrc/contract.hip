@@ -43,14 +43,16 @@ __global__ void __launch_bounds__(256) triple_dot_kernel(const T* __restrict__ G
   if (threadIdx.x == 0) partial[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
 }
 
-__global__ void sum_partials_kernel(const double* __restrict__ partial, int nparts, int nout, double* __restrict__ out,
-                                    int accumulate) {
-  // one thread per output; partial layout [nparts][nout]
-  for (int o = blockIdx.x * blockDim.x + threadIdx.x; o < nout; o += gridDim.x * blockDim.x) {
-    double s = 0.0;
-    for (int p = 0; p < nparts; ++p) s += partial[(int64_t)p * nout + o];
-    out[o] = accumulate ? out[o] + s : s;
-  }
+// one 64-lane wave per output; partial layout [nparts][nout]; lanes stride
+// over the partials, fixed-order butterfly (deterministic)
+__global__ void __launch_bounds__(64) sum_partials_kernel(const double* __restrict__ partial, int nparts, int nout,
+                                                          double* __restrict__ out, int accumulate) {
+  const int o = blockIdx.x;
+  if (o >= nout) return;
+  double s = 0.0;
+  for (int p = threadIdx.x; p < nparts; p += 64) s += partial[(int64_t)p * nout + o];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  if (threadIdx.x == 0) out[o] = accumulate ? out[o] + s : s;
 }
 
 hipError_t launch_triple_dot(int prec, const void* G, const void* A, const void* B, int64_t n, double* partial,
@@ -61,7 +63,7 @@ hipError_t launch_triple_dot(int prec, const void* G, const void* A, const void*
   else
     hipLaunchKernelGGL(triple_dot_kernel<float>, dim3(nblocks), dim3(256), 0, st, (const float*)G, (const float*)A,
                        (const float*)B, n, partial);
-  hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(64), 0, st, partial, nblocks, 1, out, 0);
+  hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(64), 0, st, (const double*)partial, nblocks, 1, out, 0);
   return hipGetLastError();
 }
 

@@ -84,9 +84,16 @@ class ShellBuilder:
         self.batch = batch
         if method not in ("synth", "filter"):
             raise ValueError("shell_method must be 'synth' or 'filter'")
-        # the pruned synthesis needs |n_d| <= R < n/2 and bins that fit int8
+        # the pruned synthesis needs |n_d| <= R < n/2, bins that fit int8,
+        # a grid that factors into 2, 3, 5, 7 and its two intermediates
+        # ((2R+1)(R+1) + (R+1) n per plane) inside the work area
         R = math.ceil(self.edges[-1] / (2 * math.pi / boxsize))
-        if method == "synth" and (len(self.edges) - 1 > 127 or R >= n // 2):
+        m = n
+        for p in (2, 3, 5, 7):
+            while m % p == 0:
+                m //= p
+        fits = (2 * R + 1) * (R + 1) + (R + 1) * n <= n * (n // 2 + 1)
+        if method == "synth" and (len(self.edges) - 1 > 127 or R >= n // 2 or m != 1 or not fits):
             method = "filter"
         self.method = method
         self.work = ctx.spectrum(n, prec, batch)
