@@ -327,18 +327,20 @@ int cnpc_shell_synth(cnpc_ctx* ctx, int n, double boxsize, int prec, const void*
   int zb = 32;
   while (zb > 8 && xsynth_lds_bytes(prec, n, R, nshell, zb) > 64 * 1024) zb /= 2;
   REQUIRE(xsynth_lds_bytes(prec, n, R, nshell, zb) <= 64 * 1024, "shell_synth: k_max too large for the LDS tile");
-  REQUIRE(synth_lines_per_block(prec, n) >= 1, "shell_synth: mesh too large for the LDS line buffers");
   SynthArgs a;
   std::memset(&a, 0, sizeof(a));
   a.n = n; a.R = R; a.bin0 = bin0; a.nshell = nshell; a.kf = kf;
-  {  // mixed-radix factorisation of n into 4, 2, 3, 5, 7
+  int rmin = 8;
+  {  // mixed-radix factorisation of n into 4, 2, 3, 5
     int m = n, k = 0;
     while (m % 4 == 0 && k < CNPC_MAX_RADICES) { a.rad[k++] = 4; m /= 4; }
-    for (int r : {2, 3, 5, 7})
+    for (int r : {2, 3, 5})
       while (m % r == 0 && k < CNPC_MAX_RADICES) { a.rad[k++] = r; m /= r; }
-    REQUIRE(m == 1, "shell_synth: nmesh must factor into 2, 3, 5 and 7");
+    REQUIRE(m == 1, "shell_synth: nmesh must factor into 2, 3 and 5");
     a.nrad = k;
+    for (int i = 0; i < k; ++i) rmin = a.rad[i] < rmin ? a.rad[i] : rmin;
   }
+  REQUIRE(synth_lines_per_block(prec, n, rmin) >= 1, "shell_synth: mesh too large for the LDS line buffers");
   const size_t cs = prec == 0 ? 16 : 8;
   const size_t P = 2 * (size_t)R + 1, Q = (size_t)R + 1, cube = P * Q * P;
   // work holds U (nshell n P Q) and T (nshell n Q n); its size is nshell

@@ -151,12 +151,12 @@ struct SynthArgs {
   int bin0;     // first shell
   int nshell;   // shells in this call
   double kf;
-  int nrad;     // mixed-radix factorisation of n (radices 2, 3, 4, 5, 7)
+  int nrad;     // mixed-radix factorisation of n (radices 2, 3, 4, 5)
   int rad[CNPC_MAX_RADICES];
 };
 
 size_t xsynth_lds_bytes(int prec, int n, int R, int nshell, int zb);
-int synth_lines_per_block(int prec, int n);
+int synth_lines_per_block(int prec, int n, int rmin);
 hipError_t launch_synth(int prec, const SynthArgs& a, const BinEdges& E, const PolySpec& W, const void* D, void* V,
                         signed char* bins, void* tw, void* work, void* out, int zb, hipStream_t st);
 

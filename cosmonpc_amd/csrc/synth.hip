@@ -73,7 +73,9 @@ __global__ void __launch_bounds__(256) cube_kernel(SynthArgs a, BinEdges E, Poly
 
 // ------------------------------------------------------------------ x pass
 // one workgroup per (ky, block of ZB consecutive kz); threads = ZB lanes x
-// (256 / ZB) x-rows.  Output U[s][x][ky][kz], ky index = ny + R.
+// (256 / ZB) x-rows.  Output U[s][x][ky][kz], ky index = ny + R.  The lines'
+// values and bins are staged in LDS first (coalesced), then every shell's
+// |kx| interval is found by scanning the staged bins.
 template <typename T, int ZB>
 __global__ void __launch_bounds__(256) xsynth_kernel(SynthArgs a, const typename cplx<T>::type* __restrict__ V,
                                                      const signed char* __restrict__ bins,
@@ -91,6 +93,7 @@ __global__ void __launch_bounds__(256) xsynth_kernel(SynthArgs a, const typename
   C* lv = tw + n;                        // ZB lines x Q values at +kx
   C* lvn = lv + ZB * Q;                  // ... and at -kx
   int* ab = (int*)(lvn + ZB * Q);        // [ZB][nshell][2] |kx| interval per shell
+  signed char* lb = (signed char*)(ab + 2 * ZB * a.nshell);  // [ZB][Q] bins at +kx
   for (int j = threadIdx.x; j < n; j += blockDim.x) tw[j] = twg[j];
   for (int t = threadIdx.x; t < ZB * Q; t += blockDim.x) {
     const int l = t / Q, kx = t % Q;
@@ -98,26 +101,27 @@ __global__ void __launch_bounds__(256) xsynth_kernel(SynthArgs a, const typename
     C vp, vn;
     vp.x = vn.x = (T)0;
     vp.y = vn.y = (T)0;
+    signed char b = -1;
     if (z < Q) {
       const int64_t base = ((int64_t)kyi * Q + z) * P;
       vp = V[base + R + kx];
       vn = V[base + R - kx];
+      b = bins[base + R + kx];
     }
     lv[l * Q + kx] = vp;
     lvn[l * Q + kx] = vn;
+    lb[l * Q + kx] = b;
   }
+  __syncthreads();
   for (int t = threadIdx.x; t < ZB * a.nshell; t += blockDim.x) {
     const int l = t / a.nshell, s = t % a.nshell;
-    const int z = kz0 + l;
+    const signed char want = (signed char)(a.bin0 + s);
+    const signed char* bl = lb + l * Q;
     int lo = 1, hi = 0;  // empty
-    if (z < Q) {
-      const int64_t base = ((int64_t)kyi * Q + z) * P + R;
-      const int want = a.bin0 + s;
-      for (int kx = 0; kx <= R; ++kx) {
-        if (bins[base + kx] == want) {
-          if (lo > hi) lo = kx;
-          hi = kx;
-        }
+    for (int kx = 0; kx < Q; ++kx) {
+      if (bl[kx] == want) {
+        if (lo > hi) lo = kx;
+        hi = kx;
       }
     }
     ab[(l * a.nshell + s) * 2 + 0] = lo;
@@ -156,64 +160,94 @@ __global__ void __launch_bounds__(256) xsynth_kernel(SynthArgs a, const typename
 }
 
 // ------------------------------------------------------------------ LDS FFT
-// One pass of the mixed-radix Stockham inverse FFT (e^{+2 pi i}) over nl
-// lines of length n: thread item i in [0, n/R) of each line reads
-// src[i + r n/R], multiplies by e^{2 pi i r k/(p R)} (k = i mod p), does an
-// R-point DFT and writes dst[(i/p) p R + k + r p].
-template <int RR, typename C>
-__device__ __forceinline__ void stockham_pass(const C* __restrict__ src, C* __restrict__ dst, const C* __restrict__ W,
-                                              int n, int nl, int p) {
+// cos / sin of 2 pi j / R for the small DFTs
+template <int RR> struct Roots;
+template <> struct Roots<2> {
+  static constexpr double c[2] = {1.0, -1.0};
+  static constexpr double s[2] = {0.0, 0.0};
+};
+template <> struct Roots<3> {
+  static constexpr double c[3] = {1.0, -0.5, -0.5};
+  static constexpr double s[3] = {0.0, 0.86602540378443865, -0.86602540378443865};
+};
+template <> struct Roots<4> {
+  static constexpr double c[4] = {1.0, 0.0, -1.0, 0.0};
+  static constexpr double s[4] = {0.0, 1.0, 0.0, -1.0};
+};
+template <> struct Roots<5> {
+  static constexpr double c[5] = {1.0, 0.30901699437494742, -0.80901699437494742, -0.80901699437494742,
+                                  0.30901699437494742};
+  static constexpr double s[5] = {0.0, 0.95105651629515357, 0.58778525229247313, -0.58778525229247313,
+                                  -0.95105651629515357};
+};
+// One in-place pass of the mixed-radix Stockham inverse FFT (e^{+2 pi i})
+// over nl lines of length n in LDS.  Butterfly i in [0, n/R) of a line reads
+// buf[i + r n/R], multiplies by e^{2 pi i r k/(p R)} (k = i mod p), does an
+// R-point DFT and writes buf[(i/p) p R + k + r p].  All reads are staged in
+// registers before the barrier, so one buffer suffices; a thread owns at most
+// MAXB butterflies (the host sizes nl accordingly).
+template <int RR, int MAXB, typename C>
+__device__ __forceinline__ void stockham_pass(C* __restrict__ buf, const C* __restrict__ W, int n, int nl, int p) {
+  typedef decltype(C().x) T;
   const int Tn = n / RR;
   const int pR = p * RR;
   const int tstep = n / pR;
-  const int dstep = n / RR;
-  for (int t = threadIdx.x; t < nl * Tn; t += blockDim.x) {
-    const int l = t / Tn, i = t - l * Tn;
-    const int k = i % p;
-    const C* s = src + l * n;
-    C u[RR];
+  const int total = nl * Tn;
+  C u[MAXB][RR];
+  int dsto[MAXB];
 #pragma unroll
-    for (int r = 0; r < RR; ++r) {
-      u[r] = s[i + r * Tn];
-      if (r) u[r] = cmul(u[r], W[r * k * tstep]);
-    }
-    C* d = dst + l * n + (i / p) * pR + k;
+  for (int it = 0; it < MAXB; ++it) {
+    const int t = threadIdx.x + it * blockDim.x;
+    dsto[it] = -1;
+    if (t < total) {
+      const int l = t / Tn, i = t - l * Tn;
+      const int k = i % p;
+      const C* s = buf + l * n;
 #pragma unroll
-    for (int b = 0; b < RR; ++b) {
-      C v = u[0];
-#pragma unroll
-      for (int q = 1; q < RR; ++q) {
-        const C w = W[((q * b) % RR) * dstep];
-        const C m = cmul(u[q], w);
-        v.x += m.x;
-        v.y += m.y;
+      for (int r = 0; r < RR; ++r) {
+        u[it][r] = s[i + r * Tn];
+        if (r) u[it][r] = cmul(u[it][r], W[r * k * tstep]);
       }
-      d[b * p] = v;
+      dsto[it] = l * n + (i / p) * pR + k;
     }
   }
+  __syncthreads();
+#pragma unroll
+  for (int it = 0; it < MAXB; ++it) {
+    if (dsto[it] >= 0) {
+      C* d = buf + dsto[it];
+#pragma unroll
+      for (int b = 0; b < RR; ++b) {
+        C v = u[it][0];
+#pragma unroll
+        for (int q = 1; q < RR; ++q) {
+          const int j = (q * b) % RR;
+          const T wc = (T)Roots<RR>::c[j], ws = (T)Roots<RR>::s[j];
+          v.x += u[it][q].x * wc - u[it][q].y * ws;
+          v.y += u[it][q].x * ws + u[it][q].y * wc;
+        }
+        d[b * p] = v;
+      }
+    }
+  }
+  __syncthreads();
 }
 
-// runs all passes; returns the buffer that holds the result
-template <typename C>
-__device__ C* stockham_lds(C* b0, C* b1, const C* W, int n, int nl, const SynthArgs& a) {
-  C* src = b0;
-  C* dst = b1;
+template <int MAXB, typename C>
+__device__ void stockham_lds(C* buf, const C* W, int n, int nl, const SynthArgs& a) {
   int p = 1;
   for (int s = 0; s < a.nrad; ++s) {
     const int R = a.rad[s];
-    if (R == 4) stockham_pass<4>(src, dst, W, n, nl, p);
-    else if (R == 2) stockham_pass<2>(src, dst, W, n, nl, p);
-    else if (R == 3) stockham_pass<3>(src, dst, W, n, nl, p);
-    else if (R == 5) stockham_pass<5>(src, dst, W, n, nl, p);
-    else stockham_pass<7>(src, dst, W, n, nl, p);
-    __syncthreads();
-    C* t = src;
-    src = dst;
-    dst = t;
+    if (R == 4) stockham_pass<4, MAXB>(buf, W, n, nl, p);
+    else if (R == 2) stockham_pass<2, MAXB>(buf, W, n, nl, p);
+    else if (R == 3) stockham_pass<3, MAXB>(buf, W, n, nl, p);
+    else stockham_pass<5, MAXB>(buf, W, n, nl, p);
     p *= R;
   }
-  return src;
 }
+
+// butterflies per thread and pass (bounds the registers of the staged pass)
+template <typename T> struct MaxB { static constexpr int v = 4; };
 
 // ------------------------------------------------------------------ y pass
 // one workgroup per (plane s*n + x, block of NL kz columns)
@@ -227,12 +261,11 @@ __global__ void __launch_bounds__(256) ysynth_kernel(SynthArgs a, int NL, const 
   const int kz0 = blockIdx.y * NL;
   extern __shared__ unsigned char smem[];
   C* W = (C*)smem;
-  C* b0 = W + n;
-  C* b1 = b0 + NL * n;
+  C* buf = W + n;
   for (int j = threadIdx.x; j < n; j += blockDim.x) W[j] = twg[j];
   for (int j = threadIdx.x; j < NL * n; j += blockDim.x) {
-    b0[j].x = (T)0;
-    b0[j].y = (T)0;
+    buf[j].x = (T)0;
+    buf[j].y = (T)0;
   }
   __syncthreads();
   for (int t = threadIdx.x; t < NL * P; t += blockDim.x) {
@@ -241,15 +274,15 @@ __global__ void __launch_bounds__(256) ysynth_kernel(SynthArgs a, int NL, const 
     if (kz < Q) {
       int y = kyi - R;
       if (y < 0) y += n;
-      b0[l * n + y] = U[(plane * P + kyi) * Q + kz];
+      buf[l * n + y] = U[(plane * P + kyi) * Q + kz];
     }
   }
   __syncthreads();
-  const C* res = stockham_lds(b0, b1, W, n, NL, a);
+  stockham_lds<MaxB<T>::v>(buf, W, n, NL, a);
   for (int t = threadIdx.x; t < NL * n; t += blockDim.x) {
     const int l = t / n, y = t - l * n;
     const int kz = kz0 + l;
-    if (kz < Q) Tm[(plane * Q + kz) * n + y] = res[l * n + y];
+    if (kz < Q) Tm[(plane * Q + kz) * n + y] = buf[l * n + y];
   }
 }
 
@@ -266,12 +299,11 @@ __global__ void __launch_bounds__(256) zsynth_kernel(SynthArgs a, int NL, const 
   const int y0 = blockIdx.y * 2 * NL;
   extern __shared__ unsigned char smem[];
   C* W = (C*)smem;
-  C* b0 = W + n;
-  C* b1 = b0 + NL * n;
+  C* buf = W + n;
   for (int j = threadIdx.x; j < n; j += blockDim.x) W[j] = twg[j];
   for (int j = threadIdx.x; j < NL * n; j += blockDim.x) {
-    b0[j].x = (T)0;
-    b0[j].y = (T)0;
+    buf[j].x = (T)0;
+    buf[j].y = (T)0;
   }
   __syncthreads();
   for (int t = threadIdx.x; t < NL * Q; t += blockDim.x) {
@@ -282,7 +314,7 @@ __global__ void __launch_bounds__(256) zsynth_kernel(SynthArgs a, int NL, const 
     const int64_t row = (plane * Q + kz) * n;
     if (ya < n) A = Tm[row + ya];
     if (yb < n) B = Tm[row + yb];
-    C* L = b0 + l * n;
+    C* L = buf + l * n;
     if (kz == 0) {  // c2r reads only the real part of the zero mode
       L[0].x = A.x;
       L[0].y = B.x;
@@ -294,12 +326,12 @@ __global__ void __launch_bounds__(256) zsynth_kernel(SynthArgs a, int NL, const 
     }
   }
   __syncthreads();
-  const C* res = stockham_lds(b0, b1, W, n, NL, a);
+  stockham_lds<MaxB<T>::v>(buf, W, n, NL, a);
   for (int t = threadIdx.x; t < 2 * NL * n; t += blockDim.x) {
     const int r = t / n, z = t - r * n;
     const int y = y0 + r;
     if (y < n) {
-      const C v = res[(r >> 1) * n + z];
+      const C v = buf[(r >> 1) * n + z];
       out[(plane * n + y) * n + z] = (r & 1) ? v.y : v.x;
     }
   }
@@ -317,15 +349,20 @@ __global__ void twiddle_kernel(int n, typename cplx<T>::type* tw) {
 
 size_t xsynth_lds_bytes(int prec, int n, int R, int nshell, int zb) {
   const size_t cs = prec == 0 ? 16 : 8;
-  return cs * (size_t)n + 2 * cs * (size_t)zb * (R + 1) + sizeof(int) * 2 * (size_t)zb * nshell;
+  return cs * (size_t)n + 2 * cs * (size_t)zb * (R + 1) + sizeof(int) * 2 * (size_t)zb * nshell +
+         (size_t)zb * (R + 1);
 }
 
-// lines per y / z workgroup so that twiddles + two line buffers fit 64 KB
-int synth_lines_per_block(int prec, int n) {
+// lines per y / z workgroup: twiddles + one line buffer within 48 KB, and at
+// most MAXB butterflies per thread in every pass (smallest radix rmin)
+int synth_lines_per_block(int prec, int n, int rmin) {
   const size_t cs = prec == 0 ? 16 : 8;
-  const size_t cap = 64 * 1024 / cs;
+  const size_t cap = 48 * 1024 / cs;
   if (cap <= (size_t)n) return 0;
-  int nl = (int)((cap - n) / (2 * (size_t)n));
+  int nl = (int)((cap - n) / (size_t)n);
+  const int maxb = 4;  // MaxB<T>::v
+  const int nb = maxb * 256 * rmin / n;
+  if (nl > nb) nl = nb;
   return nl > 16 ? 16 : nl;
 }
 
@@ -355,8 +392,10 @@ static hipError_t launch_synth_t(const SynthArgs& a, const BinEdges& E, const Po
   if (zb == 32) launch_x<T, 32>(a, V, bins, tw, U, lx, st);
   else if (zb == 16) launch_x<T, 16>(a, V, bins, tw, U, lx, st);
   else launch_x<T, 8>(a, V, bins, tw, U, lx, st);
-  const int NL = synth_lines_per_block(prec, n);
-  const size_t lyz = sizeof(C) * ((size_t)n + 2 * (size_t)NL * n);
+  int rmin = 8;
+  for (int i = 0; i < a.nrad; ++i) rmin = a.rad[i] < rmin ? a.rad[i] : rmin;
+  const int NL = synth_lines_per_block(prec, n, rmin);
+  const size_t lyz = sizeof(C) * ((size_t)n + (size_t)NL * n);
   const int64_t planes = (int64_t)a.nshell * n;
   hipLaunchKernelGGL(ysynth_kernel<T>, dim3((unsigned)planes, (Q + NL - 1) / NL), blk, lyz, st, a, NL, (const C*)U,
                      (const C*)tw, Tm);

@@ -89,10 +89,10 @@ class ShellBuilder:
         # ((2R+1)(R+1) + (R+1) n per plane) inside the work area
         R = math.ceil(self.edges[-1] / (2 * math.pi / boxsize))
         m = n
-        for p in (2, 3, 5, 7):
+        for p in (2, 3, 5):
             while m % p == 0:
                 m //= p
-        fits = (2 * R + 1) * (R + 1) + (R + 1) * n <= n * (n // 2 + 1)
+        fits =(2 * R + 1) * (R + 1) + (R + 1) * n <= n * (n // 2 + 1)
         if method == "synth" and (len(self.edges) - 1 > 127 or R >= n // 2 or m != 1 or not fits):
             method = "filter"
         self.method = method
@@ -260,11 +260,12 @@ def _bk_signal(ctx, src1, src2, G, kinfo, mcfg, tjv, ell_1, ell_2, sym_allowed, 
 
 
 def _nice_size(m):
-    """Smallest m' >= m with m' % 4 == 0 and prime factors in {2, 3, 5, 7}."""
+    """Smallest m' >= m with m' % 4 == 0 and prime factors in {2, 3, 5}
+    (the radices of the shell synthesis)."""
     while True:
         if m % 4 == 0:
             r = m
-            for p in (2, 3, 5, 7):
+            for p in (2, 3, 5):
                 while r % p == 0:
                     r //= p
             if r == 1:
