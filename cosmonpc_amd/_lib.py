@@ -58,6 +58,17 @@ SIGNATURES = {
     "cnpc_absmax_f32": [_P, _P, _L, _P],
     "cnpc_zero": [_P, _P, _L],
     "cnpc_sync": [_P],
+    "cnpc_fft_c2c": [_P, _I, _I, _I, _I, _P, _P],
+    "cnpc_full_spectrum": [_P, _I, _D, _I, _P, _P, _I, _I, _I, _D, _P],
+    "cnpc_xweight": [_P, _I, _D, _I, _P, _P, _P, _P],
+    "cnpc_kcombine": [_P, _I, _D, _I, _P, _P, _I, _I, _I, _I, _D, _D, _P, _D, _D, _I, _P],
+    "cnpc_full_bin": [_P, _I, _D, _I, _P, _P, _I, _P, _I, _P],
+    "cnpc_triple_sum": [_P, _I, _L, _P, _P, _P, _P],
+    "cnpc_split_complex": [_P, _I, _L, _P, _P, _P],
+    "cnpc_resample_full": [_P, _I, _I, _I, _P, _I, _P],
+    "cnpc_particle_sums": [_P, _L, _P, _P, _P, _P, _I, _P, _P],
+    "cnpc_pair_weights": [_P, _L, _P, _P, _I, _D, _P],
+    "cnpc_radecz_to_xyz": [_P, _L, _P, _P, _P, _D, _P],
 }
 
 _lib = None

@@ -3,6 +3,7 @@
 // paint.hip, spectral.hip and contract.hip.  See include/cnpc.h.
 #include <rocfft/rocfft.h>
 
+#include <cmath>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -101,7 +102,7 @@ static int ensure_synth(cnpc_ctx* ctx, size_t bytes) {
 }
 
 // dims = 3: N^3 transforms; dims = 2: N^2 planes (the (y, z) pass of the
-// shell synthesis).  Real transforms, default packed layouts.
+// shell synthesis).  Default packed layouts.
 static int get_plan(cnpc_ctx* ctx, int n, int batch, int prec, int dir, FftPlan** out, int dims = 3) {
   auto key = std::make_tuple(n, batch, prec, dir, dims);
   auto it = ctx->plans.find(key);
@@ -113,9 +114,13 @@ static int get_plan(cnpc_ctx* ctx, int n, int batch, int prec, int dir, FftPlan*
   std::call_once(once, [] { rocfft_setup(); });
   FftPlan p;
   size_t lengths[3] = {(size_t)n, (size_t)n, (size_t)n};
+  // dir: 0 real forward, 1 real inverse, 2 complex forward, 3 complex inverse
+  const rocfft_transform_type tt = dir == 0   ? rocfft_transform_type_real_forward
+                                   : dir == 1 ? rocfft_transform_type_real_inverse
+                                   : dir == 2 ? rocfft_transform_type_complex_forward
+                                              : rocfft_transform_type_complex_inverse;
   rocfft_status s = rocfft_plan_create(
-      &p.plan, rocfft_placement_notinplace,
-      dir == 0 ? rocfft_transform_type_real_forward : rocfft_transform_type_real_inverse,
+      &p.plan, rocfft_placement_notinplace, tt,
       prec == 0 ? rocfft_precision_double : rocfft_precision_single, (size_t)dims, lengths, (size_t)batch, nullptr);
   REQUIRE(s == rocfft_status_success, "rocfft_plan_create failed (status " + std::to_string((int)s) + ")");
   s = rocfft_plan_get_work_buffer_size(p.plan, &p.work_bytes);
@@ -445,6 +450,196 @@ int cnpc_absmax_f32(cnpc_ctx* ctx, const float* x, int64_t n, double* out) {
   double m = 0;
   for (double v : h) m = v > m ? v : m;
   *out = m;
+  return 0;
+}
+
+// ---------------------------------------------------------------- survey
+
+int cnpc_fft_c2c(cnpc_ctx* ctx, int n, int batch, int prec, int dir, void* in, void* out) {
+  REQUIRE(dir == -1 || dir == 1, "fft_c2c: dir must be -1 (forward) or +1 (inverse)");
+  return run_fft(ctx, n, batch, prec, dir == -1 ? 2 : 3, in, out);
+}
+
+int cnpc_full_spectrum(cnpc_ctx* ctx, int n, double boxsize, int prec, const void* c1, const void* c2,
+                       int interlaced, int comp, int window, double scale, void* out) {
+  REQUIRE(ctx, "null context");
+  REQUIRE(n >= 2 && n % 2 == 0 && boxsize > 0, "full_spectrum: bad mesh");
+  REQUIRE(prec == 0 || prec == 1, "full_spectrum: prec must be 0 or 1");
+  REQUIRE(comp >= 0 && comp <= 2 && window >= 1 && window <= 4, "full_spectrum: bad compensation / window");
+  REQUIRE(c1 && out && (!interlaced || c2), "full_spectrum: null buffers");
+  REQUIRE(out != c1 && out != c2, "full_spectrum: out must not alias the half spectra");
+  FullSpecArgs a;
+  a.n = n; a.interlaced = interlaced ? 1 : 0; a.comp = comp; a.p = window;
+  a.kf = 2.0 * M_PI / boxsize; a.H = boxsize / n; a.scale = scale;
+  CHECK_HIP(launch_full_spectrum(prec, a, c1, c2, out, ctx->stream));
+  return 0;
+}
+
+static int check_poly(const cnpc_poly* w, const char* who) {
+  REQUIRE(w->deg >= 0 && w->deg < CNPC_ZDEG && w->m >= 0 && w->part >= 0 && w->part <= 3,
+          std::string(who) + ": bad weight polynomial");
+  return 0;
+}
+
+int cnpc_xweight(cnpc_ctx* ctx, int n, double boxsize, int prec, const double* boxcenter, const cnpc_poly* w,
+                 const void* F, void* out) {
+  REQUIRE(ctx, "null context");
+  REQUIRE(n >= 2 && n % 2 == 0 && boxsize > 0, "xweight: bad mesh");
+  REQUIRE(prec == 0 || prec == 1, "xweight: prec must be 0 or 1");
+  REQUIRE(F && out && w && boxcenter, "xweight: null arguments");
+  if (int rc = check_poly(w, "xweight")) return rc;
+  XWeightArgs a;
+  a.n = n; a.pad = 0; a.H = boxsize / n;
+  for (int d = 0; d < 3; ++d) a.off[d] = 0.5 * boxsize / n + boxcenter[d];
+  PolySpec W;
+  to_spec(W, *w);
+  CHECK_HIP(launch_xweight(prec, a, W, F, out, ctx->stream));
+  return 0;
+}
+
+int cnpc_kcombine(cnpc_ctx* ctx, int n, double boxsize, int prec, const void* A, const void* B, int opb, int comp,
+                  int window, int shot, double s_re, double s_im, const cnpc_poly* w, double c_re, double c_im,
+                  int accumulate, void* out) {
+  REQUIRE(ctx, "null context");
+  REQUIRE(n >= 2 && n % 2 == 0 && boxsize > 0, "kcombine: bad mesh");
+  REQUIRE(prec == 0 || prec == 1, "kcombine: prec must be 0 or 1");
+  REQUIRE(opb >= 0 && opb <= 2 && comp >= 0 && comp <= 2 && window >= 1 && window <= 4, "kcombine: bad options");
+  REQUIRE(A && out && (opb == 0 || B), "kcombine: null buffers");
+  REQUIRE(B != out || opb == 0 || B == A, "kcombine: B must not alias out");
+  KCombineArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.n = n; a.opb = opb; a.comp = comp; a.p = window; a.shot = shot ? 1 : 0; a.poly = w ? 1 : 0;
+  a.acc = accumulate ? 1 : 0; a.kf = 2.0 * M_PI / boxsize;
+  a.s_re = s_re; a.s_im = s_im; a.c_re = c_re; a.c_im = c_im;
+  PolySpec P;
+  std::memset(&P, 0, sizeof(P));
+  if (w) {
+    if (int rc = check_poly(w, "kcombine")) return rc;
+    to_spec(P, *w);
+  }
+  CHECK_HIP(launch_kcombine(prec, a, P, A, B, out, ctx->stream));
+  return 0;
+}
+
+int cnpc_full_bin(cnpc_ctx* ctx, int n, double boxsize, int prec, const void* A, const void* B, int opb,
+                  const double* edges, int nbins, double* out) {
+  REQUIRE(ctx, "null context");
+  REQUIRE(n >= 2 && n % 2 == 0 && boxsize > 0, "full_bin: bad mesh");
+  REQUIRE(prec == 0 || prec == 1, "full_bin: prec must be 0 or 1");
+  REQUIRE(nbins >= 1 && nbins <= CNPC_MAX_BINS, "full_bin: nbins must be in [1, 256]");
+  REQUIRE(opb >= 0 && opb <= 2 && A && out && edges && (opb == 0 || B), "full_bin: bad arguments");
+  for (int i = 0; i < nbins; ++i) REQUIRE(edges[i + 1] > edges[i], "full_bin: edges must increase");
+  BinEdges E;
+  fill_edges(E, edges, nbins);
+  if (int rc = cnpc_zero(ctx, out, (int64_t)2 * nbins * 8)) return rc;
+  CHECK_HIP(launch_full_bin(prec, n, opb, 2.0 * M_PI / boxsize, E, A, B, out, ctx->stream));
+  return 0;
+}
+
+int cnpc_triple_sum(cnpc_ctx* ctx, int prec, int64_t count, const void* A, const void* B, const void* C, double* out) {
+  REQUIRE(ctx, "null context");
+  REQUIRE(prec == 0 || prec == 1, "triple_sum: prec must be 0 or 1");
+  REQUIRE(count >= 1 && A && out, "triple_sum: bad arguments");
+  if (int rc = ensure_scratch(ctx, (size_t)triple_sum_blocks(count) * 16)) return rc;
+  CHECK_HIP(launch_triple_sum(prec, count, A, B, C, ctx->scratch, out, ctx->stream));
+  return 0;
+}
+
+int cnpc_split_complex(cnpc_ctx* ctx, int prec, int64_t count, const void* src, void* re, void* im) {
+  REQUIRE(ctx, "null context");
+  REQUIRE(prec == 0 || prec == 1, "split_complex: prec must be 0 or 1");
+  REQUIRE(count >= 0 && src && re && im && re != im, "split_complex: bad arguments");
+  if (count == 0) return 0;
+  CHECK_HIP(launch_split(prec, count, src, re, im, ctx->stream));
+  return 0;
+}
+
+int cnpc_resample_full(cnpc_ctx* ctx, int n, int m, int prec, const void* in, int nmax, void* out) {
+  REQUIRE(ctx, "null context");
+  REQUIRE(n >= 2 && m >= 2 && n % 2 == 0 && m % 2 == 0, "resample_full: bad grids");
+  REQUIRE(prec == 0 || prec == 1, "resample_full: prec must be 0 or 1");
+  REQUIRE(nmax >= 0 && 2 * nmax < m && 2 * nmax < n, "resample_full: nmax must be below m/2 and n/2");
+  REQUIRE(in && out && in != out, "resample_full: null or aliased buffers");
+  CHECK_HIP(launch_resample_full(prec, n, m, nmax, in, out, ctx->stream));
+  return 0;
+}
+
+int cnpc_particle_sums(cnpc_ctx* ctx, int64_t np, const double* pos, const double* W, const double* Wfkp,
+                       const double* NZ, int npoly, const cnpc_poly* polys, double* out) {
+  REQUIRE(ctx, "null context");
+  REQUIRE(np >= 0 && out && npoly >= 0 && npoly <= CNPC_MAX_POLY, "particle_sums: bad arguments");
+  REQUIRE(np == 0 || (W && Wfkp && (npoly == 0 || pos)), "particle_sums: null arrays");
+  REQUIRE(npoly == 0 || polys, "particle_sums: null polys");
+  const int nv = 5 + 2 * npoly;
+  if (np == 0) return cnpc_zero(ctx, out, (int64_t)nv * 8);
+  PolySet P;
+  std::memset(&P, 0, sizeof(P));
+  for (int q = 0; q < npoly; ++q) {
+    if (int rc = check_poly(&polys[q], "particle_sums")) return rc;
+    to_spec(P.s[q], polys[q]);
+  }
+  if (int rc = ensure_scratch(ctx, (size_t)particle_sums_blocks(np) * nv * 8)) return rc;
+  CHECK_HIP(launch_particle_sums(np, pos, W, Wfkp, NZ, P, npoly, (double*)ctx->scratch, out, ctx->stream));
+  return 0;
+}
+
+int cnpc_pair_weights(cnpc_ctx* ctx, int64_t np, const double* W, const double* Wfkp, int power, double scale,
+                      double* out) {
+  REQUIRE(ctx, "null context");
+  REQUIRE(np >= 0 && (power == 1 || power == 2), "pair_weights: bad arguments");
+  REQUIRE(np == 0 || (W && out), "pair_weights: null arrays");
+  if (np == 0) return 0;
+  CHECK_HIP(launch_pair_weights(np, W, Wfkp, power, scale, out, ctx->stream));
+  return 0;
+}
+
+// Gauss-Legendre nodes/weights on [-1, 1] by Newton iteration on P_n.
+static void gauss_legendre(int n, double* t, double* w) {
+  for (int i = 0; i < (n + 1) / 2; ++i) {
+    double x = std::cos(M_PI * (i + 0.75) / (n + 0.5));
+    double dp = 0.0;
+    for (int it = 0; it < 100; ++it) {
+      double p0 = 1.0, p1 = x;
+      for (int k = 2; k <= n; ++k) {
+        const double p2 = ((2.0 * k - 1.0) * x * p1 - (k - 1.0) * p0) / k;
+        p0 = p1;
+        p1 = p2;
+      }
+      dp = n * (x * p1 - p0) / (x * x - 1.0);
+      const double dx = p1 / dp;
+      x -= dx;
+      if (std::fabs(dx) < 1e-16) break;
+    }
+    double p0 = 1.0, p1 = x;
+    for (int k = 2; k <= n; ++k) {
+      const double p2 = ((2.0 * k - 1.0) * x * p1 - (k - 1.0) * p0) / k;
+      p0 = p1;
+      p1 = p2;
+    }
+    dp = n * (x * p1 - p0) / (x * x - 1.0);
+    t[i] = -x;
+    t[n - 1 - i] = x;
+    w[i] = w[n - 1 - i] = 2.0 / ((1.0 - x * x) * dp * dp);
+  }
+}
+
+int cnpc_radecz_to_xyz(cnpc_ctx* ctx, int64_t np, const double* ra, const double* dec, const double* z, double omega_m,
+                       double* pos) {
+  REQUIRE(ctx, "null context");
+  REQUIRE(np >= 0 && omega_m > 0.0 && omega_m <= 1.0, "radecz_to_xyz: bad arguments (0 < omega_m <= 1)");
+  REQUIRE(np == 0 || (ra && dec && z && pos), "radecz_to_xyz: null arrays");
+  if (np == 0) return 0;
+  static RadeczArgs base;
+  static std::once_flag once;
+  std::call_once(once, [] {
+    std::memset(&base, 0, sizeof(base));
+    base.nodes = 48;
+    gauss_legendre(base.nodes, base.t, base.w);
+  });
+  RadeczArgs a = base;
+  a.om = omega_m;
+  a.dh = 299792.458 / 100.0;
+  CHECK_HIP(launch_radecz(np, ra, dec, z, a, pos, ctx->stream));
   return 0;
 }
 

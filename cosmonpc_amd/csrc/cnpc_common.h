@@ -93,6 +93,58 @@ __device__ __forceinline__ double2 poly_eval(const PolySpec& s, double x, double
   return make_double2(s.pre_re * re - s.pre_im * im, s.pre_re * im + s.pre_im * re);
 }
 
+// Window compensations on the circular frequency w_d = 2 pi n_d / N.
+// comp: 0 none, 1 divide by prod_d sinc(w_d/2)^p  (CompensateNGP..PCS,
+//       math_evaluator.py:206-248), 2 divide by prod_d sqrt(C_p(w_d))
+//       (Compensate*Shotnoise, math_evaluator.py:251-275).
+__device__ __forceinline__ double alias_c(int p, double s) {
+  if (p == 2) return 1.0 - 2.0 / 3 * s;
+  if (p == 3) return 1.0 - s + 2.0 / 15 * s * s;
+  if (p == 4) return 1.0 - 4.0 / 3.0 * s + 2.0 / 5.0 * s * s - 4.0 / 315.0 * s * s * s;
+  return 1.0;
+}
+
+__device__ __forceinline__ double sinc_half(double w) {  // np.sinc(0.5 w / pi)
+  if (w == 0.0) return 1.0;
+  double h = 0.5 * w;
+  return sin(h) / h;
+}
+
+__device__ __forceinline__ double comp_factor(int comp, int p, int nx, int ny, int nz, int n) {
+  if (comp == 0) return 1.0;
+  const double c = 2.0 * M_PI / n;
+  double w[3] = {nx * c, ny * c, nz * c};
+  double f = 1.0;
+  for (int d = 0; d < 3; ++d) {
+    if (comp == 1) {
+      double t = sinc_half(w[d]);
+      double tp = t;
+      for (int i = 1; i < p; ++i) tp *= t;
+      f /= tp;
+    } else if (p > 1) {
+      double s = sin(0.5 * w[d]);
+      f /= sqrt(alias_c(p, s * s));
+    }
+  }
+  return f;
+}
+
+// C_p(w) / sinc(w/2)^{2p} product (Compensate_bk_noise_*, math_evaluator.py:278-309)
+__device__ __forceinline__ double bk_noise_factor(int p, int nx, int ny, int nz, int n) {
+  if (p <= 1) return 1.0;
+  const double c = 2.0 * M_PI / n;
+  double w[3] = {nx * c, ny * c, nz * c};
+  double f = 1.0;
+  for (int d = 0; d < 3; ++d) {
+    double s = sin(0.5 * w[d]);
+    double t = sinc_half(w[d]);
+    double tp = t;
+    for (int i = 1; i < p; ++i) tp *= t;
+    f *= alias_c(p, s * s) / (tp * tp);
+  }
+  return f;
+}
+
 template <typename T> struct cplx;
 template <> struct cplx<double> { typedef double2 type; };
 template <> struct cplx<float> { typedef float2 type; };
@@ -154,6 +206,71 @@ struct SynthArgs {
   int nrad;     // mixed-radix factorisation of n (radices 2, 3, 4, 5)
   int rad[CNPC_MAX_RADICES];
 };
+
+// ---------------------------------------------------------------- survey (full grids)
+struct FullSpecArgs {
+  int n;
+  int interlaced;
+  int comp;
+  int p;
+  double kf;
+  double H;
+  double scale;
+};
+
+struct XWeightArgs {
+  int n;
+  int pad;
+  double H;
+  double off[3];  // H/2 + boxcenter
+};
+
+// out (+)= c * w(khat) * [comp(k) * (A op B) - s * C_bk(k)]
+struct KCombineArgs {
+  int n;
+  int opb;   // 0: A, 1: A*B, 2: A*conj(B)
+  int comp;  // comp_factor code
+  int p;     // window order
+  int shot;  // subtract s * bk_noise_factor
+  int poly;  // multiply by the polynomial weight of khat
+  int acc;   // accumulate into out
+  int pad;
+  double kf;
+  double s_re, s_im;
+  double c_re, c_im;
+};
+
+#define CNPC_GL_MAX 64
+struct RadeczArgs {
+  int nodes;
+  int pad;
+  double om;  // Omega_m
+  double dh;  // c / (100 km/s/Mpc) in Mpc/h
+  double t[CNPC_GL_MAX];
+  double w[CNPC_GL_MAX];
+};
+
+#define CNPC_PSUM_MAX (5 + 2 * CNPC_MAX_POLY)
+
+hipError_t launch_full_spectrum(int prec, const FullSpecArgs& a, const void* c1, const void* c2, void* out,
+                                hipStream_t st);
+hipError_t launch_xweight(int prec, const XWeightArgs& a, const PolySpec& W, const void* F, void* out, hipStream_t st);
+hipError_t launch_kcombine(int prec, const KCombineArgs& a, const PolySpec& P, const void* A, const void* B, void* out,
+                           hipStream_t st);
+hipError_t launch_full_bin(int prec, int n, int opb, double kf, const BinEdges& E, const void* A, const void* B,
+                           double* out, hipStream_t st);
+int triple_sum_blocks(int64_t count);
+hipError_t launch_triple_sum(int prec, int64_t count, const void* A, const void* B, const void* C, void* partial,
+                             double* out, hipStream_t st);
+hipError_t launch_split(int prec, int64_t count, const void* src, void* re, void* im, hipStream_t st);
+hipError_t launch_resample_full(int prec, int n, int m, int nmax, const void* in, void* out, hipStream_t st);
+int particle_sums_blocks(int64_t np);
+hipError_t launch_particle_sums(int64_t np, const double* pos, const double* W, const double* WF, const double* NZ,
+                                const PolySet& P, int npoly, double* partial, double* out, hipStream_t st);
+hipError_t launch_pair_weights(int64_t np, const double* W, const double* WF, int power, double scale, double* out,
+                               hipStream_t st);
+hipError_t launch_radecz(int64_t np, const double* ra, const double* dec, const double* z, const RadeczArgs& a,
+                         double* pos, hipStream_t st);
 
 size_t xsynth_lds_bytes(int prec, int n, int R, int nshell, int zb);
 int synth_lines_per_block(int prec, int n, int rmin);

@@ -14,57 +14,7 @@
 namespace cnpc {
 
 // ------------------------------------------------------------- prepare
-// comp: 0 none, 1 divide by prod_d sinc(w_d/2)^p  (CompensateNGP..PCS,
-//       math_evaluator.py:206-248), 2 divide by prod_d sqrt(C_p(w_d))
-//       (Compensate*Shotnoise, math_evaluator.py:251-275).
-
-__device__ __forceinline__ double alias_c(int p, double s) {
-  if (p == 2) return 1.0 - 2.0 / 3 * s;
-  if (p == 3) return 1.0 - s + 2.0 / 15 * s * s;
-  if (p == 4) return 1.0 - 4.0 / 3.0 * s + 2.0 / 5.0 * s * s - 4.0 / 315.0 * s * s * s;
-  return 1.0;
-}
-
-__device__ __forceinline__ double sinc_half(double w) {  // np.sinc(0.5 w / pi)
-  if (w == 0.0) return 1.0;
-  double h = 0.5 * w;
-  return sin(h) / h;
-}
-
-__device__ __forceinline__ double comp_factor(int comp, int p, int nx, int ny, int nz, int n) {
-  if (comp == 0) return 1.0;
-  const double c = 2.0 * M_PI / n;
-  double w[3] = {nx * c, ny * c, nz * c};
-  double f = 1.0;
-  for (int d = 0; d < 3; ++d) {
-    if (comp == 1) {
-      double t = sinc_half(w[d]);
-      double tp = t;
-      for (int i = 1; i < p; ++i) tp *= t;
-      f /= tp;
-    } else if (p > 1) {
-      double s = sin(0.5 * w[d]);
-      f /= sqrt(alias_c(p, s * s));
-    }
-  }
-  return f;
-}
-
-// C_p(w) / sinc(w/2)^{2p} product (Compensate_bk_noise_*, math_evaluator.py:278-309)
-__device__ __forceinline__ double bk_noise_factor(int p, int nx, int ny, int nz, int n) {
-  if (p <= 1) return 1.0;
-  const double c = 2.0 * M_PI / n;
-  double w[3] = {nx * c, ny * c, nz * c};
-  double f = 1.0;
-  for (int d = 0; d < 3; ++d) {
-    double s = sin(0.5 * w[d]);
-    double t = sinc_half(w[d]);
-    double tp = t;
-    for (int i = 1; i < p; ++i) tp *= t;
-    f *= alias_c(p, s * s) / (tp * tp);
-  }
-  return f;
-}
+// Interlace combine, scaling and the compensation comp_factor() codes.
 
 template <typename T>
 __global__ void __launch_bounds__(256) prepare_kernel(PrepareArgs a, const typename cplx<T>::type* __restrict__ c1,

@@ -208,6 +208,137 @@ class GpuContext:
     def sync(self):
         check(self.lib.cnpc_sync(self.h), "cnpc_sync")
 
+    # ---------------------------------------------------------------- survey (full grids)
+    def full(self, n, prec, batch=None):
+        shape = (n, n, n) if batch is None else (batch, n, n, n)
+        return torch.empty(shape, dtype=CPLX_DTYPE[prec], device=self.device)
+
+    @staticmethod
+    def _check_full(t, n, prec):
+        if t.dtype != CPLX_DTYPE[prec] or tuple(t.shape) != (n, n, n) or not t.is_contiguous() or not t.is_cuda:
+            raise ValueError(f"expected contiguous device full mesh {(n, n, n)} {CPLX_DTYPE[prec]}, "
+                             f"got {tuple(t.shape)} {t.dtype}")
+
+    def fft_c2c(self, src, out, n, prec, direction):
+        """Unnormalised c2c transform of a full mesh; direction -1 forward, +1 inverse."""
+        self._check_full(src, n, prec)
+        self._check_full(out, n, prec)
+        check(self.lib.cnpc_fft_c2c(self.h, n, 1, prec, int(direction), _ptr(src), _ptr(out)), "cnpc_fft_c2c")
+
+    def full_spectrum(self, c1, c2, out, n, boxsize, prec, interlaced, comp, sampler, scale):
+        self._check_spec(c1, n, prec)
+        if interlaced:
+            self._check_spec(c2, n, prec)
+        self._check_full(out, n, prec)
+        check(self.lib.cnpc_full_spectrum(self.h, n, float(boxsize), prec, _ptr(c1), _ptr(c2 if interlaced else None),
+                                          int(bool(interlaced)), int(comp), WINDOW[sampler], float(scale), _ptr(out)),
+              "cnpc_full_spectrum")
+
+    def xweight(self, F, out, n, boxsize, prec, boxcenter, poly):
+        self._check_full(F, n, prec)
+        self._check_full(out, n, prec)
+        c = (ctypes.c_double * 3)(*[float(v) for v in boxcenter])
+        check(self.lib.cnpc_xweight(self.h, n, float(boxsize), prec, c, ctypes.byref(poly), _ptr(F), _ptr(out)),
+              "cnpc_xweight")
+
+    def kcombine(self, A, out, n, boxsize, prec, B=None, opb=0, comp=0, sampler="cic", shot=None, poly=None,
+                 coef=1.0, accumulate=False):
+        """out (+)= coef * w(khat) * [comp(k) (A op B) - shot * C_bk(k)]."""
+        self._check_full(A, n, prec)
+        self._check_full(out, n, prec)
+        if opb:
+            self._check_full(B, n, prec)
+        s = complex(shot) if shot is not None else 0j
+        c = complex(coef)
+        check(self.lib.cnpc_kcombine(self.h, n, float(boxsize), prec, _ptr(A), _ptr(B if opb else None), int(opb),
+                                     int(comp), WINDOW[sampler], int(shot is not None), s.real, s.imag,
+                                     ctypes.byref(poly) if poly is not None else None, c.real, c.imag,
+                                     int(bool(accumulate)), _ptr(out)), "cnpc_kcombine")
+
+    def full_bin(self, A, n, boxsize, prec, edges, B=None, opb=0):
+        """Complex per-bin sums of A op B over the full grid (host complex128)."""
+        edges = np.ascontiguousarray(edges, dtype=np.float64)
+        nb = len(edges) - 1
+        self._check_full(A, n, prec)
+        if opb:
+            self._check_full(B, n, prec)
+        out = torch.empty(2 * nb, dtype=torch.float64, device=self.device)
+        check(self.lib.cnpc_full_bin(self.h, n, float(boxsize), prec, _ptr(A), _ptr(B if opb else None), int(opb),
+                                     edges.ctypes.data_as(ctypes.c_void_p), nb, _ptr(out)), "cnpc_full_bin")
+        o = out.cpu().numpy()
+        return o[:nb] + 1j * o[nb:]
+
+    def triple_sum(self, prec, A, B=None, C=None):
+        """sum_x A*B*C over complex meshes of equal size (host complex)."""
+        want = CPLX_DTYPE[prec]
+        for t in (A, B, C):
+            if t is not None and (t.dtype != want or not t.is_contiguous() or not t.is_cuda or t.numel() != A.numel()):
+                raise ValueError("triple_sum operands must be contiguous device complex meshes of one size")
+        out = torch.empty(2, dtype=torch.float64, device=self.device)
+        check(self.lib.cnpc_triple_sum(self.h, prec, A.numel(), _ptr(A), _ptr(B), _ptr(C), _ptr(out)),
+              "cnpc_triple_sum")
+        o = out.cpu().numpy()
+        return complex(o[0], o[1])
+
+    def split_complex(self, src, prec):
+        if src.dtype != CPLX_DTYPE[prec] or not src.is_contiguous() or not src.is_cuda:
+            raise ValueError("split_complex needs a contiguous device complex tensor")
+        re = torch.empty(src.shape, dtype=REAL_DTYPE[prec], device=self.device)
+        im = torch.empty(src.shape, dtype=REAL_DTYPE[prec], device=self.device)
+        check(self.lib.cnpc_split_complex(self.h, prec, src.numel(), _ptr(src), _ptr(re), _ptr(im)),
+              "cnpc_split_complex")
+        return re, im
+
+    def resample_full(self, src, n, m, prec, nmax, out=None):
+        self._check_full(src, n, prec)
+        if out is None:
+            out = self.full(m, prec)
+        self._check_full(out, m, prec)
+        check(self.lib.cnpc_resample_full(self.h, n, m, prec, _ptr(src), int(nmax), _ptr(out)), "cnpc_resample_full")
+        return out
+
+    @staticmethod
+    def _check_vec(t, np_, what):
+        if t is None:
+            return
+        if t.dtype != torch.float64 or not t.is_contiguous() or not t.is_cuda or t.shape != (np_,):
+            raise ValueError(f"{what} must be a contiguous device float64 vector of length {np_}")
+
+    def particle_sums(self, pos, W, Wfkp, NZ=None, polys=()):
+        """FP64 catalog sums (host array): [sum W Wfkp, sum W, sum W NZ Wfkp^2,
+        sum (W Wfkp)^2, sum NZ^2 W^2 Wfkp^3, then (re, im) per poly]."""
+        np_ = W.shape[0]
+        self._check_vec(W, np_, "W")
+        self._check_vec(Wfkp, np_, "WEIGHT_FKP")
+        self._check_vec(NZ, np_, "NZ")
+        if polys and (pos.dtype != torch.float64 or tuple(pos.shape) != (np_, 3) or not pos.is_contiguous()):
+            raise ValueError("positions must be a contiguous (Np, 3) float64 device tensor")
+        npoly = len(polys)
+        parr = (cnpc_poly * max(npoly, 1))(*polys) if npoly else None
+        out = torch.empty(5 + 2 * npoly, dtype=torch.float64, device=self.device)
+        check(self.lib.cnpc_particle_sums(self.h, np_, _ptr(pos if npoly else None), _ptr(W), _ptr(Wfkp), _ptr(NZ),
+                                          npoly, ctypes.cast(parr, ctypes.c_void_p) if parr is not None else None,
+                                          _ptr(out)), "cnpc_particle_sums")
+        return out.cpu().numpy()
+
+    def pair_weights(self, W, Wfkp, power=1, scale=1.0):
+        np_ = W.shape[0]
+        self._check_vec(W, np_, "W")
+        self._check_vec(Wfkp, np_, "WEIGHT_FKP")
+        out = torch.empty(np_, dtype=torch.float64, device=self.device)
+        check(self.lib.cnpc_pair_weights(self.h, np_, _ptr(W), _ptr(Wfkp), int(power), float(scale), _ptr(out)),
+              "cnpc_pair_weights")
+        return out
+
+    def radecz_to_xyz(self, ra, dec, z, omega_m):
+        np_ = ra.shape[0]
+        for t, nm in ((ra, "RA"), (dec, "DEC"), (z, "Z")):
+            self._check_vec(t, np_, nm)
+        out = torch.empty((np_, 3), dtype=torch.float64, device=self.device)
+        check(self.lib.cnpc_radecz_to_xyz(self.h, np_, _ptr(ra), _ptr(dec), _ptr(z), float(omega_m), _ptr(out)),
+              "cnpc_radecz_to_xyz")
+        return out
+
 
 _CTX = {}
 

@@ -169,92 +169,109 @@ def _weight_components(ell, m, conj_w):
     return [(p, (-1) ** m * ph) for p, ph in A.shell_components(ell, -m)]
 
 
-def _bk_signal(ctx, src1, src2, G, kinfo, mcfg, tjv, ell_1, ell_2, sym_allowed, mode, block_size, dvm,
-               conj_w=False, shell_method="synth"):
-    """Sum over magnetic configs of 3j * sum_x G F_i F'_j with the reference's
-    symmetry fill and 2 Re accumulation (``clustering_estimator.py:867-1061``;
-    with ``conj_w`` and null sources, the S3 loop of ``:1219-1376``).
+def _config_matrix(ctx, builder, src1, src2, Gs, kinfo, ell_1, m1, ell_2, m2, sym_eq, sym_cj, mode, block_size,
+                   dvm, conj_w=False):
+    """3j-free signal matrix of one magnetic configuration,
+    S_ij = sum_x G F_i F'_j, with the reference's upper-triangle evaluation and
+    lower fill (transpose for F' = F, conjugate for F' = +-conj F;
+    ``clustering_estimator.py:911-989``, ``:1773-1858``).
 
     F_i = c2r(w1(khat) src1 mask_i), F'_j = c2r(w2(khat) src2 mask_j) with
-    w = Y (or conj Y); src None means a unit spectrum."""
+    w = Y_lm (or conj Y_lm), split into real shell components; src None is a
+    unit spectrum.  ``Gs`` lists (real mesh, complex factor) pairs whose sum is
+    the (possibly complex) G."""
     n, L, prec, k_edge, k_bins = kinfo
     N3 = n ** 3
+    use_sym = sym_eq or sym_cj
+    comps1 = _weight_components(ell_1, m1, conj_w)
+    if use_sym:
+        sign = (-1) ** (m1 + ell_1)
+        # F' = F (equal) or (-1)^(m+l) conj(F): same real shells, new phases
+        comps2 = [(p, ph if sym_eq else sign * np.conj(ph)) for p, ph in comps1]
+    else:
+        comps2 = _weight_components(ell_2, m2, conj_w)
+    nc1, nc2 = len(comps1), len(comps2)
+    # resident shells per block: nc1 * bs (+ nc2 * bs off the diagonal)
+    shell_bytes = N3 * (8 if prec == 0 else 4)
+    bs_fit = _fit_block(k_bins, shell_bytes * (nc1 + nc2), ctx)
+    if use_sym and _fit_block(k_bins, shell_bytes * nc1, ctx) == k_bins:
+        bs_fit = k_bins  # one diagonal block: leg 2 reuses leg 1's shells
+    if dvm == "full":
+        sub = np.zeros((k_bins, k_bins), dtype=np.complex128)
+        bs = min(bs_fit, k_bins if block_size == "full" else int(block_size))
+    else:
+        sub = np.zeros(k_bins, dtype=np.complex128)
+        bs = bs_fit
+    for bi in range(0, k_bins, bs):
+        ib = list(range(bi, min(k_bins, bi + bs)))
+        R1 = ctx.mesh(n, prec, nc1 * len(ib)).view(nc1 * len(ib), N3)
+        for c, (poly, _) in enumerate(comps1):
+            builder.build(src1, poly, ib, R1[c * len(ib):(c + 1) * len(ib)])
+        jstarts = [bi] if dvm == "diagonal" else list(range(bi if use_sym else 0, k_bins, bs))
+        for bj in jstarts:
+            jb = list(range(bj, min(k_bins, bj + bs)))
+            if bj == bi and use_sym:
+                R2 = None
+            else:
+                R2 = ctx.mesh(n, prec, nc2 * len(jb)).view(nc2 * len(jb), N3)
+                for c, (poly, _) in enumerate(comps2):
+                    if use_sym:
+                        builder.build(src1, comps1[c][0], jb, R2[c * len(jb):(c + 1) * len(jb)])
+                    else:
+                        builder.build(src2, poly, jb, R2[c * len(jb):(c + 1) * len(jb)])
+            # pairs this block contributes (reference loop order)
+            want = []
+            for i in ib:
+                if dvm == "diagonal":
+                    js = [i]
+                elif use_sym and bj == bi:
+                    js = [j for j in jb if j >= i]
+                else:
+                    js = jb
+                for j in js:
+                    want.append((i, j))
+            rows2 = len(ib) if R2 is None else len(jb)
+            pairs = [(c1 * len(ib) + (i - bi), c2 * rows2 + (j - bj)) for (i, j) in want
+                     for c1 in range(nc1) for c2 in range(nc2)]
+            C = 0j
+            for G, gph in Gs:
+                C = C + gph * _contract(ctx, mode, prec, G, R1, R2, pairs)
+            for (i, j) in want:
+                s = 0j
+                for c1 in range(nc1):
+                    for c2 in range(nc2):
+                        s += comps1[c1][1] * comps2[c2][1] * C[c1 * len(ib) + (i - bi), c2 * rows2 + (j - bj)]
+                if dvm == "full":
+                    sub[i, j] = s
+                else:
+                    sub[i] = s
+            del R2
+        del R1
+    if dvm == "full" and use_sym:
+        iu = np.triu_indices(k_bins, 1)
+        sub[iu[1], iu[0]] = sub[iu] if sym_eq else np.conj(sub[iu])
+    return sub
+
+
+def _bk_signal(ctx, src1, src2, G, kinfo, mcfg, tjv, ell_1, ell_2, sym_allowed, mode, block_size, dvm,
+               conj_w=False, shell_method="synth"):
+    """Sum over the box magnetic configs of 3j * sum_x G F_i F'_j with the
+    reference's symmetry fill and 2 Re accumulation
+    (``clustering_estimator.py:867-1061``; with ``conj_w`` and null sources,
+    the S3 loop of ``:1219-1376``)."""
+    n, L, prec, k_edge, k_bins = kinfo
     builder = ShellBuilder(ctx, n, L, prec, k_edge, method=shell_method)
     same_ell = ell_1 == ell_2
     if dvm == "full":
         total = np.zeros((k_bins, k_bins), dtype=np.complex128)
     else:
         total = np.zeros(k_bins, dtype=np.complex128)
-    ca = src1
+    Gs = G if isinstance(G, list) else [(G, 1.0)]
     for ss, (m1, m2, M) in enumerate(mcfg):
         sym_eq = sym_allowed and same_ell and m1 == m2
         sym_cj = sym_allowed and same_ell and m1 == -m2
-        use_sym = sym_eq or sym_cj
-        comps1 = _weight_components(ell_1, m1, conj_w)
-        if use_sym:
-            sign = (-1) ** (m1 + ell_1)
-            # F' = F (equal) or (-1)^(m+l) conj(F): same real shells, new phases
-            comps2 = [(p, ph if sym_eq else sign * np.conj(ph)) for p, ph in comps1]
-        else:
-            comps2 = _weight_components(ell_2, m2, conj_w)
-        nc1, nc2 = len(comps1), len(comps2)
-        # resident shells per block: nc1 * bs (+ nc2 * bs off the diagonal)
-        shell_bytes = N3 * (8 if prec == 0 else 4)
-        bs_fit = _fit_block(k_bins, shell_bytes * (nc1 + nc2), ctx)
-        if use_sym and _fit_block(k_bins, shell_bytes * nc1, ctx) == k_bins:
-            bs_fit = k_bins  # one diagonal block: leg 2 reuses leg 1's shells
-        if dvm == "full":
-            sub = np.zeros((k_bins, k_bins), dtype=np.complex128)
-            bs = min(bs_fit, k_bins if block_size == "full" else int(block_size))
-        else:
-            sub = np.zeros(k_bins, dtype=np.complex128)
-            bs = bs_fit
-        for bi in range(0, k_bins, bs):
-            ib = list(range(bi, min(k_bins, bi + bs)))
-            R1 = ctx.mesh(n, prec, nc1 * len(ib)).view(nc1 * len(ib), N3)
-            for c, (poly, _) in enumerate(comps1):
-                builder.build(ca, poly, ib, R1[c * len(ib):(c + 1) * len(ib)])
-            jstarts = [bi] if dvm == "diagonal" else list(range(bi if use_sym else 0, k_bins, bs))
-            for bj in jstarts:
-                jb = list(range(bj, min(k_bins, bj + bs)))
-                if bj == bi and use_sym:
-                    R2 = None
-                else:
-                    R2 = ctx.mesh(n, prec, nc2 * len(jb)).view(nc2 * len(jb), N3)
-                    for c, (poly, _) in enumerate(comps2):
-                        if use_sym:
-                            builder.build(ca, comps1[c][0], jb, R2[c * len(jb):(c + 1) * len(jb)])
-                        else:
-                            builder.build(src2, poly, jb, R2[c * len(jb):(c + 1) * len(jb)])
-                # pairs this block contributes (reference loop order)
-                want = []
-                for i in ib:
-                    if dvm == "diagonal":
-                        js = [i]
-                    elif use_sym and bj == bi:
-                        js = [j for j in jb if j >= i]
-                    else:
-                        js = jb
-                    for j in js:
-                        want.append((i, j))
-                rows2 = len(ib) if R2 is None else len(jb)
-                pairs = [(c1 * len(ib) + (i - bi), c2 * rows2 + (j - bj)) for (i, j) in want
-                         for c1 in range(nc1) for c2 in range(nc2)]
-                C = _contract(ctx, mode, prec, G, R1, R2, pairs)
-                for (i, j) in want:
-                    s = 0j
-                    for c1 in range(nc1):
-                        for c2 in range(nc2):
-                            s += comps1[c1][1] * comps2[c2][1] * C[c1 * len(ib) + (i - bi), c2 * rows2 + (j - bj)]
-                    if dvm == "full":
-                        sub[i, j] = tjv[ss] * s
-                    else:
-                        sub[i] = tjv[ss] * s
-                del R2
-            del R1
-        if dvm == "full" and use_sym:
-            iu = np.triu_indices(k_bins, 1)
-            sub[iu[1], iu[0]] = sub[iu] if sym_eq else np.conj(sub[iu])
+        sub = tjv[ss] * _config_matrix(ctx, builder, src1, src2, Gs, kinfo, ell_1, m1, ell_2, m2, sym_eq, sym_cj,
+                                       mode, block_size, dvm, conj_w)
         total += sub if (m1, m2, M) == (0, 0, 0) else 2 * np.real(sub)
     return total
 

@@ -14,6 +14,8 @@ import numpy as np
 
 from .estimators import calculate_bk_sugi_box, calculate_power_spectrum_box
 from .mesh import get_mesh_box
+from .survey import (calculate_bk_sugi_survey, calculate_power_spectrum_survey, get_mesh_bk_survey,
+                     get_mesh_pk_survey)
 from .param_helper import (catalog_check, validate_config, validate_poles, validate_sugi_poles, validate_tracer)
 
 
@@ -41,27 +43,39 @@ def run_task(statistic, correlation_mode, geometry, catalogs, **kwargs):
                 f"k_max {k_max} is larger than the Nyquist frequency {nyquist_freq}. Please choose a smaller k_max.")
         validate_poles(kwargs["poles"])
         if geometry == "survey-like":
-            raise NotImplementedError("survey-like geometry is not available on the GPU path yet")
-        poles = kwargs["poles"]
-        if all(p % 2 == 1 for p in poles):
-            raise ValueError(
-                f"All poles in {poles} are odd. Please choose at least one even pole, since odd poles are "
-                "defaultly set to 0 for periodic boxes.")
-        stat_attrs, fa, fb = get_mesh_box(
-            catalogs, correlation_mode, nmesh=kwargs["nmesh"], geometry=geometry, boxsize=kwargs["boxsize"],
-            sampler=kwargs["sampler"], interlaced=kwargs["interlaced"], column_names=kwargs["column_names"],
-            statistic=statistic, apply_rsd=kwargs["apply_rsd"], para_cosmo=kwargs["cosmology"],
-            redshift_box=kwargs["redshift_box"], los=kwargs["rsd"], precision=precision, arrays=arrays)
+            logging.info("Using survey-like geometry...")
+            stat_attrs, fa, fb = get_mesh_pk_survey(
+                catalogs, correlation_mode, nmesh=nmesh, geometry=geometry, column_names=kwargs["column_names"],
+                boxsize=boxsize, sampler=kwargs["sampler"], interlaced=kwargs["interlaced"],
+                z_range=kwargs["z_range"], comp_weight_plan=kwargs["comp_weight_plan"],
+                para_cosmo=kwargs["cosmology"], normalization_scheme=kwargs["normalization_scheme"],
+                alpha_scheme=kwargs.get("alpha_scheme", "pypower"), precision=precision, arrays=arrays)
+        else:
+            logging.info("Using box geometry...")
+            poles = kwargs["poles"]
+            if all(p % 2 == 1 for p in poles):
+                raise ValueError(
+                    f"All poles in {poles} are odd. Please choose at least one even pole, since odd poles are "
+                    "defaultly set to 0 for periodic boxes.")
+            stat_attrs, fa, fb = get_mesh_box(
+                catalogs, correlation_mode, nmesh=kwargs["nmesh"], geometry=geometry, boxsize=kwargs["boxsize"],
+                sampler=kwargs["sampler"], interlaced=kwargs["interlaced"], column_names=kwargs["column_names"],
+                statistic=statistic, apply_rsd=kwargs["apply_rsd"], para_cosmo=kwargs["cosmology"],
+                redshift_box=kwargs["redshift_box"], los=kwargs["rsd"], precision=precision, arrays=arrays)
         stat_attrs.update(kwargs)
         stat_attrs["nyquist_freq"] = nyquist_freq
         t1 = time.time()
         logging.info(f"Time to create (FKP) overdensity field(s): {t1 - t0:.2f} seconds")
-        pk_res = calculate_power_spectrum_box(fa, fb, correlation_mode, stat_attrs, **kwargs)
+        if geometry == "survey-like":
+            pk_res = calculate_power_spectrum_survey(stat_attrs, fa, fb, correlation_mode, **kwargs)
+        else:
+            pk_res = calculate_power_spectrum_box(fa, fb, correlation_mode, stat_attrs, **kwargs)
         del fa, fb
         gc.collect()
         pk_res.update(stat_attrs)
-        for key in ("z_range", "scheme"):
-            pk_res.pop(key, None)
+        if geometry == "box-like":
+            for key in ("z_range", "scheme"):
+                pk_res.pop(key, None)
         out_dir = kwargs.get("output_dir")
         name_a = os.path.splitext(os.path.basename(catalogs["data_a"]))[0]
         os.makedirs(out_dir, exist_ok=True)
@@ -90,21 +104,39 @@ def run_task(statistic, correlation_mode, geometry, catalogs, **kwargs):
                 f"2 * k_max = {2 * k_max} is larger than the Nyquist frequency {nyquist_freq}. "
                 "Please choose a smaller k_max.")
         validate_sugi_poles(angu_config, geometry)
-        if geometry != "box-like":
-            raise NotImplementedError("survey-like geometry is not available on the GPU path yet")
-        stat_attrs, fa, fb, fc = get_mesh_box(
-            catalogs, correlation_mode, statistic=statistic, nmesh=kwargs["nmesh"], geometry=geometry,
-            boxsize=kwargs["boxsize"], sampler=kwargs["sampler"], interlaced=kwargs["interlaced"],
-            column_names=kwargs["column_names"], tracer_type=tracer_type, apply_rsd=kwargs["apply_rsd"],
-            para_cosmo=kwargs["cosmology"], redshift_box=kwargs["redshift_box"], los=kwargs["rsd"],
-            precision=precision, arrays=arrays)
+        if geometry == "box-like":
+            logging.info("Using box geometry...")
+            stat_attrs, fa, fb, fc = get_mesh_box(
+                catalogs, correlation_mode, statistic=statistic, nmesh=kwargs["nmesh"], geometry=geometry,
+                boxsize=kwargs["boxsize"], sampler=kwargs["sampler"], interlaced=kwargs["interlaced"],
+                column_names=kwargs["column_names"], tracer_type=tracer_type, apply_rsd=kwargs["apply_rsd"],
+                para_cosmo=kwargs["cosmology"], redshift_box=kwargs["redshift_box"], los=kwargs["rsd"],
+                precision=precision, arrays=arrays)
+        else:
+            logging.info("Using survey-like geometry...")
+            stat_attrs, fa, fb, fc = get_mesh_bk_survey(
+                catalogs, correlation_mode, tracer_type=tracer_type, angu_config=angu_config, nmesh=nmesh,
+                geometry=geometry, column_names=kwargs["column_names"], boxsize=boxsize,
+                sampler=kwargs["sampler"], interlaced=kwargs["interlaced"], z_range=kwargs["z_range"],
+                comp_weight_plan=kwargs["comp_weight_plan"], para_cosmo=kwargs["cosmology"],
+                normalization_scheme=kwargs.get("normalization_scheme", "particle"),
+                alpha_scheme=kwargs.get("alpha_scheme", "pypower"), precision=precision, arrays=arrays)
         stat_attrs.update(kwargs)
         stat_attrs["nyquist_freq"] = nyquist_freq
         t1 = time.time()
         logging.info(f"Time to create overdensity field(s): {t1 - t0:.2f} seconds")
-        bk_res = calculate_bk_sugi_box(fa, fb, fc, correlation_mode, stat_attrs, **kwargs)
+        if geometry == "box-like":
+            bk_res = calculate_bk_sugi_box(fa, fb, fc, correlation_mode, stat_attrs, **kwargs)
+        else:
+            bk_res = calculate_bk_sugi_survey(fa, fb, fc, correlation_mode, stat_attrs, **kwargs)
+        del fa, fb, fc
+        gc.collect()
         bk_res.update(stat_attrs)
-        for key in ("z_range", "comp_weight_plan", "scheme", "alpha_scheme", "normalization_scheme"):
+        if geometry == "box-like":
+            drop = ("z_range", "comp_weight_plan", "scheme", "alpha_scheme", "normalization_scheme")
+        else:
+            drop = ("redshift_box", "los", "apply_rsd")
+        for key in drop:
             bk_res.pop(key, None)
         out_dir = kwargs.get("output_dir")
         os.makedirs(out_dir, exist_ok=True)

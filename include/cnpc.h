@@ -153,6 +153,84 @@ int cnpc_zero(cnpc_ctx* ctx, void* ptr, int64_t nbytes);
 /* Device synchronisation of the context stream. */
 int cnpc_sync(cnpc_ctx* ctx);
 
+/* ------------------------------------------------------------------ survey
+ * Survey fields live on full N^3 complex meshes ("full"), index
+ * (ix*N + iy)*N + iz, as the reference keeps them on complex128 c2c meshes. */
+
+/* Unnormalised 3-D complex-to-complex FFT of full meshes, out of place.
+ * dir: -1 forward (e^{-ik.x}), +1 inverse.  Replaces pmesh r2c / c2r of the
+ * survey estimators' complex fields (clustering_estimator.py:3116-3126,
+ * :1711-1717, :2257-2269, :2375-2381). */
+int cnpc_fft_c2c(cnpc_ctx* ctx, int nmesh, int batch, int prec, int dir, void* in, void* out);
+
+/* Full spectrum of a painted field from its half spectra:
+ * out(k) = scale * comp(k) * combine(c1, c2)(k) on the whole N^3 grid, with
+ * combine as in cnpc_prepare_spectrum.  Gives r2c(rfield) * scale, with the
+ * non-Hermitian Nyquist planes of the interlaced field intact
+ * (mesh_generator.py:207-219 followed by clustering_estimator.py:334-343). */
+int cnpc_full_spectrum(cnpc_ctx* ctx, int nmesh, double boxsize, int prec, const void* c1, const void* c2,
+                       int interlaced, int comp, int window, double scale, void* out);
+
+/* out(x) = F(x) * w(xhat) on a full mesh, xhat the unit vector of the cell
+ * centre x_wrapped + H/2 + boxcenter.  out may alias F.
+ * Replaces rfield * Y_lm(xgrid) (get_xgrid math_evaluator.py:25-32;
+ * clustering_estimator.py:3116, :3120, :1710, :2256, :2374). */
+int cnpc_xweight(cnpc_ctx* ctx, int nmesh, double boxsize, int prec, const double* boxcenter, const cnpc_poly* w,
+                 const void* F, void* out);
+
+/* out(k) (+)= c * w(khat) * [comp(k) * (A op B)(k) - s * C_bk(k)] on a full
+ * spectrum.  opb: 0 A, 1 A*B, 2 A*conj(B) (B may alias A); comp codes as in
+ * cnpc_prepare_spectrum; shot != 0 subtracts s times the bk shot-noise
+ * window prod_d C_p/sinc^{2p}; w may be NULL (w = 1); accumulate != 0 adds
+ * to out.  out may alias A.  Replaces the Y_lm(kgrid) weighting, the
+ * compensation .apply() calls and the S_LM subtraction of
+ * clustering_estimator.py:3117-3134, :1711-1714, :2259-2294, :2376-2396,
+ * :2541-2564. */
+int cnpc_kcombine(cnpc_ctx* ctx, int nmesh, double boxsize, int prec, const void* A, const void* B, int opb, int comp,
+                  int window, int shot, double s_re, double s_im, const cnpc_poly* w, double c_re, double c_im,
+                  int accumulate, void* out);
+
+/* Per-bin sums of (A op B)(k) over the full grid into the device FP64 array
+ * out = [re(nbins), im(nbins)].  opb as in cnpc_kcombine.
+ * Replaces radial_binning on complex c2c fields (math_evaluator.py:504-539;
+ * clustering_estimator.py:456, :2306, :2398). */
+int cnpc_full_bin(cnpc_ctx* ctx, int nmesh, double boxsize, int prec, const void* A, const void* B, int opb,
+                  const double* edges, int nbins, double* out);
+
+/* (re, im) of sum_x A(x)*B(x)*C(x) over count complex values (B, C may be
+ * NULL) into the device FP64 array out[2]; FP64, deterministic order.
+ * Replaces the mixed-mesh / mesh normalisation sums
+ * (mesh_generator.py:499-504, :828, :920-922, :940-987). */
+int cnpc_triple_sum(cnpc_ctx* ctx, int prec, int64_t count, const void* A, const void* B, const void* C, double* out);
+
+/* Split count complex values into real and imaginary arrays (same precision). */
+int cnpc_split_complex(cnpc_ctx* ctx, int prec, int64_t count, const void* src, void* re, void* im);
+
+/* Full spectrum on N^3 -> its |n_d| <= nmax modes on an M^3 full grid
+ * (M > 2 nmax, other modes zero).  Spectral truncation used by the reduced
+ * contraction grid; no reference counterpart (the reference contracts on N^3). */
+int cnpc_resample_full(cnpc_ctx* ctx, int n, int m, int prec, const void* in, int nmax, void* out);
+
+/* FP64 catalog sums over np objects into the device FP64 array out:
+ *   out[0] sum W*Wfkp, out[1] sum W, out[2] sum W*NZ*Wfkp^2,
+ *   out[3] sum (W*Wfkp)^2, out[4] sum NZ^2 W^2 Wfkp^3, then for each of the
+ *   npoly weights (re, im) of sum (W*Wfkp)^3 * w(pos/|pos|).
+ * NZ may be NULL (1).  Replaces compute_particle_sums_pk / _bk
+ * (mesh_generator.py:1131-1268). */
+int cnpc_particle_sums(cnpc_ctx* ctx, int64_t np, const double* pos_xyz, const double* W, const double* Wfkp,
+                       const double* NZ, int npoly, const cnpc_poly* polys, double* out);
+
+/* out[i] = scale * (W[i] * Wfkp[i])^power, power 1 or 2 (Wfkp may be NULL).
+ * The paint weights of mesh_generator.py:362-382, :1084-1120. */
+int cnpc_pair_weights(cnpc_ctx* ctx, int64_t np, const double* W, const double* Wfkp, int power, double scale,
+                      double* out);
+
+/* RA/DEC (degrees) and redshift -> comoving Cartesian positions (np x 3,
+ * Mpc/h) in flat LCDM without radiation (h only sets units: Mpc/h).
+ * Replaces ra_dec_z_to_xyz (catalog_processor.py:589-645). */
+int cnpc_radecz_to_xyz(cnpc_ctx* ctx, int64_t np, const double* ra, const double* dec, const double* z, double omega_m,
+                       double* pos_xyz);
+
 #ifdef __cplusplus
 }
 #endif
