@@ -66,6 +66,7 @@ SIGNATURES = {
     "cnpc_triple_sum": [_P, _I, _L, _P, _P, _P, _P],
     "cnpc_split_complex": [_P, _I, _L, _P, _P, _P],
     "cnpc_resample_full": [_P, _I, _I, _I, _P, _I, _P],
+    "cnpc_reflect_conj": [_P, _I, _I, _P, _D, _D, _P],
     "cnpc_particle_sums": [_P, _L, _P, _P, _P, _P, _I, _P, _P],
     "cnpc_pair_weights": [_P, _L, _P, _P, _I, _D, _P],
     "cnpc_radecz_to_xyz": [_P, _L, _P, _P, _P, _D, _P],

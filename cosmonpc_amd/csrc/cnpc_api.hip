@@ -564,6 +564,15 @@ int cnpc_resample_full(cnpc_ctx* ctx, int n, int m, int prec, const void* in, in
   return 0;
 }
 
+int cnpc_reflect_conj(cnpc_ctx* ctx, int n, int prec, const void* A, double c1, double c2, void* out) {
+  REQUIRE(ctx, "null context");
+  REQUIRE(n >= 2 && n % 2 == 0, "reflect_conj: bad mesh");
+  REQUIRE(prec == 0 || prec == 1, "reflect_conj: prec must be 0 or 1");
+  REQUIRE(A && out && A != out, "reflect_conj: null or aliased buffers");
+  CHECK_HIP(launch_reflect(prec, n, c1, c2, A, out, ctx->stream));
+  return 0;
+}
+
 int cnpc_particle_sums(cnpc_ctx* ctx, int64_t np, const double* pos, const double* W, const double* Wfkp,
                        const double* NZ, int npoly, const cnpc_poly* polys, double* out) {
   REQUIRE(ctx, "null context");

@@ -264,6 +264,7 @@ hipError_t launch_triple_sum(int prec, int64_t count, const void* A, const void*
                              double* out, hipStream_t st);
 hipError_t launch_split(int prec, int64_t count, const void* src, void* re, void* im, hipStream_t st);
 hipError_t launch_resample_full(int prec, int n, int m, int nmax, const void* in, void* out, hipStream_t st);
+hipError_t launch_reflect(int prec, int n, double c1, double c2, const void* A, void* out, hipStream_t st);
 int particle_sums_blocks(int64_t np);
 hipError_t launch_particle_sums(int64_t np, const double* pos, const double* W, const double* WF, const double* NZ,
                                 const PolySet& P, int npoly, double* partial, double* out, hipStream_t st);

@@ -16,6 +16,7 @@
 //   triple_sum     sum_x A*B*C (complex, FP64 partials)
 //   split          complex mesh -> real and imaginary meshes
 //   resample_full  full spectrum on N^3 -> its |n_d| <= nmax modes on M^3
+//   reflect        c1 A(k) + c2 conj(A(-k))
 //   particle_sums  FP64 catalog sums (alpha, I22, I33, N0, S_LM)
 //   pair_weights   (W * W_fkp)^p * scale per object
 //   radecz         RA/DEC/Z -> comoving Cartesian positions (flat LCDM)
@@ -272,6 +273,26 @@ __global__ void __launch_bounds__(256) resample_full_kernel(int n, int m, int nm
   }
 }
 
+// ------------------------------------------------------------- reflection
+// out(k) = c1 * A(k) + c2 * conj(A(-k)), -k the index map i -> (-i) mod N on
+// every axis (space_inversion_transposed_complex, math_evaluator.py:806-977).
+template <typename T>
+__global__ void __launch_bounds__(256) reflect_kernel(int n, double c1, double c2, const C2<T>* __restrict__ A,
+                                                      C2<T>* __restrict__ out) {
+  const int64_t total = (int64_t)n * n * n;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (int64_t)gridDim.x * blockDim.x) {
+    int ix, iy, iz;
+    full_index(idx, n, &ix, &iy, &iz);
+    const int jx = ix == 0 ? 0 : n - ix, jy = iy == 0 ? 0 : n - iy, jz = iz == 0 ? 0 : n - iz;
+    const C2<T> a = A[idx];
+    const C2<T> b = A[((int64_t)jx * n + jy) * n + jz];
+    C2<T> o;
+    o.x = (T)(c1 * (double)a.x + c2 * (double)b.x);
+    o.y = (T)(c1 * (double)a.y - c2 * (double)b.y);
+    out[idx] = o;
+  }
+}
+
 // ------------------------------------------------------------- catalog sums
 // Per-object terms (mesh_generator.py:1131-1268):
 //   s0 W*Wfkp, s1 W, s2 W*NZ*Wfkp*Wfkp, s3 (W*Wfkp)^2, s4 NZ^2*W^2*Wfkp^3,
@@ -456,6 +477,17 @@ hipError_t launch_resample_full(int prec, int n, int m, int nmax, const void* in
   else
     hipLaunchKernelGGL(resample_full_kernel<float>, grid_n(total), dim3(256), 0, st, n, m, nmax,
                        (const float2*)in, (float2*)out);
+  return hipGetLastError();
+}
+
+hipError_t launch_reflect(int prec, int n, double c1, double c2, const void* A, void* out, hipStream_t st) {
+  const int64_t total = (int64_t)n * n * n;
+  if (prec == 0)
+    hipLaunchKernelGGL(reflect_kernel<double>, grid_n(total), dim3(256), 0, st, n, c1, c2, (const double2*)A,
+                       (double2*)out);
+  else
+    hipLaunchKernelGGL(reflect_kernel<float>, grid_n(total), dim3(256), 0, st, n, c1, c2, (const float2*)A,
+                       (float2*)out);
   return hipGetLastError();
 }
 

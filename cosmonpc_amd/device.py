@@ -297,6 +297,14 @@ class GpuContext:
         check(self.lib.cnpc_resample_full(self.h, n, m, prec, _ptr(src), int(nmax), _ptr(out)), "cnpc_resample_full")
         return out
 
+    def reflect_conj(self, A, n, prec, c1=0.5, c2=0.5):
+        """c1 A(k) + c2 conj(A(-k)) as a new full spectrum."""
+        self._check_full(A, n, prec)
+        out = self.full(n, prec)
+        check(self.lib.cnpc_reflect_conj(self.h, n, prec, _ptr(A), float(c1), float(c2), _ptr(out)),
+              "cnpc_reflect_conj")
+        return out
+
     @staticmethod
     def _check_vec(t, np_, what):
         if t is None:

@@ -239,12 +239,38 @@ def _G_ell(ctx, field, ell, comp):
     return G
 
 
+def high_order_strategy(poles, high_order_mode, correlation_mode):
+    """``get_high_order_strategy`` (``clustering_estimator.py:243-331``):
+    which exact and fast (P4b, P6b, P8b) multipoles to form."""
+    supported = ([0, 2, 4], [0, 2, 4, 6], [0, 2, 4, 6, 8])
+    if high_order_mode == "default":
+        logging.info("[high-order mode] using exact multipoles only.")
+        return None
+    if correlation_mode != "auto" or poles not in supported:
+        logging.info("[high-order mode] disabled (auto-correlation and poles "
+                     f"{list(supported)} only).")
+        return None
+    high = {p for p in poles if p in (4, 6, 8)}
+    fast, exact, internal = set(), set(), set()
+    if high_order_mode == "fast":
+        fast = {(0, 2, 4): {4}, (0, 2, 4, 6): {6}, (0, 2, 4, 6, 8): {6, 8}}.get(tuple(poles), set())
+        exact = {(0, 2, 4): set(), (0, 2, 4, 6): {4}, (0, 2, 4, 6, 8): {4}}.get(tuple(poles), set())
+    elif high_order_mode == "compare":
+        fast, exact = set(high), set(high)
+    if any(p in fast for p in (6, 8)):
+        internal.add(4)  # P6b and P8b use the exact P4
+    return {"mode": high_order_mode, "compute_fast_poles": fast, "compute_exact_poles": exact | internal,
+            "visible_exact_poles": exact, "need_f2_cache": any(p in fast for p in (4, 6, 8)),
+            "need_f4_cache": any(p in fast for p in (6, 8))}
+
+
 def calculate_power_spectrum_survey(stat_attrs, rfield_a, rfield_b, correlation_mode, comm=None, **kwargs):
-    """``clustering_estimator.py:205-563`` (``high_order_mode`` "default")."""
+    """``clustering_estimator.py:205-563``, including the ``high_order_mode``
+    fast / compare estimators P4b, P6b, P8b built from
+    F_l = (G_l(k) + conj G_l(-k)) / 2 (:429-444, :487-550)."""
     ctx = get_context()
     poles = stat_attrs["poles"]
-    if stat_attrs.get("high_order_mode", "default") != "default":
-        raise NotImplementedError("high_order_mode 'fast' / 'compare' is not available on the GPU path yet")
+    strat = high_order_strategy(poles, stat_attrs.get("high_order_mode", "default"), correlation_mode)
     k_min, k_max, k_bins = stat_attrs["k_min"], stat_attrs["k_max"], stat_attrs["k_bins"]
     interlaced = stat_attrs["interlaced"]
     N0 = stat_attrs["N0"]
@@ -267,6 +293,7 @@ def calculate_power_spectrum_survey(stat_attrs, rfield_a, rfield_b, correlation_
     k_num = count
     with np.errstate(invalid="ignore", divide="ignore"):
         results = {"k_eff": ksum / k_num, "k_num": k_num, "I_norm": I_norm}
+    F2, F4 = None, None
     for pole in poles:
         if pole == 0:
             if correlation_mode == "auto":
@@ -274,7 +301,21 @@ def calculate_power_spectrum_survey(stat_attrs, rfield_a, rfield_b, correlation_
             else:
                 s = ctx.full_bin(ca, n, L, prec, k_edge, B=cb, opb=2)
         else:
+            store = True
+            if pole in (4, 6, 8) and strat is not None:
+                store = pole in strat["compute_exact_poles"]
+            cache2 = pole == 2 and strat is not None and strat["need_f2_cache"]
+            cache4 = pole == 4 and strat is not None and strat["need_f4_cache"]
+            if not store and not cache2 and not cache4:
+                continue
             G = _G_ell(ctx, rfield_b, pole, comp)
+            if cache2:
+                F2 = ctx.reflect_conj(G, n, prec)
+            if cache4:
+                F4 = ctx.reflect_conj(G, n, prec)
+            if not store:
+                del G
+                continue
             s = ctx.full_bin(ca, n, L, prec, k_edge, B=G, opb=2)
             del G
         with np.errstate(invalid="ignore", divide="ignore"):
@@ -285,6 +326,26 @@ def calculate_power_spectrum_survey(stat_attrs, rfield_a, rfield_b, correlation_
         else:
             res = np.imag(res) if pole & 1 else np.real(res)
         results[f"P{pole}"] = res
+    if strat is not None and strat["compute_fast_poles"]:
+        fast = strat["compute_fast_poles"]
+        with np.errstate(invalid="ignore", divide="ignore"):
+            if 4 in fast:
+                P22 = ctx.full_bin(F2, n, L, prec, k_edge, B=F2, opb=2) / k_num
+                P22 = P22 * (9 / I_norm) - N0 * 9 / 5
+                results["P4b"] = np.real(35 / 18 * P22 - results["P2"] - 7 / 2 * results["P0"])
+            if 6 in fast or 8 in fast:
+                s42 = ctx.full_bin(F4, n, L, prec, k_edge, B=F2, opb=2)
+                P42 = s42 / k_num * (13 / I_norm)
+                if 6 in fast:
+                    results["P6b"] = np.real(11 / 5 * P42 - 52 / 63 * results["P4"] - 286 / 175 * results["P2"])
+                if 8 in fast:
+                    P44 = ctx.full_bin(F4, n, L, prec, k_edge, B=F4, opb=2) / k_num * (17 / I_norm)
+                    P44 = P44 - N0 * 17 / 9
+                    results["P8b"] = np.real(1287 / 490 * P44 - 374 / 245 * P42 - 3553 / 15435 * results["P4"]
+                                             - 1326 / 8575 * results["P2"] - 2431 / 490 * results["P0"])
+        if strat["mode"] == "fast":
+            for pole in fast:
+                results.pop(f"P{pole}", None)
     return results
 
 

@@ -211,6 +211,12 @@ int cnpc_split_complex(cnpc_ctx* ctx, int prec, int64_t count, const void* src, 
  * contraction grid; no reference counterpart (the reference contracts on N^3). */
 int cnpc_resample_full(cnpc_ctx* ctx, int n, int m, int prec, const void* in, int nmax, void* out);
 
+/* out(k) = c1 * A(k) + c2 * conj(A(-k)) on a full spectrum, -k the wrapped
+ * index inverse on every axis (out must not alias A).  Replaces
+ * G + conj(space_inversion_transposed_complex(G)) of the survey pk fast
+ * estimators (clustering_estimator.py:429-444; math_evaluator.py:806-977). */
+int cnpc_reflect_conj(cnpc_ctx* ctx, int nmesh, int prec, const void* A, double c1, double c2, void* out);
+
 /* FP64 catalog sums over np objects into the device FP64 array out:
  *   out[0] sum W*Wfkp, out[1] sum W, out[2] sum W*NZ*Wfkp^2,
  *   out[3] sum (W*Wfkp)^2, out[4] sum NZ^2 W^2 Wfkp^3, then for each of the

@@ -149,11 +149,37 @@ def get_G_ell(rfield, ell, kgrid, xgrid, comp, w, boxsize):
     return G * (4 * np.pi * float(boxsize) ** 3 / (2 * ell + 1))
 
 
+def space_inversion(G):
+    """G(-k) on the full grid: index i -> (-i) mod N on every axis (the
+    single-process form of ``space_inversion_transposed_complex``,
+    ``math_evaluator.py:806-977``, with ``_get_flip_partner`` :618-629)."""
+    n = G.shape[0]
+    inv = (-np.arange(n)) % n
+    return G[np.ix_(inv, inv, inv)]
+
+
+def high_order_strategy(poles, high_order_mode, correlation_mode):
+    """``get_high_order_strategy`` (``clustering_estimator.py:243-331``)."""
+    supported = ([0, 2, 4], [0, 2, 4, 6], [0, 2, 4, 6, 8])
+    if high_order_mode == "default" or correlation_mode != "auto" or poles not in supported:
+        return None
+    high = {p for p in poles if p in (4, 6, 8)}
+    fast, exact, internal = set(), set(), set()
+    if high_order_mode == "fast":
+        fast = {(0, 2, 4): {4}, (0, 2, 4, 6): {6}, (0, 2, 4, 6, 8): {6, 8}}.get(tuple(poles), set())
+        exact = {(0, 2, 4): set(), (0, 2, 4, 6): {4}, (0, 2, 4, 6, 8): {4}}.get(tuple(poles), set())
+    elif high_order_mode == "compare":
+        fast, exact = set(high), set(high)
+    if any(p in fast for p in (6, 8)):
+        internal.add(4)
+    return {"mode": high_order_mode, "compute_fast_poles": fast, "compute_exact_poles": exact | internal,
+            "visible_exact_poles": exact, "need_f2_cache": any(p in fast for p in (4, 6, 8)),
+            "need_f4_cache": any(p in fast for p in (6, 8))}
+
+
 def calculate_power_spectrum_survey(stat_attrs, rfield_a, rfield_b, correlation_mode):
-    """``clustering_estimator.py:205-563`` (high_order_mode "default")."""
+    """``clustering_estimator.py:205-563``."""
     poles = stat_attrs["poles"]
-    if stat_attrs.get("high_order_mode", "default") != "default":
-        raise NotImplementedError("high_order_mode fast/compare")
     n = int(np.atleast_1d(stat_attrs["nmesh"])[0])
     L = float(np.atleast_1d(stat_attrs["boxsize"])[0])
     boxcenter = np.array(stat_attrs["boxcenter"])
@@ -162,6 +188,7 @@ def calculate_power_spectrum_survey(stat_attrs, rfield_a, rfield_b, correlation_
     N0 = stat_attrs["N0"]
     I_norm = stat_attrs["I_rand"] if stat_attrs["normalization_scheme"] == "particle" else stat_attrs["I_mesh"]
     k_edge = np.linspace(k_min, k_max, k_bins + 1)
+    strat = high_order_strategy(poles, stat_attrs.get("high_order_mode", "default"), correlation_mode)
     comp = A.get_compensation(interlaced, sampler)
     w = C.circular_axes(n)
     V = L ** 3
@@ -171,6 +198,7 @@ def calculate_power_spectrum_survey(stat_attrs, rfield_a, rfield_b, correlation_
     xgrid = A.get_xgrid(C.x_axes(n, L), boxcenter, np.array([L] * 3), np.array([n] * 3))
     k_num, ksum = A.get_kbin_count(k_bins, k_edge, knorm)
     res = {"k_eff": ksum / k_num, "k_num": k_num, "I_norm": I_norm}
+    F2, F4 = None, None
     for pole in poles:
         if pole == 0:
             if correlation_mode == "auto":
@@ -178,7 +206,20 @@ def calculate_power_spectrum_survey(stat_attrs, rfield_a, rfield_b, correlation_
             else:
                 P = ca * np.conj(cb)
         else:
+            store = True
+            if pole in (4, 6, 8) and strat is not None:
+                store = pole in strat["compute_exact_poles"]
+            cache2 = pole == 2 and strat is not None and strat["need_f2_cache"]
+            cache4 = pole == 4 and strat is not None and strat["need_f4_cache"]
+            if not store and not cache2 and not cache4:
+                continue
             G = get_G_ell(rfield_b, pole, kgrid, xgrid, comp, w, L)
+            if cache2:
+                F2 = (G + np.conj(space_inversion(G))) * (1 / 2)
+            if cache4:
+                F4 = (G + np.conj(space_inversion(G))) * (1 / 2)
+            if not store:
+                continue
             P = ca * np.conj(G)
         s = A.radial_binning(P, k_bins, k_edge, knorm)
         r = s / k_num
@@ -188,6 +229,26 @@ def calculate_power_spectrum_survey(stat_attrs, rfield_a, rfield_b, correlation_
         else:
             r = np.imag(r) if pole & 1 else np.real(r)
         res[f"P{pole}"] = r
+    if strat is not None and strat["compute_fast_poles"]:
+        fast = strat["compute_fast_poles"]
+        if 4 in fast:
+            P22 = A.radial_binning(F2 * np.conj(F2), k_bins, k_edge, knorm) / k_num
+            P22 = P22 * (9 / I_norm) - N0 * 9 / 5
+            res["P4b"] = np.real(35 / 18 * P22 - res["P2"] - 7 / 2 * res["P0"])
+        if 6 in fast or 8 in fast:
+            s42 = A.radial_binning(F4 * np.conj(F2), k_bins, k_edge, knorm)
+            if 6 in fast:
+                P42 = s42 / k_num * (13 / I_norm)
+                res["P6b"] = np.real(11 / 5 * P42 - 52 / 63 * res["P4"] - 286 / 175 * res["P2"])
+            if 8 in fast:
+                P42 = s42 / k_num * (13 / I_norm)
+                P44 = A.radial_binning(F4 * np.conj(F4), k_bins, k_edge, knorm) / k_num * (17 / I_norm)
+                P44 = P44 - N0 * 17 / 9
+                res["P8b"] = np.real(1287 / 490 * P44 - 374 / 245 * P42 - 3553 / 15435 * res["P4"]
+                                     - 1326 / 8575 * res["P2"] - 2431 / 490 * res["P0"])
+        if strat["mode"] == "fast":
+            for pole in fast:
+                res.pop(f"P{pole}", None)
     return res
 
 

@@ -95,6 +95,28 @@ def test_pk_survey_vs_oracle(gpu_ctx, corr, ns, sampler, interlaced, poles):
         _close(res[f"P{p}"], ores[f"P{p}"], what=f"P{p}")
 
 
+@pytest.mark.parametrize("poles,mode", [([0, 2, 4], "fast"), ([0, 2, 4, 6], "fast"), ([0, 2, 4, 6, 8], "compare"),
+                                        ([0, 2, 4, 6, 8], "fast")])
+def test_pk_survey_high_order_modes(gpu_ctx, poles, mode):
+    from cosmonpc_amd import survey as GS
+    from oracle.cnpc_ref import survey as S
+
+    print(f"[pk survey high-order] {poles} {mode}", flush=True)
+    rows = _rows(["a"], seed=61)
+    kw = dict(k_min=0.01, k_max=0.24, k_bins=8, poles=poles, boxsize=[L] * 3, nmesh=[N] * 3, sampler="tsc",
+              interlaced=True, normalization_scheme="particle", high_order_mode=mode)
+    attrs, fa, fb = GS.get_mesh_pk_survey(None, "auto", [N] * 3, "survey-like", ["RA", "DEC", "Z"], [L] * 3, "tsc",
+                                          True, None, NO_COMP, COSMO, normalization_scheme="particle", arrays=rows)
+    res = GS.calculate_power_spectrum_survey(dict(attrs, **kw), fa, fb, "auto")
+    cats = oracle_survey_cats(rows)
+    oattrs, oa, ob = S.get_mesh_pk_survey(cats, "auto", N, L, "tsc", True, "particle")
+    ores = S.calculate_power_spectrum_survey(dict(oattrs, **kw), oa, ob, "auto")
+    assert set(res) == set(ores)
+    for key in ores:
+        if key.startswith("P"):
+            _close(res[key], ores[key], what=key)
+
+
 BK_CASES = [
     ("auto", "aaa", [0, 0, 0], "full", "particle", "both", "tsc", True, "gemm_fp64"),
     ("auto", "aaa", [2, 0, 2], "diagonal", "mesh", "ana", "cic", False, "gemm_fp64"),

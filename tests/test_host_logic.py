@@ -121,3 +121,74 @@ def test_catalog_reader_box_npy(tmp_path):
     assert np.all(ww == 1.0)
     with pytest.raises(ValueError, match="Unsupported data file type: txt"):
         catalog.read_box_catalog("cat.txt", cols)
+
+
+def test_catalog_reader_box_fits_and_rsd(tmp_path):
+    """Box FITS catalogs (catalog_processor.py:388-449): X/Y/Z or x/y/z
+    positions, weight 'w', RSD shift (1+z)/(100 E(z)) v along the LOS with a
+    periodic wrap; E(z) = (1+z)^1.5 for Omega_m = 1."""
+    from cosmonpc_amd import catalog, fits
+
+    rng = np.random.default_rng(1)
+    pos = rng.uniform(0, 100, (200, 3))
+    vel = rng.normal(0, 300, (200, 3))
+    w = rng.uniform(0.5, 1.5, 200)
+    f = tmp_path / "box.fits"
+    fits.write_table(str(f), {"X": pos[:, 0], "Y": pos[:, 1], "Z": pos[:, 2], "VX": vel[:, 0], "VY": vel[:, 1],
+                              "VZ": vel[:, 2], "w": w})
+    cols = ["X", "Y", "Z", "VX", "VY", "VZ", "w"]
+    p, ww = catalog.read_box_catalog(str(f), cols)
+    np.testing.assert_array_equal(p, pos)
+    np.testing.assert_array_equal(ww, w)
+    zb = 0.5
+    p, _ = catalog.read_box_catalog(str(f), cols, apply_rsd=True, para_cosmo={"h": 0.7, "Omega0": 1.0},
+                                    boxsize=[100.0] * 3, redshift_box=zb, los=[0, 0, 1])
+    fac = (1 + zb) / ((1 + zb) ** 1.5 * 100)
+    np.testing.assert_allclose(p[:, 2], np.mod(pos[:, 2] + fac * vel[:, 2], 100.0), rtol=1e-13)
+    np.testing.assert_array_equal(p[:, :2], pos[:, :2])
+    with pytest.raises(ValueError, match="LOS must be a 3d unit vector"):
+        catalog.read_box_catalog(str(f), cols, apply_rsd=True, para_cosmo={"h": 0.7, "Omega0": 0.3},
+                                 boxsize=[100.0] * 3, redshift_box=zb, los=[0, 1, 1])
+    # columns not requested are not read: no weight column -> 1
+    p, ww = catalog.read_box_catalog(str(f), ["X", "Y", "Z"])
+    assert np.all(ww == 1.0)
+
+
+def test_fits_format_cards_and_foreign_file(tmp_path):
+    """Header cards follow the FITS fixed format (keyword in columns 1-8,
+    '= ' in 9-10, values right-justified to column 30), and a table written
+    by an independent byte-level writer reads back."""
+    import struct
+
+    from cosmonpc_amd import fits
+
+    f = tmp_path / "a.fits"
+    fits.write_table(str(f), {"RA": np.array([1.5, 2.5]), "N": np.array([3, 4], dtype=np.int32)})
+    raw = f.read_bytes()
+    assert len(raw) % 2880 == 0
+    assert raw[:80].decode() == "SIMPLE  =" + " " * 20 + "T" + " " * 50
+    ext = raw[2880:2880 + 2880].decode()
+    assert ext[:80].startswith("XTENSION= 'BINTABLE'")
+    assert "TFORM1  = 'D       '" in ext and "TFORM2  = 'J       '" in ext
+
+    def card(s):
+        return s.ljust(80).encode()
+
+    hdr0 = card("SIMPLE  =                    T") + card("BITPIX  =                    8") + \
+        card("NAXIS   =                    0") + card("END")
+    hdr1 = b"".join(card(c) for c in [
+        "XTENSION= 'BINTABLE'", "BITPIX  =                    8", "NAXIS   =                    2",
+        "NAXIS1  =                   12", "NAXIS2  =                    3", "PCOUNT  =                    0",
+        "GCOUNT  =                    1", "TFIELDS =                    2", "TTYPE1  = 'Z       '",
+        "TFORM1  = 'D       '", "TTYPE2  = 'WEIGHT_FKP'", "TFORM2  = 'E       '", "END"])
+    data = b"".join(struct.pack(">df", z, w) for z, w in [(0.5, 1.0), (0.6, 0.25), (0.7, 2.0)])
+
+    def pad(b, c):
+        return b + c * ((-len(b)) % 2880)
+
+    g = tmp_path / "b.fits"
+    g.write_bytes(pad(hdr0, b" ") + pad(hdr1, b" ") + pad(data, b"\0"))
+    t = fits.read_table(str(g), ["WEIGHT_FKP", "Z", "missing"])
+    assert t.dtype.names == ("WEIGHT_FKP", "Z")
+    np.testing.assert_array_equal(t["Z"], [0.5, 0.6, 0.7])
+    np.testing.assert_array_equal(t["WEIGHT_FKP"], np.array([1.0, 0.25, 2.0], dtype=np.float32))

@@ -154,3 +154,29 @@ def test_bk_survey_signal_distant_observer_equals_box(angu, dvm):
     bsa = dict(sa, NZ_a=NZ, N_gal_a=1.0)
     box = B.calculate_bk_sugi_box(rbox, None, None, "auto", bsa, return_signal_only=True)
     np.testing.assert_allclose(sig, (L ** 3) ** 2 * box, rtol=1e-7, atol=1e-7 * np.max(np.abs(sig)))
+
+
+@pytest.mark.parametrize("poles,mode", [([0, 2, 4], "fast"), ([0, 2, 4, 6], "compare"), ([0, 2, 4, 6, 8], "compare"),
+                                        ([0, 2, 4, 6, 8], "fast")])
+def test_pk_survey_fast_modes_distant_observer(poles, mode):
+    """Distant observer: F_l = G_l and L_2^2, L_4 L_2, L_4^2 expand in
+    Legendre polynomials with the coefficients behind P4b/P6b/P8b, so the fast
+    estimators equal the exact P4/P6/P8 (and the output keys follow the
+    reference's fast/compare tables)."""
+    n, L = 16, 200.0
+    cats = _far_catalogs(1e12, 7, L=L)
+    attrs, fa, fb = S.get_mesh_pk_survey(cats, "auto", n, L, "cic", True, "particle")
+    sa = dict(attrs, poles=poles, boxsize=[L] * 3, nmesh=[n] * 3, k_min=0.02, k_max=0.2, k_bins=5, sampler="cic",
+              interlaced=True, normalization_scheme="particle")
+    exact = S.calculate_power_spectrum_survey(dict(sa, poles=[0, 2, 4, 6, 8]), fa, fb, "auto")
+    res = S.calculate_power_spectrum_survey(dict(sa, high_order_mode=mode), fa, fb, "auto")
+    fast = {(0, 2, 4): [4], (0, 2, 4, 6): [6], (0, 2, 4, 6, 8): [6, 8]}[tuple(poles)] if mode == "fast" \
+        else [p for p in poles if p >= 4]
+    for p in fast:
+        scale = np.max(np.abs(exact[f"P{p}"])) + np.max(np.abs(exact["P0"])) * 1e-3
+        np.testing.assert_allclose(res[f"P{p}b"], exact[f"P{p}"], rtol=0, atol=1e-6 * scale)
+    keys = {k for k in res if k.startswith("P")}
+    want = {f"P{p}" for p in poles} | {f"P{p}b" for p in fast}
+    if mode == "fast":
+        want -= {f"P{p}" for p in fast}
+    assert keys == want
