@@ -206,7 +206,7 @@ def kernel_roofline(ctx, delta, n, L, k_edge, torch, np, A):
         e1.synchronize()
         return e0.elapsed_time(e1) / reps
 
-    # production path: pruned synthesis (x-DFT kernels + batched 2-D c2r)
+    # production path: pruned shell synthesis (cube, x, y and z FFT kernels)
     t_synth = []
     for b0 in range(0, K, batch):
         c = min(batch, K - b0)

@@ -235,6 +235,26 @@ int cnpc_paint(cnpc_ctx* ctx, const double* pos, const double* w, int64_t np, in
   return 0;
 }
 
+int cnpc_paint_batch(cnpc_ctx* ctx, const double* pos, const double* w, const int64_t* offsets, int nbatch,
+                     int64_t max_count, int n, double boxsize, int window, int interlaced, const double* center, int prec,
+                     void* mesh0, void* mesh1) {
+  REQUIRE(ctx, "null context");
+  REQUIRE(n >= 2 && n % 2 == 0 && n <= 4096, "cnpc_paint_batch: nmesh must be even and in [2, 4096]");
+  REQUIRE(window >= 1 && window <= 4, "cnpc_paint_batch: window must be 1..4 (ngp, cic, tsc, pcs)");
+  REQUIRE(prec == 0 || prec == 1, "cnpc_paint_batch: prec must be 0 (fp64) or 1 (fp32)");
+  REQUIRE(boxsize > 0, "cnpc_paint_batch: boxsize must be positive");
+  REQUIRE(nbatch >= 1 && nbatch <= 65535 && max_count >= 0, "cnpc_paint_batch: bad batch size");
+  REQUIRE(offsets && mesh0 && (!interlaced || mesh1), "cnpc_paint_batch: null offsets or meshes");
+  REQUIRE(max_count == 0 || (pos && w), "cnpc_paint_batch: null particle arrays");
+  const int64_t bytes = (int64_t)nbatch * n * n * n * (prec == 0 ? 8 : 4);
+  if (int rc = cnpc_zero(ctx, mesh0, bytes)) return rc;
+  if (interlaced)
+    if (int rc = cnpc_zero(ctx, mesh1, bytes)) return rc;
+  CHECK_HIP(launch_paint_batch(pos, w, offsets, nbatch, max_count, n, boxsize, window, center, interlaced ? 2 : 1,
+                               prec, mesh0, interlaced ? mesh1 : nullptr, ctx->stream));
+  return 0;
+}
+
 static int run_fft(cnpc_ctx* ctx, int n, int batch, int prec, int dir, void* in, void* out, int dims = 3) {
   REQUIRE(ctx, "null context");
   REQUIRE(n >= 2 && n % 2 == 0 && batch >= 1, "fft: bad size");
@@ -329,23 +349,20 @@ int cnpc_shell_synth(cnpc_ctx* ctx, int n, double boxsize, int prec, const void*
   // every mode with |k| < edges[nbins] has |n_d| <= R
   const int R = (int)std::ceil(edges[nbins] / kf);
   REQUIRE(R < n / 2, "shell_synth: k_max reaches the Nyquist index (needs edges[nbins] < pi n / L)");
-  int zb = 32;
-  while (zb > 8 && xsynth_lds_bytes(prec, n, R, nshell, zb) > 64 * 1024) zb /= 2;
-  REQUIRE(xsynth_lds_bytes(prec, n, R, nshell, zb) <= 64 * 1024, "shell_synth: k_max too large for the LDS tile");
   SynthArgs a;
   std::memset(&a, 0, sizeof(a));
   a.n = n; a.R = R; a.bin0 = bin0; a.nshell = nshell; a.kf = kf;
-  int rmin = 8;
-  {  // mixed-radix factorisation of n into 4, 2, 3, 5
+  {  // mixed-radix factorisation of n, odd radices first (their Stockham
+     // writes stride by an odd count and stay off the LDS bank conflicts)
     int m = n, k = 0;
-    while (m % 4 == 0 && k < CNPC_MAX_RADICES) { a.rad[k++] = 4; m /= 4; }
-    for (int r : {2, 3, 5})
+    for (int r : {5, 3})
       while (m % r == 0 && k < CNPC_MAX_RADICES) { a.rad[k++] = r; m /= r; }
+    while (m % 4 == 0 && k < CNPC_MAX_RADICES) { a.rad[k++] = 4; m /= 4; }
+    while (m % 2 == 0 && k < CNPC_MAX_RADICES) { a.rad[k++] = 2; m /= 2; }
     REQUIRE(m == 1, "shell_synth: nmesh must factor into 2, 3 and 5");
     a.nrad = k;
-    for (int i = 0; i < k; ++i) rmin = a.rad[i] < rmin ? a.rad[i] : rmin;
   }
-  REQUIRE(synth_lines_per_block(prec, n, rmin) >= 1, "shell_synth: mesh too large for the LDS line buffers");
+  REQUIRE(synth_lines_per_block(prec, a) >= 1, "shell_synth: mesh too large for the LDS line buffers");
   const size_t cs = prec == 0 ? 16 : 8;
   const size_t P = 2 * (size_t)R + 1, Q = (size_t)R + 1, cube = P * Q * P;
   // work holds U (nshell n P Q) and T (nshell n Q n); its size is nshell
@@ -359,7 +376,7 @@ int cnpc_shell_synth(cnpc_ctx* ctx, int n, double boxsize, int prec, const void*
   PolySpec W;
   to_spec(W, *weight);
   char* base = (char*)ctx->synth;
-  CHECK_HIP(launch_synth(prec, a, E, W, D, base, (signed char*)(base + vbytes), base + vbytes + bbytes, work, out, zb,
+  CHECK_HIP(launch_synth(prec, a, E, W, D, base, (signed char*)(base + vbytes), base + vbytes + bbytes, work, out,
                          ctx->stream));
   return 0;
 }

@@ -273,13 +273,15 @@ hipError_t launch_pair_weights(int64_t np, const double* W, const double* WF, in
 hipError_t launch_radecz(int64_t np, const double* ra, const double* dec, const double* z, const RadeczArgs& a,
                          double* pos, hipStream_t st);
 
-size_t xsynth_lds_bytes(int prec, int n, int R, int nshell, int zb);
-int synth_lines_per_block(int prec, int n, int rmin);
+int synth_lines_per_block(int prec, const SynthArgs& a);
 hipError_t launch_synth(int prec, const SynthArgs& a, const BinEdges& E, const PolySpec& W, const void* D, void* V,
-                        signed char* bins, void* tw, void* work, void* out, int zb, hipStream_t st);
+                        signed char* bins, void* tw, void* work, void* out, hipStream_t st);
 
 hipError_t launch_paint(const double* pos, const double* w, int64_t np, int n, double boxsize, int order,
                         const double* center, int ncopy, int prec, void* m0, void* m1, hipStream_t st);
+hipError_t launch_paint_batch(const double* pos, const double* w, const int64_t* off, int nb, int64_t np_max, int n,
+                              double boxsize, int order, const double* center, int ncopy, int prec, void* m0, void* m1,
+                              hipStream_t st);
 hipError_t launch_prepare(int prec, const PrepareArgs& a, const void* c1, const void* c2, void* out, hipStream_t st);
 hipError_t launch_bin(int prec, const BinArgs& a, const BinEdges& E, const PolySet& P, const void* A, const void* B,
                       double* out, hipStream_t st);

@@ -88,6 +88,25 @@ class GpuContext:
         check(self.lib.cnpc_paint(self.h, _ptr(pos), _ptr(w), pos.shape[0], n, float(boxsize), WINDOW[sampler],
                                   int(bool(interlaced)), center, prec, _ptr(mesh0), _ptr(mesh1)), "cnpc_paint")
 
+    def paint_batch(self, pos, w, offsets, n, boxsize, sampler, interlaced, prec, mesh0, mesh1=None):
+        """Paint the catalogs [offsets[b], offsets[b+1]) of the concatenated
+        device arrays into meshes mesh0[b] (and mesh1[b]); ``offsets`` is a
+        host int sequence of nbatch + 1 entries."""
+        off = np.asarray(offsets, dtype=np.int64)
+        nb = len(off) - 1
+        if nb < 1 or off[0] != 0 or np.any(np.diff(off) < 0) or off[-1] != pos.shape[0]:
+            raise ValueError("offsets must start at 0, increase and end at the particle count")
+        if (pos.dtype != torch.float64 or w.dtype != torch.float64 or pos.dim() != 2 or pos.shape[1] != 3
+                or w.shape != (pos.shape[0],) or not (pos.is_contiguous() and w.is_contiguous() and pos.is_cuda)):
+            raise ValueError("positions must be a contiguous (Np, 3) and weights an (Np,) float64 device tensor")
+        self._check_mesh(mesh0, n, prec, nb)
+        if interlaced:
+            self._check_mesh(mesh1, n, prec, nb)
+        off_d = torch.from_numpy(off).to(self.device)
+        check(self.lib.cnpc_paint_batch(self.h, _ptr(pos), _ptr(w), _ptr(off_d), nb, int(np.diff(off).max()), n,
+                                        float(boxsize), WINDOW[sampler], int(bool(interlaced)), None, prec,
+                                        _ptr(mesh0), _ptr(mesh1 if interlaced else None)), "cnpc_paint_batch")
+
     def r2c(self, mesh, out, n, prec, batch=None):
         self._check_mesh(mesh, n, prec, batch)
         self._check_spec(out, n, prec, batch)

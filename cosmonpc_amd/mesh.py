@@ -86,7 +86,7 @@ def get_mesh_box(catalogs, correlation_mode, nmesh, geometry, boxsize, sampler, 
             pos, w = read_box_catalog(catalogs[key], column_names, apply_rsd=apply_rsd, para_cosmo=para_cosmo,
                                       boxsize=boxsize, redshift_box=redshift_box, los=los)
         attrs["N_gal_" + t] = len(w)
-        attrs["NZ_" + t] = float(np.sum(w)) / V
+        attrs["NZ_" + t] = np.sum(w) / V
         fields[t] = paint_field(ctx, pos, w, n, L, sampler, interlaced, prec)
         logging.info(f"Mesh for tracer_{t} generated.")
     ordered = {}

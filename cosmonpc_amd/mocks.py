@@ -1,0 +1,131 @@
+"""Batched box mocks: one measurement configuration over many catalogs.
+
+A mock campaign runs the same box-like auto-correlation tasks (pk and/or
+bk_sugi) on many catalogs, one ``run_stats`` call per catalog in the
+reference (``task_executor.py:330-343``).  ``run_mocks`` shares the GPU work
+of a chunk of catalogs instead:
+
+* all catalogs of the chunk are painted in one launch (``cnpc_paint_batch``,
+  one catalog per grid row, uneven sizes welcome);
+* one batched rocFFT r2c per mesh copy transforms the whole chunk;
+* each catalog's spectra then go through the same estimators and writers as
+  ``run_stats`` (``run_task`` with the painted field), so every output file
+  and result dict is the one ``run_stats`` would write for that catalog.
+
+Catalogs whose outputs all exist already are skipped, so an interrupted
+campaign resumes where it stopped.
+"""
+import logging
+import os
+import time
+
+import numpy as np
+import torch
+
+from .catalog import read_box_catalog
+from .device import PREC, get_context
+from .mesh import DeviceField
+from .param_helper import catalog_check, validate_config
+from .task_executor import check_task, result_path, run_task
+
+MESH_KEYS = ("nmesh", "boxsize", "sampler", "interlaced", "column_names", "precision", "apply_rsd", "cosmology",
+             "redshift_box", "rsd")
+
+LAST_TIMINGS = {}
+
+
+def _task_kwargs(cfg):
+    return {k: v for k, v in cfg.items() if k not in ("statistic", "correlation_mode", "geometry", "catalogs")}
+
+
+def run_mocks(configs, catalogs, batch=16, skip_existing=True, arrays=None):
+    """Run every config of ``configs`` (box-like, auto-correlation run_stats
+    configs that agree on the mesh keys) on every catalog path in
+    ``catalogs``.  ``arrays`` may map a path to (Position, WEIGHT) instead of
+    reading the file.  Returns {path: [result dict per config]} for the
+    catalogs that ran."""
+    configs = [dict(c) for c in configs]
+    catalogs = list(catalogs)
+    if not configs:
+        raise ValueError("run_mocks needs at least one config")
+    if batch < 1:
+        raise ValueError("batch must be at least 1")
+    for c in configs:
+        if c.get("geometry") != "box-like" or c.get("correlation_mode") != "auto":
+            raise ValueError("run_mocks supports box-like auto-correlation configs only")
+        c["catalogs"] = {"data_a": catalogs[0] if catalogs else "mock"}
+        validate_config(c)
+        catalog_check(c["catalogs"], c["geometry"], c["correlation_mode"], c["statistic"])
+        check_task(c["statistic"], c["correlation_mode"], c["geometry"], **_task_kwargs(c))
+    base = configs[0]
+    for c in configs[1:]:
+        for k in MESH_KEYS:
+            if c.get(k) != base.get(k):
+                raise ValueError(f"all configs of a mock run must share '{k}'")
+    todo = []
+    for path in catalogs:
+        outs = [result_path(c["statistic"], "auto", {"data_a": path}, **_task_kwargs(c)) for c in configs]
+        if skip_existing and all(os.path.exists(o) for o in outs):
+            logging.info(f"mock {path}: outputs exist, skipped")
+            continue
+        todo.append(path)
+    LAST_TIMINGS.clear()
+    LAST_TIMINGS.update({"paint_fft_s": 0.0, "estimators_s": 0.0, "mocks": len(todo)})
+    if not todo:
+        return {}
+    ctx = get_context()
+    n = int(base["nmesh"][0])
+    L = float(base["boxsize"][0])
+    V = float(np.prod(np.asarray(base["boxsize"], dtype=float)))
+    prec = PREC[base.get("precision", "fp64")]
+    sampler, interlaced = base["sampler"], bool(base["interlaced"])
+    results = {}
+    t_paint = t_est = 0.0
+    for c0 in range(0, len(todo), batch):
+        chunk = todo[c0:c0 + batch]
+        t0 = time.perf_counter()
+        pos_l, w_l = [], []
+        for path in chunk:
+            if arrays is not None and path in arrays:
+                pos, w = arrays[path]
+            else:
+                pos, w = read_box_catalog(path, base["column_names"], apply_rsd=base.get("apply_rsd", False),
+                                          para_cosmo=base.get("cosmology"), boxsize=base["boxsize"],
+                                          redshift_box=base.get("redshift_box"), los=base.get("rsd"))
+            pos_l.append(np.ascontiguousarray(pos, dtype=np.float64))
+            w_l.append(np.ascontiguousarray(w, dtype=np.float64))
+        off = np.concatenate([[0], np.cumsum([len(w) for w in w_l])]).astype(np.int64)
+        pos_d = torch.from_numpy(np.concatenate(pos_l)).to(ctx.device)
+        w_d = torch.from_numpy(np.concatenate(w_l)).to(ctx.device)
+        nb = len(chunk)
+        m0 = ctx.mesh(n, prec, nb)
+        m1 = ctx.mesh(n, prec, nb) if interlaced else None
+        ctx.paint_batch(pos_d, w_d, off, n, L, sampler, interlaced, prec, m0, m1)
+        del pos_d, w_d
+        s1 = ctx.spectrum(n, prec, nb)
+        ctx.r2c(m0, s1, n, prec, nb)
+        del m0
+        s2 = None
+        if interlaced:
+            s2 = ctx.spectrum(n, prec, nb)
+            ctx.r2c(m1, s2, n, prec, nb)
+            del m1
+        torch.cuda.synchronize(ctx.device)
+        t1 = time.perf_counter()
+        for b, path in enumerate(chunk):
+            field = DeviceField(s1[b], s2[b] if interlaced else None, n, L, prec, sampler, interlaced)
+            attrs = {"NZ_a": np.sum(w_l[b]) / V, "N_gal_a": len(w_l[b]), "interlaced": interlaced,
+                     "sampler": sampler}
+            res = []
+            for c in configs:
+                cfg = dict(c)
+                cfg["catalogs"] = {"data_a": path}
+                cfg["fields"] = (attrs, field)
+                res.append(run_task(**cfg))
+            results[path] = res
+        torch.cuda.synchronize(ctx.device)
+        t_paint += t1 - t0
+        t_est += time.perf_counter() - t1
+        del s1, s2
+    LAST_TIMINGS.update({"paint_fft_s": t_paint, "estimators_s": t_est})
+    return results

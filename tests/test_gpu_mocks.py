@@ -1,0 +1,79 @@
+"""Batched mock driver on the GPU: every output of run_mocks equals what
+run_stats writes for the same catalog alone (FP64, 1e-10 relative to max|.|
+per entry: the two paths differ only in the order of the atomic additions),
+and a second run resumes by skipping the finished mocks."""
+import os
+
+import numpy as np
+import pytest
+
+from tests.helpers import catalog, rel_to_max, weights
+
+pytestmark = pytest.mark.gpu
+
+
+def _cfgs(out):
+    base = dict(correlation_mode="auto", geometry="box-like", column_names=["x", "y", "z"], rsd=[0, 0, 1],
+                apply_rsd=False, nmesh=[32] * 3, boxsize=[500.0] * 3, sampler="cic", interlaced=True,
+                compensation=True, cosmology={"h": 0.6766, "Omega0": 0.30966}, redshift_box=0.5,
+                output_dir=str(out))
+    pk = dict(base, statistic="pk", k_min=0.01, k_max=0.3, k_bins=10, poles=[0, 2, 4])
+    bk = dict(base, statistic="bk_sugi", k_min=0.02, k_max=0.14, k_bins=5, angu_config=[2, 0, 2],
+              tracer_type="aaa", data_vector_mode="full", shotnoise_mode="both")
+    return pk, bk
+
+
+def _arrays(tmp_path):
+    arrays = {}
+    for i, npart in enumerate((3000, 700, 0, 5000, 1200)):
+        path = str(tmp_path / "mocks" / f"mock_{i}.npy")
+        pos = catalog(max(npart, 1), 500.0, 20 + i, ngrid=32)[:npart]
+        arrays[path] = (pos, weights(npart, seed=30 + i))
+    return arrays
+
+
+def test_run_mocks_matches_run_stats(gpu_ctx, tmp_path):
+    from cosmonpc_amd import run_stats
+    from cosmonpc_amd.mocks import LAST_TIMINGS, run_mocks
+
+    arrays = _arrays(tmp_path)
+    paths = [p for p in arrays if len(arrays[p][1]) > 0]
+    pk, bk = _cfgs(tmp_path / "batched")
+    print(f"run_mocks over {len(paths)} mocks, batch 2", flush=True)
+    got = run_mocks([pk, bk], paths, batch=2, arrays=arrays)
+    assert LAST_TIMINGS["mocks"] == len(paths) and set(got) == set(paths)
+    for path in paths:
+        print(f"run_stats {os.path.basename(path)}", flush=True)
+        for cfg, res in zip(_cfgs(tmp_path / "single"), got[path]):
+            cfg = dict(cfg, catalogs={"data_a": path}, arrays={"data_a": arrays[path]})
+            ref = run_stats(cfg)
+            assert sorted(ref) == sorted(res)
+            for key, val in ref.items():
+                if isinstance(val, np.ndarray) and val.dtype.kind in "fc" and val.size and np.any(val != 0):
+                    assert rel_to_max(res[key], val) < 1e-10, (path, cfg["statistic"], key)
+                elif isinstance(val, np.ndarray):
+                    np.testing.assert_array_equal(res[key], val)
+                elif isinstance(val, float):
+                    assert res[key] == pytest.approx(val, rel=1e-12), key
+                else:
+                    assert res[key] == val, key
+    names = sorted(os.listdir(tmp_path / "batched"))
+    assert names == sorted(os.listdir(tmp_path / "single"))
+    again = run_mocks([pk, bk], paths, batch=2, arrays=arrays)
+    assert again == {} and LAST_TIMINGS["mocks"] == 0
+
+
+def test_run_mocks_empty_catalog(gpu_ctx, tmp_path):
+    """A mock with no particles paints an empty mesh inside a batch; its
+    neighbours are unaffected (checked against a batch without it)."""
+    from cosmonpc_amd.mocks import run_mocks
+
+    arrays = _arrays(tmp_path)
+    paths = list(arrays)
+    pk, _ = _cfgs(tmp_path / "with_empty")
+    with np.errstate(divide="ignore", invalid="ignore"):
+        got = run_mocks([pk], paths, batch=len(paths), arrays=arrays)
+    pk2, _ = _cfgs(tmp_path / "without")
+    alone = run_mocks([pk2], [paths[3]], arrays=arrays)
+    assert rel_to_max(got[paths[3]][0]["P0"], alone[paths[3]][0]["P0"]) < 1e-10
+    assert got[paths[2]][0]["N_gal_a"] == 0

@@ -2,6 +2,7 @@
 messages of the reference's param_helper.py / task_executor.py), config
 normalisation, and failures raised before any GPU work."""
 import copy
+import os
 
 import numpy as np
 import pytest
@@ -192,3 +193,31 @@ def test_fits_format_cards_and_foreign_file(tmp_path):
     assert t.dtype.names == ("WEIGHT_FKP", "Z")
     np.testing.assert_array_equal(t["Z"], [0.5, 0.6, 0.7])
     np.testing.assert_array_equal(t["WEIGHT_FKP"], np.array([1.0, 0.25, 2.0], dtype=np.float32))
+
+
+def test_run_mocks_validation_and_resume(tmp_path):
+    """The batched mock driver accepts only box-like auto configs that share
+    one mesh, and skips catalogs whose outputs all exist (no GPU needed)."""
+    from cosmonpc_amd.mocks import LAST_TIMINGS, run_mocks
+
+    out = tmp_path / "out"
+    pk = _base(output_dir=str(out))
+    bk = _base(output_dir=str(out), statistic="bk_sugi", tracer_type="aaa", angu_config=[0, 0, 0],
+               data_vector_mode="diagonal", k_max=0.1)
+    with pytest.raises(ValueError, match="box-like auto-correlation configs only"):
+        run_mocks([_base(correlation_mode="cross", catalogs={"data_a": "a", "data_b": "b"})], ["m.npy"])
+    with pytest.raises(ValueError, match="must share 'nmesh'"):
+        run_mocks([pk, dict(bk, nmesh=[64] * 3)], ["m.npy"])
+    with pytest.raises(ValueError, match="at least one config"):
+        run_mocks([], ["m.npy"])
+    with pytest.raises(ValueError, match="batch must be"):
+        run_mocks([pk], ["m.npy"], batch=0)
+    with pytest.raises(ValueError, match="data_vector_mode must be"):
+        run_mocks([dict(bk, data_vector_mode="band")], ["m.npy"])
+    out.mkdir()
+    mocks = [str(tmp_path / "mocks" / f"mock_{i}.npy") for i in range(3)]
+    for m in mocks:
+        np.save(out / f"pk_res_auto_{os.path.splitext(os.path.basename(m))[0]}.npy", {})
+        np.save(out / f"bk_sugi_res_diagonal_000_auto_aaa_mocks_{os.path.splitext(os.path.basename(m))[0]}.npy", {})
+    assert run_mocks([pk, bk], mocks) == {}
+    assert LAST_TIMINGS["mocks"] == 0
