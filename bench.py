@@ -7,10 +7,12 @@ angu_config [0,0,0] and [2,0,2], tracer aaa, 5e6 lognormal + RSD particles
 
 One step = the signal stage of both angular configurations: from the
 compensated delta_k resident in HBM (FP32) through G_00, Ylm-weighted shell
-filtering, the batched shell inverse FFTs and the split-bf16 MFMA
-contraction, to the two 30 x 30 signal matrices.  bins/step = 2 x 465.
+filtering, the pruned shell synthesis (x and y passes) and the split-bf16
+MFMA contraction with the z pass fused in, to the two 30 x 30 signal
+matrices.  bins/step = 2 x 465.
 
-Also reported (outside the timed steps): the plain FP64 pairwise path and the
+Also reported (outside the timed steps): the same stage with the real shells
+written to HBM and contracted apart, the plain FP64 pairwise path and the
 FP64 MFMA GEMM path on the same catalog, the split-bf16 error relative to
 max|B| against the FP64 result, per-kernel roofline timings, and the CPU
 baseline (the FP64 numpy oracle on a bounded sample, extrapolated).
@@ -43,6 +45,12 @@ def parse():
     p.add_argument("--no-fp64", action="store_true", help="skip the FP64 reference paths")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     p.add_argument("--roofline-only", action="store_true")
+    p.add_argument("--workload", choices=("metric", "mocks", "kernels"), default="metric",
+                   help="mocks: 64 box mocks (pk + bk_sugi diagonal at 256^3), batched driver vs per-mock loop; "
+                        "kernels: one launch of each hot kernel at 512^3 plus a 1 GiB device copy (for counters)")
+    p.add_argument("--mocks", type=int, default=64)
+    p.add_argument("--mock-batch", type=int, default=16)
+    p.add_argument("--loop-mocks", type=int, default=16, help="mocks timed in the per-mock run_stats loop")
     return p.parse_args()
 
 
@@ -52,6 +60,10 @@ def main():
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the path is not sharded across GPUs.",
               file=sys.stderr)
         sys.exit(2)
+    if args.workload == "mocks":
+        return bench_mocks(args)
+    if args.workload == "kernels":
+        return bench_kernels(args)
     import numpy as np
     import torch
 
@@ -94,13 +106,14 @@ def main():
     del f32
     log(f"fp32 paint+r2c+prepare {t_paint32 * 1e3:.1f} ms")
 
-    def signal_step(delta, prec, mode):
+    def signal_step(delta, prec, mode, zsynth="fused"):
         """G_00 + both configurations' signal matrices, on the contraction
         grid the estimator uses (reduced for the GEMM modes)."""
         m, nmax = contraction_grid(n, L, kmax, mode)
         dm = ctx.resample(delta, n, m, prec, nmax) if m != n else delta
         G = bk_box_g00(ctx, dm, m, prec)
-        out = [(n / m) ** 3 * bk_box_signal_auto(ctx, dm, G, m, L, prec, k_edge, a, mode) for a in angus]
+        out = [(n / m) ** 3 * bk_box_signal_auto(ctx, dm, G, m, L, prec, k_edge, a, mode, zsynth=zsynth)
+               for a in angus]
         del G, dm
         return out
 
@@ -122,6 +135,15 @@ def main():
     value = bins_per_step / t_step
     log(f"bf16x3 signal stage {t_step * 1e3:.1f} ms/step -> {value:.1f} bins/s")
 
+    # the same stage with the real shells written to HBM and contracted apart
+    signal_step(d32, 1, "bf16x3", "separate")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res_sep = signal_step(d32, 1, "bf16x3", "separate")
+    torch.cuda.synchronize()
+    t_sep = time.perf_counter() - t0
+    log(f"bf16x3 separate z pass: {t_sep * 1e3:.1f} ms/step -> {bins_per_step / t_sep:.1f} bins/s")
+
     # ---------------------------------------------------------- per-kernel roofline
     m, nmax = contraction_grid(n, L, kmax, "bf16x3")
     dm32 = ctx.resample(d32, n, m, 1, nmax) if m != n else d32
@@ -129,7 +151,7 @@ def main():
     del dm32
     log(f"kernels (grid {m}^3): {json.dumps(roof['stages'])}")
 
-    extra = {}
+    extra = {"separate_z_ms_per_step": t_sep * 1e3, "separate_z_bins_per_s": bins_per_step / t_sep}
     if not args.no_fp64:
         paint_field(ctx, pos_d, w_d, n, L, sampler, True, 0)
         torch.cuda.synchronize()
@@ -152,9 +174,10 @@ def main():
             log(f"fp64 {mode}: {dt * 1e3:.1f} ms/step -> {bins_per_step / dt:.1f} bins/s")
         ref = res["pairwise"]
         extra["speedup_vs_fp64_plain"] = extra["pairwise_ms_per_step"] / (t_step * 1e3)
-        for a, r64, rb, rg in zip(angus, ref, res_bf16, res["gemm_fp64"]):
+        for a, r64, rb, rs, rg in zip(angus, ref, res_bf16, res_sep, res["gemm_fp64"]):
             tag = "".join(map(str, a))
             extra[f"err_bf16x3_B{tag}"] = float(np.max(np.abs(rb - r64)) / np.max(np.abs(r64)))
+            extra[f"err_bf16x3_separate_B{tag}"] = float(np.max(np.abs(rs - r64)) / np.max(np.abs(r64)))
             extra[f"err_gemm_fp64_B{tag}"] = float(np.max(np.abs(rg - r64)) / np.max(np.abs(r64)))
         del d64
     cpu = None if args.no_cpu else cpu_baseline(n, L, k_edge, npairs)
@@ -196,6 +219,9 @@ def kernel_roofline(ctx, delta, n, L, k_edge, torch, np, A):
     shells = ctx.mesh(n, 1, K).view(K, N3)
     G = bk_box_g00(ctx, delta, n, 1)
     poly = A.shell_components(0, 0)[0][0]
+    kmax = float(k_edge[-1])
+    R = ctx.synth_radius(L, kmax)
+    H = ctx.half_shells(n, R, K)
 
     def timed(fn, reps=1):
         e0, e1 = ev(), ev()
@@ -206,7 +232,15 @@ def kernel_roofline(ctx, delta, n, L, k_edge, torch, np, A):
         e1.synchronize()
         return e0.elapsed_time(e1) / reps
 
-    # production path: pruned shell synthesis (cube, x, y and z FFT kernels)
+    # production path: x and y synthesis passes into the half-synthesised
+    # layout, then the z pass fused into the split-bf16 GEMM
+    t_half = []
+    for b0 in range(0, K, batch):
+        c = min(batch, K - b0)
+        t_half.append((timed(lambda: ctx.shell_synth_half(delta, n, L, k_edge, b0, c, poly, work, H, b0)), c))
+    ctx.contract_zgemm(n, L, kmax, G, H, K)
+    t_zgemm = timed(lambda: ctx.contract_zgemm(n, L, kmax, G, H, K), 3)
+    # the separate path: real shells (cube, x, y and z FFT kernels) + GEMM
     t_synth = []
     for b0 in range(0, K, batch):
         c = min(batch, K - b0)
@@ -229,10 +263,15 @@ def kernel_roofline(ctx, delta, n, L, k_edge, torch, np, A):
     # spectra read + meshes written (c2r); delta read + spectra written
     # (filter); every shell and G read once (GEMM)
     b_synth = batch * N3 * 4
+    b_half = batch * n * n * (R + 1) * 8
+    b_zgemm = K * n * n * (R + 1) * 8 + N3 * 4
     b_c2r = batch * (half * 8 + N3 * 4)
     b_filter = half * 8 + batch * half * 8
     b_gemm = (K + 1) * N3 * 4
     stages = {
+        "shell_synth_half_batch4": {"ms_per_launch": per_launch(t_half), "launches": len(t_half),
+                                    "total_ms": sum(x[0] for x in t_half), "bytes": b_half},
+        "zgemm_bf16x3": {"ms_per_launch": t_zgemm, "launches": 1, "total_ms": t_zgemm, "bytes": b_zgemm},
         "shell_synth_batch4": {"ms_per_launch": per_launch(t_synth), "launches": len(t_synth),
                                "total_ms": sum(x[0] for x in t_synth), "bytes": b_synth},
         "gemm_x3_bf16": {"ms_per_launch": t_gemm, "launches": 1, "total_ms": t_gemm, "bytes": b_gemm},
@@ -243,11 +282,11 @@ def kernel_roofline(ctx, delta, n, L, k_edge, torch, np, A):
     }
     for v in stages.values():
         v["GBps"] = v["bytes"] / v["ms_per_launch"] / 1e6
-    dom = max(("shell_synth_batch4", "gemm_x3_bf16"), key=lambda k: stages[k]["total_ms"])
+    dom = max(("shell_synth_half_batch4", "zgemm_bf16x3"), key=lambda k: stages[k]["total_ms"])
     ach = stages[dom]["GBps"]
     roof = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
             "traffic": None, "kernel": dom, "algorithmic_bytes_per_launch": stages[dom]["bytes"]}
-    del work, shells, G
+    del work, shells, G, H
     return {"roofline": roof, "stages": stages}
 
 
@@ -284,6 +323,185 @@ def cpu_baseline(n, L, k_edge, npairs):
             "sample": f"one Ylm-weighted shell IFFT ({t_shell:.1f}s) and one triple sum ({t_pair:.2f}s) of the "
                       f"{n}^3 complex128 numpy oracle (single thread), extrapolated to {K + 1} IFFTs + {npairs} "
                       f"pair sums per angular configuration; host {platform.processor() or platform.machine()}"}
+
+
+def bench_kernels(args):
+    """One launch of each hot kernel of the 512^3 workload, timed with HIP
+    events on the context stream, and a plain 1 GiB device-to-device copy of
+    known size (the reference point for hardware byte counters).  Prints one
+    JSON line of per-kernel times and algorithmic bytes."""
+    import numpy as np
+    import torch
+
+    from cosmonpc_amd import angular as A
+    from cosmonpc_amd import synth
+    from cosmonpc_amd.device import get_context
+    from cosmonpc_amd.estimators import bk_box_g00, contraction_grid
+
+    n, L, kmax, K = args.nmesh, 2000.0, 0.31, args.k_bins
+    k_edge = np.linspace(0.01, kmax, K + 1)
+    pos = synth.lognormal_box(args.nparticles, boxsize=L, ngrid=256, seed=2024)
+    ctx = get_context(0)
+    dev = ctx.device
+    pos_d = torch.from_numpy(pos).to(dev)
+    w_d = torch.ones(len(pos), dtype=torch.float64, device=dev)
+    half = n * n * (n // 2 + 1)
+
+    def timed(fn):
+        fn()  # first call: plans and work areas
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        fn()
+        e1.record()
+        e1.synchronize()
+        return e0.elapsed_time(e1)
+
+    out = {}
+    m0, m1 = ctx.mesh(n, 1), ctx.mesh(n, 1)
+    out["paint_pcs_interlaced"] = (timed(lambda: ctx.paint(pos_d, w_d, n, L, "pcs", True, 1, m0, m1)),
+                                   len(pos) * 32 + 2 * n ** 3 * 4)
+    s1, s2 = ctx.spectrum(n, 1), ctx.spectrum(n, 1)
+    out["rocfft_r2c_512"] = (timed(lambda: ctx.r2c(m0, s1, n, 1)), n ** 3 * 4 + half * 8)
+    ctx.r2c(m1, s2, n, 1)
+    d = ctx.spectrum(n, 1)
+    out["prepare"] = (timed(lambda: ctx.prepare(s1, s2, d, n, L, 1, True, 1, "pcs", 1.0 / n ** 3)), 3 * half * 8)
+    polys = [A.make_poly(0, [1.0], A.PART_FULL, 1.0, 1.0)] + [A.legendre_poly(p, np.array([0.0, 0.0, 1.0]))
+                                                              for p in (2, 4)]
+    out["bin_pk_3poles"] = (timed(lambda: ctx.bin(d, d, n, L, 1, np.linspace(0.005, 0.305, 31), polys)), half * 8)
+    m, nmax = contraction_grid(n, L, kmax, "bf16x3")
+    dm = ctx.resample(d, n, m, 1, nmax)
+    G = bk_box_g00(ctx, dm, m, 1)
+    work = ctx.spectrum(m, 1, 4)
+    shells = ctx.mesh(m, 1, K).view(K, m ** 3)
+    poly = A.shell_components(0, 0)[0][0]
+    out["shell_synth_batch4"] = (timed(lambda: ctx.shell_synth(dm, m, L, 1, k_edge, 8, 4, poly, work,
+                                                               shells[8:12].view(4, m, m, m))), 4 * m ** 3 * 4)
+    for b0 in range(0, K, 4):
+        c = min(4, K - b0)
+        ctx.shell_synth(dm, m, L, 1, k_edge, b0, c, poly, work, shells[b0:b0 + c].view(c, m, m, m))
+    out["gemm_x3_bf16"] = (timed(lambda: ctx.contract_gemm("bf16x3", G, shells, None)), (K + 1) * m ** 3 * 4)
+    del shells
+    R = ctx.synth_radius(L, kmax)
+    H = ctx.half_shells(m, R, K)
+    out["shell_synth_half_batch4"] = (timed(lambda: ctx.shell_synth_half(dm, m, L, k_edge, 8, 4, poly, work, H, 8)),
+                                      4 * m * m * (R + 1) * 8)
+    for b0 in range(0, K, 4):
+        ctx.shell_synth_half(dm, m, L, k_edge, b0, min(4, K - b0), poly, work, H, b0)
+    out["zgemm_bf16x3"] = (timed(lambda: ctx.contract_zgemm(m, L, kmax, G, H, K)),
+                           K * m * m * (R + 1) * 8 + m ** 3 * 4)
+    del H
+    src = torch.empty(1 << 28, dtype=torch.float32, device=dev)
+    dst = torch.empty_like(src)
+    src.fill_(1.0)
+    out["copy_1GiB"] = (timed(lambda: dst.copy_(src)), 2 * (1 << 30))
+    torch.cuda.synchronize()
+    line = {k: {"ms": t, "bytes": b, "GBps": b / t / 1e6} for k, (t, b) in out.items()}
+    for k, v in line.items():
+        log(f"{k}: {v['ms']:.3f} ms, {v['bytes'] / 1e9:.3f} GB algorithmic -> {v['GBps']:.0f} GB/s")
+    print(json.dumps({"workload": f"kernels of the bk_sugi {n}^3 pcs workload (contraction grid {m}^3)",
+                      "kernels": line}), flush=True)
+
+
+def _mock_catalog(i):
+    """Mock i: its own seed, N ~ U[5e5, 2e6], lognormal + RSD on a 256^3
+    grid in a 1000 Mpc/h box."""
+    import numpy as np
+
+    from cosmonpc_amd import synth
+
+    npart = int(np.random.default_rng(1000 + i).integers(500_000, 2_000_001))
+    return synth.lognormal_box(npart, boxsize=1000.0, ngrid=256, seed=5000 + i)
+
+
+def bench_mocks(args):
+    """Batched mocks: ``args.mocks`` box catalogs, pk poles [0,2,4] and
+    bk_sugi [0,0,0] diagonal (19 bins 0.01-0.20) at 256^3 CIC interlaced,
+    FP64.  One step = every mock through both tasks with the batched driver
+    (``run_mocks``: one paint launch and one batched r2c per chunk); the
+    reference's way, one ``run_stats`` per mock and task, is timed beside it
+    on ``args.loop_mocks`` mocks.  Outputs go to a temporary directory."""
+    import shutil
+    import tempfile
+    from concurrent.futures import ProcessPoolExecutor
+
+    import numpy as np
+
+    t = time.time()
+    # catalogs are drawn in worker processes before this process touches the GPU
+    with ProcessPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
+        cats = list(ex.map(_mock_catalog, range(args.mocks)))
+    log(f"{args.mocks} mock catalogs ({sum(len(c) for c in cats)} particles) in {time.time() - t:.1f}s")
+    import torch
+
+    from cosmonpc_amd import run_stats
+    from cosmonpc_amd.mocks import LAST_TIMINGS, run_mocks
+
+    tmp = tempfile.mkdtemp(prefix="cnpc_mocks_")
+    paths = [os.path.join(tmp, "mocks", f"mock_{i:03d}.npy") for i in range(args.mocks)]
+    arrays = {p: (c, np.ones(len(c))) for p, c in zip(paths, cats)}
+    base = dict(correlation_mode="auto", geometry="box-like", column_names=["x", "y", "z"], rsd=[0, 0, 1],
+                apply_rsd=False, nmesh=[256] * 3, boxsize=[1000.0] * 3, sampler="cic", interlaced=True,
+                compensation=True, cosmology={"h": 0.6766, "Omega0": 0.30966}, redshift_box=0.5)
+    pk = dict(base, statistic="pk", k_min=0.005, k_max=0.305, k_bins=30, poles=[0, 2, 4])
+    bk = dict(base, statistic="bk_sugi", k_min=0.01, k_max=0.20, k_bins=19, angu_config=[0, 0, 0],
+              tracer_type="aaa", data_vector_mode="diagonal", shotnoise_mode="ana")
+
+    def batched(out, sel):
+        cfgs = [dict(c, output_dir=out) for c in (pk, bk)]
+        return run_mocks(cfgs, sel, batch=args.mock_batch, arrays=arrays)
+
+    def loop(out, sel):
+        for p in sel:
+            for c in (pk, bk):
+                run_stats(dict(c, output_dir=out, catalogs={"data_a": p}, arrays={"data_a": arrays[p]}))
+
+    import logging
+
+    logging.disable(logging.INFO)
+    try:
+        batched(os.path.join(tmp, "warm_b"), paths[:2])  # FFT plans, work areas
+        loop(os.path.join(tmp, "warm_l"), paths[:1])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        batched(os.path.join(tmp, "batched"), paths)
+        torch.cuda.synchronize()
+        t_b = time.perf_counter() - t0
+        timings = dict(LAST_TIMINGS)
+        nloop = min(args.loop_mocks, args.mocks)
+        t0 = time.perf_counter()
+        loop(os.path.join(tmp, "loop"), paths[:nloop])
+        torch.cuda.synchronize()
+        t_l = time.perf_counter() - t0
+        worst = 0.0
+        for p in paths[:nloop]:
+            for kind in ("pk_res_auto_{}.npy", "bk_sugi_res_diagonal_000_auto_aaa_mocks_{}.npy"):
+                name = kind.format(os.path.splitext(os.path.basename(p))[0])
+                a = np.load(os.path.join(tmp, "batched", name), allow_pickle=True).item()
+                b = np.load(os.path.join(tmp, "loop", name), allow_pickle=True).item()
+                for key in ("P0", "P2", "P4", "B_sugi"):
+                    if key in b:
+                        ref = np.asarray(b[key])
+                        worst = max(worst, float(np.max(np.abs(np.asarray(a[key]) - ref)) / np.max(np.abs(ref))))
+    finally:
+        logging.disable(logging.NOTSET)
+        shutil.rmtree(tmp, ignore_errors=True)
+    value = args.mocks / t_b
+    loop_rate = nloop / t_l
+    log(f"batched: {t_b:.2f}s for {args.mocks} mocks -> {value:.2f} mocks/s ({timings}); "
+        f"per-mock loop: {t_l:.2f}s for {nloop} -> {loop_rate:.2f} mocks/s; max rel diff {worst:.2e}")
+    line = {
+        "metric": "batched box mocks/s (pk [0,2,4] + bk_sugi [0,0,0] diagonal per mock)",
+        "value": value, "unit": "mocks/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": t_b * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic (seeded lognormal + linear RSD catalogs, one seed per mock)",
+        "config": {"workload": f"{args.mocks} box mocks 256^3 cic interlaced, box 1000, N ~ U[5e5, 2e6], "
+                               "pk 30 bins + bk_sugi [0,0,0] diagonal 19 bins", "mock_batch": args.mock_batch},
+        "per_mock_loop": {"value": loop_rate, "unit": "mocks/s", "mocks": nloop},
+        "speedup_vs_loop": value / loop_rate, "max_rel_diff_vs_loop": worst,
+        "stages_s": timings,
+    }
+    print(json.dumps(line), flush=True)
 
 
 if __name__ == "__main__":

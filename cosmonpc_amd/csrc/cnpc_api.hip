@@ -56,6 +56,8 @@ struct cnpc_ctx {
   size_t scratch_bytes = 0;
   void* synth = nullptr;    // shell-synthesis cube, bins and twiddles
   size_t synth_bytes = 0;
+  void* pwork = nullptr;    // tiled paint: keys, tile counts, sorted particles
+  size_t pwork_bytes = 0;
 };
 
 #define CHECK_HIP(expr)                                                                        \
@@ -98,6 +100,19 @@ static int ensure_synth(cnpc_ctx* ctx, size_t bytes) {
   }
   CHECK_HIP(hipMalloc(&ctx->synth, bytes));
   ctx->synth_bytes = bytes;
+  return 0;
+}
+
+static int ensure_pwork(cnpc_ctx* ctx, size_t bytes) {
+  if (ctx->pwork_bytes >= bytes) return 0;
+  if (ctx->pwork) {
+    CHECK_HIP(hipStreamSynchronize(ctx->stream));
+    CHECK_HIP(hipFree(ctx->pwork));
+    ctx->pwork = nullptr;
+    ctx->pwork_bytes = 0;
+  }
+  CHECK_HIP(hipMalloc(&ctx->pwork, bytes));
+  ctx->pwork_bytes = bytes;
   return 0;
 }
 
@@ -184,6 +199,7 @@ int cnpc_ctx_destroy(cnpc_ctx* ctx) {
   if (!ctx) return 0;
   int rc = cnpc_ctx_clear_plans(ctx);
   if (ctx->scratch) (void)hipFree(ctx->scratch);
+  if (ctx->pwork) (void)hipFree(ctx->pwork);
   if (ctx->synth) (void)hipFree(ctx->synth);
   delete ctx;
   return rc;
@@ -227,6 +243,12 @@ int cnpc_paint(cnpc_ctx* ctx, const double* pos, const double* w, int64_t np, in
   REQUIRE(boxsize > 0, "cnpc_paint: boxsize must be positive");
   REQUIRE(np >= 0 && (np == 0 || (pos && w)), "cnpc_paint: null particle arrays");
   REQUIRE(mesh0 && (!interlaced || mesh1), "cnpc_paint: null mesh");
+  if (paint_tiled_ok(n) && np < ((int64_t)1 << 31)) {  // sorted, LDS-tiled, writes every cell
+    if (int rc = ensure_pwork(ctx, paint_tiled_work_bytes(np, n))) return rc;
+    CHECK_HIP(launch_paint_tiled(pos, w, np, n, boxsize, window, center, interlaced ? 2 : 1, prec, mesh0, mesh1,
+                                 ctx->pwork, ctx->stream));
+    return 0;
+  }
   const int64_t bytes = (int64_t)n * n * n * (prec == 0 ? 8 : 4);
   if (int rc = cnpc_zero(ctx, mesh0, bytes)) return rc;
   if (interlaced)
@@ -334,6 +356,22 @@ int cnpc_cross_spectrum(cnpc_ctx* ctx, int n, int prec, const void* A, const voi
   return 0;
 }
 
+// mixed-radix factorisation of n, odd radices first (their Stockham writes
+// stride by an odd count and stay off the LDS bank conflicts); 0 if n has a
+// prime factor above 5
+static int factor_radices(int n, SynthArgs& a) {
+  int m = n, k = 0;
+  for (int r : {5, 3})
+    while (m % r == 0 && k < CNPC_MAX_RADICES) { a.rad[k++] = r; m /= r; }
+  while (m % 4 == 0 && k < CNPC_MAX_RADICES) { a.rad[k++] = 4; m /= 4; }
+  while (m % 2 == 0 && k < CNPC_MAX_RADICES) { a.rad[k++] = 2; m /= 2; }
+  a.nrad = k;
+  return m == 1;
+}
+
+// every mode with |k| < kmax has |n_d| <= R
+static int synth_radius(double kmax, double boxsize) { return (int)std::ceil(kmax / (2.0 * M_PI / boxsize)); }
+
 int cnpc_shell_synth(cnpc_ctx* ctx, int n, double boxsize, int prec, const void* D, const double* edges, int nbins,
                      int bin0, int nshell, const cnpc_poly* weight, void* work, void* out) {
   REQUIRE(ctx, "null context");
@@ -346,22 +384,12 @@ int cnpc_shell_synth(cnpc_ctx* ctx, int n, double boxsize, int prec, const void*
           "shell_synth: bad weight polynomial");
   for (int i = 0; i < nbins; ++i) REQUIRE(edges[i + 1] > edges[i] && edges[i] >= 0, "shell_synth: bad edges");
   const double kf = 2.0 * M_PI / boxsize;
-  // every mode with |k| < edges[nbins] has |n_d| <= R
-  const int R = (int)std::ceil(edges[nbins] / kf);
+  const int R = synth_radius(edges[nbins], boxsize);
   REQUIRE(R < n / 2, "shell_synth: k_max reaches the Nyquist index (needs edges[nbins] < pi n / L)");
   SynthArgs a;
   std::memset(&a, 0, sizeof(a));
   a.n = n; a.R = R; a.bin0 = bin0; a.nshell = nshell; a.kf = kf;
-  {  // mixed-radix factorisation of n, odd radices first (their Stockham
-     // writes stride by an odd count and stay off the LDS bank conflicts)
-    int m = n, k = 0;
-    for (int r : {5, 3})
-      while (m % r == 0 && k < CNPC_MAX_RADICES) { a.rad[k++] = r; m /= r; }
-    while (m % 4 == 0 && k < CNPC_MAX_RADICES) { a.rad[k++] = 4; m /= 4; }
-    while (m % 2 == 0 && k < CNPC_MAX_RADICES) { a.rad[k++] = 2; m /= 2; }
-    REQUIRE(m == 1, "shell_synth: nmesh must factor into 2, 3 and 5");
-    a.nrad = k;
-  }
+  REQUIRE(factor_radices(n, a), "shell_synth: nmesh must factor into 2, 3 and 5");
   REQUIRE(synth_lines_per_block(prec, a) >= 1, "shell_synth: mesh too large for the LDS line buffers");
   const size_t cs = prec == 0 ? 16 : 8;
   const size_t P = 2 * (size_t)R + 1, Q = (size_t)R + 1, cube = P * Q * P;
@@ -378,6 +406,92 @@ int cnpc_shell_synth(cnpc_ctx* ctx, int n, double boxsize, int prec, const void*
   char* base = (char*)ctx->synth;
   CHECK_HIP(launch_synth(prec, a, E, W, D, base, (signed char*)(base + vbytes), base + vbytes + bbytes, work, out,
                          ctx->stream));
+  return 0;
+}
+
+int cnpc_shell_synth_half(cnpc_ctx* ctx, int n, double boxsize, const void* D, const double* edges, int nbins,
+                          int bin0, int nshell, const cnpc_poly* weight, void* work, void* out, int nslots, int slot0) {
+  REQUIRE(ctx, "null context");
+  REQUIRE(n >= 8 && n % 8 == 0 && boxsize > 0, "shell_synth_half: nmesh must be a multiple of 8");
+  REQUIRE(nbins >= 1 && nbins <= 127 && edges, "shell_synth_half: nbins must be in [1, 127]");
+  REQUIRE(bin0 >= 0 && nshell >= 1 && bin0 + nshell <= nbins, "shell_synth_half: shell range outside the bins");
+  REQUIRE(slot0 >= 0 && nslots >= 1 && slot0 + nshell <= nslots, "shell_synth_half: slots outside the layout");
+  REQUIRE(work && out && weight && work != out, "shell_synth_half: null or aliased buffers");
+  REQUIRE(weight->deg >= 0 && weight->deg < CNPC_ZDEG && weight->m >= 0 && weight->part >= 0 && weight->part <= 3,
+          "shell_synth_half: bad weight polynomial");
+  for (int i = 0; i < nbins; ++i) REQUIRE(edges[i + 1] > edges[i] && edges[i] >= 0, "shell_synth_half: bad edges");
+  const int R = synth_radius(edges[nbins], boxsize);
+  REQUIRE(R < n / 2, "shell_synth_half: k_max reaches the Nyquist index (needs edges[nbins] < pi n / L)");
+  SynthArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.n = n; a.R = R; a.bin0 = bin0; a.nshell = nshell; a.kf = 2.0 * M_PI / boxsize;
+  REQUIRE(factor_radices(n, a), "shell_synth_half: nmesh must factor into 2, 3 and 5");
+  REQUIRE(synth_lines_per_block(1, a) >= 1, "shell_synth_half: mesh too large for the LDS line buffers");
+  const size_t P = 2 * (size_t)R + 1, Q = (size_t)R + 1, cube = P * Q * P;
+  // work holds U (nshell n P Q complex), at most nshell half spectra
+  REQUIRE(P * Q <= (size_t)n * (n / 2 + 1), "shell_synth_half: k_max too large for the work area");
+  const size_t vbytes = (cube * 8 + 255) / 256 * 256, bbytes = (cube + 255) / 256 * 256;
+  if (int rc = ensure_synth(ctx, vbytes + bbytes + (size_t)n * 8)) return rc;
+  BinEdges E;
+  fill_edges(E, edges, nbins);
+  PolySpec W;
+  to_spec(W, *weight);
+  char* base = (char*)ctx->synth;
+  CHECK_HIP(launch_synth_half(a, E, W, D, base, (signed char*)(base + vbytes), base + vbytes + bbytes, work, out,
+                              nslots, slot0, ctx->stream));
+  return 0;
+}
+
+int cnpc_contract_zgemm(cnpc_ctx* ctx, int n, double boxsize, double kmax, const float* G, const void* HA, int na,
+                        int sa_slots, const void* HB, int nb, int sb_slots, int nblk, double* C) {
+  REQUIRE(ctx, "null context");
+  REQUIRE(n >= 8 && n % 8 == 0 && boxsize > 0 && kmax > 0, "contract_zgemm: nmesh must be a multiple of 8");
+  REQUIRE(G && HA && C, "contract_zgemm: null buffers");
+  REQUIRE(na >= 1 && na <= 32 && sa_slots >= na, "contract_zgemm: na must be in [1, 32] and fit the slots of HA");
+  const int same = HB == nullptr || HB == HA;
+  if (same) {
+    REQUIRE(HB == nullptr || (nb == na && sb_slots == sa_slots), "contract_zgemm: B == A needs the same rows");
+    nb = na;
+    sb_slots = sa_slots;
+  } else {
+    REQUIRE(nb >= 1 && nb <= 32 && sb_slots >= nb, "contract_zgemm: nb must be in [1, 32] and fit the slots of HB");
+  }
+  REQUIRE((((uintptr_t)G | (uintptr_t)HA | (uintptr_t)(same ? HA : HB)) & 15) == 0,
+          "contract_zgemm: buffers must be 16-byte aligned");
+  const int R = synth_radius(kmax, boxsize);
+  REQUIRE(R < n / 2, "contract_zgemm: k_max reaches the Nyquist index");
+  SynthArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.n = n; a.R = R;
+  REQUIRE(factor_radices(n, a), "contract_zgemm: nmesh must factor into 2, 3 and 5");
+  int dev = 0, lds_max = 0;
+  CHECK_HIP(hipGetDevice(&dev));
+  CHECK_HIP(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+  REQUIRE(zgemm_supported(a, na, nb, same, (size_t)lds_max),
+          "contract_zgemm: grid too large for the LDS line buffers");
+  if (nblk <= 0) nblk = 256;
+  if (nblk > 4096) nblk = 4096;
+  const size_t pbytes = (size_t)nblk * 1024 * 8;
+  if (int rc = ensure_scratch(ctx, pbytes + (size_t)n * 8)) return rc;
+  char* base = (char*)ctx->scratch;
+  CHECK_HIP(launch_twiddles(n, base + pbytes, ctx->stream));
+  CHECK_HIP(launch_zgemm(a, R + 1, HA, na, sa_slots, same ? HA : HB, nb, sb_slots, same, G, base + pbytes, nblk,
+                         (double*)base, C, ctx->stream));
+  return 0;
+}
+
+int cnpc_contract_zgemm_supported(cnpc_ctx* ctx, int n, int na, int nb, int same, int* ok) {
+  REQUIRE(ctx && ok, "contract_zgemm_supported: null arguments");
+  *ok = 0;
+  if (n < 8 || n % 8 != 0) return 0;
+  SynthArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.n = n;
+  if (!factor_radices(n, a)) return 0;
+  int dev = 0, lds_max = 0;
+  CHECK_HIP(hipGetDevice(&dev));
+  CHECK_HIP(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+  *ok = zgemm_supported(a, na, same ? na : nb, same ? 1 : 0, (size_t)lds_max);
   return 0;
 }
 

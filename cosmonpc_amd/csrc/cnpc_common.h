@@ -274,9 +274,25 @@ hipError_t launch_radecz(int64_t np, const double* ra, const double* dec, const 
                          double* pos, hipStream_t st);
 
 int synth_lines_per_block(int prec, const SynthArgs& a);
+hipError_t launch_synth_half(const SynthArgs& a, const BinEdges& E, const PolySpec& W, const void* D, void* V,
+                             signed char* bins, void* tw, void* work, void* out, int nslots, int slot0,
+                             hipStream_t st);
+hipError_t launch_twiddles(int n, void* tw, hipStream_t st);
+size_t zgemm_lds_bytes(int n, int nlines);
+int zgemm_supported(const SynthArgs& a, int na, int nb, int same, size_t lds_limit);
+hipError_t launch_zgemm(const SynthArgs& a, int Q, const void* HA, int na, int sa_slots, const void* HB, int nb,
+                        int sb_slots, int same, const float* G, const void* tw, int nblk, double* partial, double* C,
+                        hipStream_t st);
+hipError_t launch_gemm_finalize(const double* partial, int nblk, int a0, int b0, int na, int nb, double inv_scale,
+                                double* C, int ldc, hipStream_t st);
 hipError_t launch_synth(int prec, const SynthArgs& a, const BinEdges& E, const PolySpec& W, const void* D, void* V,
                         signed char* bins, void* tw, void* work, void* out, hipStream_t st);
 
+int paint_tiled_ok(int n);
+size_t paint_tiled_work_bytes(int64_t np, int n);
+hipError_t launch_paint_tiled(const double* pos, const double* w, int64_t np, int n, double boxsize, int order,
+                              const double* center, int ncopy, int prec, void* m0, void* m1, void* work,
+                              hipStream_t st);
 hipError_t launch_paint(const double* pos, const double* w, int64_t np, int n, double boxsize, int order,
                         const double* center, int ncopy, int prec, void* m0, void* m1, hipStream_t st);
 hipError_t launch_paint_batch(const double* pos, const double* w, const int64_t* off, int nb, int64_t np_max, int n,

@@ -19,14 +19,9 @@
 // writes one FP64 32 x 32 partial.  A finalize kernel sums the partials in a
 // fixed order (deterministic).  Rows beyond na / nb re-read the last valid
 // row (same cache lines, no extra HBM traffic) and their outputs are dropped.
-#include "cnpc_common.h"
+#include "mfma_split.h"
 
 namespace cnpc {
-
-typedef float f32x4 __attribute__((ext_vector_type(4)));
-typedef double f64x4 __attribute__((ext_vector_type(4)));
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 
 // ------------------------------------------------------------- pairwise
 template <typename T>
@@ -83,43 +78,7 @@ struct GemmArgs {
   double* partial;    // [gridDim.x][32*32]
 };
 
-__device__ __forceinline__ void block_reduce_store(const float (&acc)[2][2][4], double* out) {
-  // acc[bi][bj][j] holds C[16*bi + 4*(lane/16) + j][16*bj + lane%16]
-  __shared__ double red[32 * 32];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  for (int i = threadIdx.x; i < 1024; i += blockDim.x) red[i] = 0.0;
-  __syncthreads();
-  for (int ww = 0; ww < 4; ++ww) {
-    if (w == ww) {
-      for (int bi = 0; bi < 2; ++bi)
-        for (int bj = 0; bj < 2; ++bj)
-          for (int j = 0; j < 4; ++j) {
-            const int r = 16 * bi + 4 * (lane >> 4) + j, c = 16 * bj + (lane & 15);
-            red[r * 32 + c] += (double)acc[bi][bj][j];
-          }
-    }
-    __syncthreads();
-  }
-  for (int i = threadIdx.x; i < 1024; i += blockDim.x) out[i] = red[i];
-}
-
 // ------------------------------------------------------------- gemm_x3
-template <bool F16>
-__device__ __forceinline__ void split8(const float (&v)[8], bf16x8& hi, bf16x8& lo, f16x8& hh, f16x8& ll) {
-#pragma unroll
-  for (int j = 0; j < 8; ++j) {
-    if (F16) {
-      _Float16 h = (_Float16)v[j];
-      hh[j] = h;
-      ll[j] = (_Float16)(v[j] - (float)h);
-    } else {
-      __bf16 h = (__bf16)v[j];
-      hi[j] = h;
-      lo[j] = (__bf16)(v[j] - (float)h);
-    }
-  }
-}
-
 template <bool F16>
 __global__ void __launch_bounds__(256) gemm_x3_kernel(GemmArgs g) {
   const float* __restrict__ G = (const float*)g.G;
@@ -295,6 +254,13 @@ __global__ void __launch_bounds__(64) gemm_finalize_kernel(const double* __restr
   for (int p = threadIdx.x; p < nblk; p += 64) s += partial[(int64_t)p * 1024 + t];
   for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
   if (threadIdx.x == 0) C[(int64_t)(a0 + r) * ldc + (b0 + c)] = s * inv_scale;
+}
+
+hipError_t launch_gemm_finalize(const double* partial, int nblk, int a0, int b0, int na, int nb, double inv_scale,
+                                double* C, int ldc, hipStream_t st) {
+  hipLaunchKernelGGL(gemm_finalize_kernel, dim3(1024), dim3(64), 0, st, partial, nblk, a0, b0, na, nb, inv_scale, C,
+                     ldc);
+  return hipGetLastError();
 }
 
 // mode: 0 bf16x3, 1 f16x3 (FP32 storage); 2 f64 (FP64 storage)

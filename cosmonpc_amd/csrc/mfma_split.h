@@ -1,0 +1,51 @@
+// MFMA helpers shared by the split-precision contractions: operand vector
+// types, the hi + lo split of FP32 values and the fixed-order reduction of
+// the per-wave 32 x 32 accumulators of a workgroup.
+#pragma once
+#include "cnpc_common.h"
+
+namespace cnpc {
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ void block_reduce_store(const float (&acc)[2][2][4], double* out) {
+  // acc[bi][bj][j] holds C[16*bi + 4*(lane/16) + j][16*bj + lane%16]
+  __shared__ double red[32 * 32];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int i = threadIdx.x; i < 1024; i += blockDim.x) red[i] = 0.0;
+  __syncthreads();
+  const int nw = blockDim.x >> 6;
+  for (int ww = 0; ww < nw; ++ww) {
+    if (w == ww) {
+      for (int bi = 0; bi < 2; ++bi)
+        for (int bj = 0; bj < 2; ++bj)
+          for (int j = 0; j < 4; ++j) {
+            const int r = 16 * bi + 4 * (lane >> 4) + j, c = 16 * bj + (lane & 15);
+            red[r * 32 + c] += (double)acc[bi][bj][j];
+          }
+    }
+    __syncthreads();
+  }
+  for (int i = threadIdx.x; i < 1024; i += blockDim.x) out[i] = red[i];
+}
+
+template <bool F16>
+__device__ __forceinline__ void split8(const float (&v)[8], bf16x8& hi, bf16x8& lo, f16x8& hh, f16x8& ll) {
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    if (F16) {
+      _Float16 h = (_Float16)v[j];
+      hh[j] = h;
+      ll[j] = (_Float16)(v[j] - (float)h);
+    } else {
+      __bf16 h = (__bf16)v[j];
+      hi[j] = h;
+      lo[j] = (__bf16)(v[j] - (float)h);
+    }
+  }
+}
+
+}  // namespace cnpc

@@ -5,8 +5,11 @@
 // u = (x - boxcenter) * N / L (+0.5 for the interlacing copy,
 // pm.affine.shift(0.5) at mesh_generator.py:201), periodic wrap.
 //
-// One thread per particle paints both the plain and the shifted copy, so each
-// particle's 32 bytes are read once.  Cell index and fractional offset always
+// Meshes with n % 8 == 0 (16 <= n <= 2048) take the sorted, LDS-tiled
+// owner-computes path below; the others, and the multi-catalog batch, take
+// the scatter path: one thread per particle paints both the plain and the
+// shifted copy with global atomics, so each particle's 32 bytes are read
+// once.  Cell index and fractional offset always
 // come from the FP64 position (a float32 position in a 2000 Mpc/h box would
 // carry ~3e-5 cell of error); weights are formed in FP64 and rounded to the
 // mesh type for the atomic add.
@@ -43,6 +46,11 @@ __device__ __forceinline__ int window(double u, double* w) {
     w[3] = f * f * f / 6.0;
     return (int)i1 - 1;
   }
+}
+
+__global__ void fill_int_kernel(int* __restrict__ p, int64_t count) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0;
 }
 
 __device__ __forceinline__ void atomic_add(double* p, double v) { unsafeAtomicAdd(p, v); }
@@ -156,6 +164,227 @@ hipError_t launch_paint_batch(const double* pos, const double* w, const int64_t*
                                         (double*)m1, st);
   return launch_paint_batch_t<float>(pos, w, off, nb, np_max, n, boxsize, order, center, ncopy, (float*)m0,
                                      (float*)m1, st);
+}
+
+// ---------------------------------------------------------------- tiled paint
+// Owner-computes painting (n % 8 == 0, 16 <= n <= 2048).  The particles are
+// binned by a counting sort on the 8^3 tile of their window's first cell
+// (plain copy).  One workgroup per tile then accumulates in LDS the window
+// weights of the particles of the 8 tiles that can reach it: the tile and its
+// lower neighbours, since a window spans at most 4 cells and the half-cell
+// shift of the interlacing copy moves its first cell by at most one.  The
+// workgroup writes its whole tile once with plain stores (which is also the
+// zero fill): no global atomics, and every cell is written by one workgroup.
+// The weights are the same FP64 products as paint_one, rounded to the mesh
+// type before the (LDS) addition.
+constexpr int PT = 8;
+constexpr int PT3 = PT * PT * PT;
+constexpr int PAINT_TILE_THREADS = 128;
+
+template <int ORDER>
+__global__ void __launch_bounds__(256) paint_key_kernel(const double* __restrict__ pos, int64_t np, int n,
+                                                        double scale, double cx, double cy, double cz, int nt,
+                                                        int* __restrict__ key, int* __restrict__ count) {
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < np; p += (int64_t)gridDim.x * blockDim.x) {
+    double wx[ORDER];
+    int a[3];
+    const double c[3] = {cx, cy, cz};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      int ad = window<ORDER>(__dadd_rn(__dmul_rn(pos[3 * p + d] - c[d], scale), 0.0), wx);
+      ad %= n;
+      if (ad < 0) ad += n;
+      a[d] = ad / PT;
+    }
+    const int k = (a[0] * nt + a[1]) * nt + a[2];
+    key[p] = k;
+    atomicAdd(count + k, 1);
+  }
+}
+
+// exclusive prefix sum of count[0 .. m) into start[0 .. m], one workgroup of
+// 1024 threads (each thread a contiguous chunk)
+__global__ void __launch_bounds__(1024) paint_scan_kernel(const int* __restrict__ count, int m,
+                                                          int* __restrict__ start) {
+  __shared__ int part[1024];
+  const int t = threadIdx.x;
+  const int chunk = (m + 1023) / 1024;
+  const int b = t * chunk, e = min(m, b + chunk);
+  int s = 0;
+  for (int i = b; i < e; ++i) s += count[i];
+  part[t] = s;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    const int v = t >= off ? part[t - off] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  int run = part[t] - s;  // exclusive prefix of this chunk
+  for (int i = b; i < e; ++i) {
+    start[i] = run;
+    run += count[i];
+  }
+  if (t == 1023) start[m] = part[1023];
+}
+
+__global__ void __launch_bounds__(256) paint_scatter_kernel(const double* __restrict__ pos,
+                                                            const double* __restrict__ w, int64_t np,
+                                                            const int* __restrict__ key,
+                                                            const int* __restrict__ start, int* __restrict__ cursor,
+                                                            double* __restrict__ spos, double* __restrict__ sw) {
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < np; p += (int64_t)gridDim.x * blockDim.x) {
+    const int k = key[p];
+    const int64_t q = (int64_t)start[k] + atomicAdd(cursor + k, 1);
+    spos[3 * q + 0] = pos[3 * p + 0];
+    spos[3 * q + 1] = pos[3 * p + 1];
+    spos[3 * q + 2] = pos[3 * p + 2];
+    sw[q] = w[p];
+  }
+}
+
+template <typename T, int ORDER>
+__global__ void __launch_bounds__(PAINT_TILE_THREADS) paint_tile_kernel(
+    const double* __restrict__ spos, const double* __restrict__ sw, const int* __restrict__ start, int n, int nt,
+    double scale, double cx, double cy, double cz, int ncopy, T* __restrict__ mesh0, T* __restrict__ mesh1) {
+  __shared__ T acc[2 * PT3];
+  const int tile = blockIdx.x;
+  const int tz = tile % nt, ty = (tile / nt) % nt, tx = tile / (nt * nt);
+  const int x0 = tx * PT, y0 = ty * PT, z0 = tz * PT;
+  for (int i = threadIdx.x; i < 2 * PT3; i += blockDim.x) acc[i] = (T)0;
+  // the 8 source tiles and their particle ranges, as one virtual range
+  int sb[8], pre[9];
+  pre[0] = 0;
+#pragma unroll
+  for (int s = 0; s < 8; ++s) {
+    const int sx = (s & 4) ? (tx == 0 ? nt - 1 : tx - 1) : tx;
+    const int sy = (s & 2) ? (ty == 0 ? nt - 1 : ty - 1) : ty;
+    const int sz = (s & 1) ? (tz == 0 ? nt - 1 : tz - 1) : tz;
+    const int src = (sx * nt + sy) * nt + sz;
+    sb[s] = start[src];
+    pre[s + 1] = pre[s] + (start[src + 1] - sb[s]);
+  }
+  __syncthreads();
+  for (int v = threadIdx.x; v < pre[8]; v += blockDim.x) {
+    int s = 0;
+#pragma unroll
+    for (int q = 1; q < 8; ++q) s += (v >= pre[q]) ? 1 : 0;
+    const int64_t p = (int64_t)sb[s] + (v - pre[s]);
+    const double x = spos[3 * p + 0] - cx;
+    const double y = spos[3 * p + 1] - cy;
+    const double z = spos[3 * p + 2] - cz;
+    const double m = sw[p];
+    for (int c = 0; c < ncopy; ++c) {
+      const double sh = 0.5 * c;
+      double wx[ORDER], wy[ORDER], wz[ORDER];
+      int ax = window<ORDER>(__dadd_rn(__dmul_rn(x, scale), sh), wx);
+      int ay = window<ORDER>(__dadd_rn(__dmul_rn(y, scale), sh), wy);
+      int az = window<ORDER>(__dadd_rn(__dmul_rn(z, scale), sh), wz);
+      ax %= n; if (ax < 0) ax += n;
+      ay %= n; if (ay < 0) ay += n;
+      az %= n; if (az < 0) az += n;
+      // offsets of the first window cell from the tile origin, in [0, n)
+      int rx = ax - x0; if (rx < 0) rx += n;
+      int ry = ay - y0; if (ry < 0) ry += n;
+      int rz = az - z0; if (rz < 0) rz += n;
+      T* a = acc + c * PT3;
+#pragma unroll
+      for (int i = 0; i < ORDER; ++i) {
+        int ix = rx + i; if (ix >= n) ix -= n;
+        if (ix >= PT) continue;
+        const double mx = m * wx[i];
+#pragma unroll
+        for (int j = 0; j < ORDER; ++j) {
+          int iy = ry + j; if (iy >= n) iy -= n;
+          if (iy >= PT) continue;
+          const double mxy = mx * wy[j];
+#pragma unroll
+          for (int k = 0; k < ORDER; ++k) {
+            int iz = rz + k; if (iz >= n) iz -= n;
+            if (iz >= PT) continue;
+            atomicAdd(a + (ix * PT + iy) * PT + iz, (T)(mxy * wz[k]));
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < ncopy * PT3; t += blockDim.x) {
+    const int c = t / PT3, cell = t - c * PT3;
+    const int ix = cell / (PT * PT), iy = (cell / PT) % PT, iz = cell % PT;
+    T* mesh = c == 0 ? mesh0 : mesh1;
+    mesh[((int64_t)(x0 + ix) * n + (y0 + iy)) * n + z0 + iz] = acc[t];
+  }
+}
+
+int paint_tiled_ok(int n) { return n % PT == 0 && n >= 2 * PT && n <= 2048; }
+
+size_t paint_tiled_work_bytes(int64_t np, int n) {
+  const int64_t nt = n / PT, m = nt * nt * nt;
+  auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+  return al(np * 4) + 3 * al((m + 1) * 4) + al(np * 24) + al(np * 8);
+}
+
+template <typename T, int ORDER>
+static void launch_tiles(const double* spos, const double* sw, const int* start, int n, int nt, double scale,
+                         double cx, double cy, double cz, int ncopy, T* m0, T* m1, hipStream_t st) {
+  hipLaunchKernelGGL((paint_tile_kernel<T, ORDER>), dim3(nt * nt * nt), dim3(PAINT_TILE_THREADS), 0, st, spos, sw,
+                     start, n, nt, scale, cx, cy, cz, ncopy, m0, m1);
+}
+
+template <typename T>
+static hipError_t launch_paint_tiled_t(const double* pos, const double* w, int64_t np, int n, double boxsize,
+                                       int order, const double* center, int ncopy, T* m0, T* m1, void* work,
+                                       hipStream_t st) {
+  const double scale = (double)n / boxsize;
+  const double cx = center ? center[0] : 0.0, cy = center ? center[1] : 0.0, cz = center ? center[2] : 0.0;
+  const int nt = n / PT;
+  const int m = nt * nt * nt;
+  auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+  char* base = (char*)work;
+  int* key = (int*)base;
+  base += al(np * 4);
+  int* count = (int*)base;
+  base += al((size_t)(m + 1) * 4);
+  int* cursor = (int*)base;
+  base += al((size_t)(m + 1) * 4);
+  int* start = (int*)base;
+  base += al((size_t)(m + 1) * 4);
+  double* spos = (double*)base;
+  base += al(np * 24);
+  double* sw = (double*)base;
+  dim3 blk(256), grd(grid_for(np > 0 ? np : 1, 256, 1 << 16));
+  hipLaunchKernelGGL(fill_int_kernel, dim3(grid_for(2 * (int64_t)(m + 1), 256, 1 << 14)), blk, 0, st, count,
+                     2 * (int64_t)al((size_t)(m + 1) * 4) / 4);
+  if (np > 0) {
+    switch (order) {
+      case 1: hipLaunchKernelGGL(paint_key_kernel<1>, grd, blk, 0, st, pos, np, n, scale, cx, cy, cz, nt, key, count); break;
+      case 2: hipLaunchKernelGGL(paint_key_kernel<2>, grd, blk, 0, st, pos, np, n, scale, cx, cy, cz, nt, key, count); break;
+      case 3: hipLaunchKernelGGL(paint_key_kernel<3>, grd, blk, 0, st, pos, np, n, scale, cx, cy, cz, nt, key, count); break;
+      case 4: hipLaunchKernelGGL(paint_key_kernel<4>, grd, blk, 0, st, pos, np, n, scale, cx, cy, cz, nt, key, count); break;
+      default: return hipErrorInvalidValue;
+    }
+  }
+  hipLaunchKernelGGL(paint_scan_kernel, dim3(1), dim3(1024), 0, st, count, m, start);
+  if (np > 0)
+    hipLaunchKernelGGL(paint_scatter_kernel, grd, blk, 0, st, pos, w, np, key, start, cursor, spos, sw);
+  switch (order) {
+    case 1: launch_tiles<T, 1>(spos, sw, start, n, nt, scale, cx, cy, cz, ncopy, m0, m1, st); break;
+    case 2: launch_tiles<T, 2>(spos, sw, start, n, nt, scale, cx, cy, cz, ncopy, m0, m1, st); break;
+    case 3: launch_tiles<T, 3>(spos, sw, start, n, nt, scale, cx, cy, cz, ncopy, m0, m1, st); break;
+    case 4: launch_tiles<T, 4>(spos, sw, start, n, nt, scale, cx, cy, cz, ncopy, m0, m1, st); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_paint_tiled(const double* pos, const double* w, int64_t np, int n, double boxsize, int order,
+                              const double* center, int ncopy, int prec, void* m0, void* m1, void* work,
+                              hipStream_t st) {
+  if (prec == 0)
+    return launch_paint_tiled_t<double>(pos, w, np, n, boxsize, order, center, ncopy, (double*)m0, (double*)m1,
+                                        work, st);
+  return launch_paint_tiled_t<float>(pos, w, np, n, boxsize, order, center, ncopy, (float*)m0, (float*)m1, work, st);
 }
 
 hipError_t launch_paint(const double* pos, const double* w, int64_t np, int n, double boxsize, int order,

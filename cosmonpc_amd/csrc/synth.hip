@@ -23,26 +23,9 @@
 // Stockham writes stride by the radix.  All transforms are the unnormalised
 // inverse e^{+i k.x} of the reference's pmesh c2r.  Twiddles come from one
 // table W[j] = e^{2 pi i j / N} in LDS.
-#include "cnpc_common.h"
+#include "lds_fft.h"
 
 namespace cnpc {
-
-template <typename C>
-__device__ __forceinline__ C cmul(C a, C b) {
-  C r;
-  r.x = a.x * b.x - a.y * b.y;
-  r.y = a.x * b.y + a.y * b.x;
-  return r;
-}
-
-// floor(a / b) for 0 <= a < 2^22 and 1 <= b <= 2^12 from a float reciprocal,
-// corrected to the exact quotient (the float estimate is within one).
-__device__ __forceinline__ int fdiv(int a, int b, float inv_b) {
-  int q = (int)(((float)a + 0.5f) * inv_b);
-  if (q * b > a) --q;
-  else if ((q + 1) * b <= a) ++q;
-  return q;
-}
 
 // ------------------------------------------------------------------ cube
 template <typename T>
@@ -81,107 +64,6 @@ __global__ void __launch_bounds__(256) cube_kernel(SynthArgs a, BinEdges E, Poly
     bins[idx] = (signed char)(b < 0 ? -1 : b);
   }
 }
-
-// ------------------------------------------------------------------ LDS FFT
-// cos / sin of 2 pi j / R for the small DFTs
-template <int RR> struct Roots;
-template <> struct Roots<2> {
-  static constexpr double c[2] = {1.0, -1.0};
-  static constexpr double s[2] = {0.0, 0.0};
-};
-template <> struct Roots<3> {
-  static constexpr double c[3] = {1.0, -0.5, -0.5};
-  static constexpr double s[3] = {0.0, 0.86602540378443865, -0.86602540378443865};
-};
-template <> struct Roots<4> {
-  static constexpr double c[4] = {1.0, 0.0, -1.0, 0.0};
-  static constexpr double s[4] = {0.0, 1.0, 0.0, -1.0};
-};
-template <> struct Roots<5> {
-  static constexpr double c[5] = {1.0, 0.30901699437494742, -0.80901699437494742, -0.80901699437494742,
-                                  0.30901699437494742};
-  static constexpr double s[5] = {0.0, 0.95105651629515357, 0.58778525229247313, -0.58778525229247313,
-                                  -0.95105651629515357};
-};
-
-// One in-place pass of the mixed-radix Stockham inverse FFT (e^{+2 pi i})
-// over nl lines of length n, line l at buf + l*ls.  Butterfly i in [0, n/R)
-// reads buf[i + r n/R], multiplies by e^{2 pi i r k/(p R)} (k = i mod p), does
-// an R-point DFT and writes buf[(i/p) p R + k + r p].  All reads are staged in
-// registers before the barrier, so one buffer suffices; a thread owns at most
-// MAXB butterflies (the host sizes nl accordingly).
-template <int RR, int MAXB, typename C>
-__device__ __forceinline__ void stockham_pass(C* __restrict__ buf, const C* __restrict__ W, int n, int ls, int nl,
-                                              int p) {
-  typedef decltype(C().x) T;
-  const int Tn = n / RR;
-  const int pR = p * RR;
-  const int tstep = n / pR;
-  const int total = nl * Tn;
-  const float inv_Tn = 1.0f / (float)Tn, inv_p = 1.0f / (float)p;
-  C u[MAXB][RR];
-  int dsto[MAXB];
-#pragma unroll
-  for (int it = 0; it < MAXB; ++it) {
-    const int t = threadIdx.x + it * blockDim.x;
-    dsto[it] = -1;
-    if (t < total) {
-      const int l = fdiv(t, Tn, inv_Tn), i = t - l * Tn;
-      const int q = fdiv(i, p, inv_p), k = i - q * p;
-      const C* s = buf + l * ls;
-#pragma unroll
-      for (int r = 0; r < RR; ++r) {
-        u[it][r] = s[i + r * Tn];
-        if (r) u[it][r] = cmul(u[it][r], W[r * k * tstep]);
-      }
-      dsto[it] = l * ls + q * pR + k;
-    }
-  }
-  __syncthreads();
-#pragma unroll
-  for (int it = 0; it < MAXB; ++it) {
-    if (dsto[it] >= 0) {
-      C* d = buf + dsto[it];
-#pragma unroll
-      for (int b = 0; b < RR; ++b) {
-        C v = u[it][0];
-#pragma unroll
-        for (int q = 1; q < RR; ++q) {
-          const int j = (q * b) % RR;
-          const T wc = (T)Roots<RR>::c[j], ws = (T)Roots<RR>::s[j];
-          v.x += u[it][q].x * wc - u[it][q].y * ws;
-          v.y += u[it][q].x * ws + u[it][q].y * wc;
-        }
-        d[b * p] = v;
-      }
-    }
-  }
-  __syncthreads();
-}
-
-template <int MAXB, typename C>
-__device__ __forceinline__ void stockham_single(C* buf, const C* W, int n, int ls, int nl, int R, int p) {
-  if (R == 4) stockham_pass<4, MAXB>(buf, W, n, ls, nl, p);
-  else if (R == 2) stockham_pass<2, MAXB>(buf, W, n, ls, nl, p);
-  else if (R == 3) stockham_pass<3, MAXB>(buf, W, n, ls, nl, p);
-  else stockham_pass<5, MAXB>(buf, W, n, ls, nl, p);
-}
-
-// Whole inverse FFT of nl lines.
-template <int MAXB, typename C>
-__device__ void stockham_lds(C* buf, const C* W, int n, int ls, int nl, const SynthArgs& a) {
-  int p = 1;
-  for (int s = 0; s < a.nrad; ++s) {
-    stockham_single<MAXB>(buf, W, n, ls, nl, a.rad[s], p);
-    p *= a.rad[s];
-  }
-}
-
-// butterflies per thread of a pass (bounds the registers of the staged pass)
-template <typename T> struct MaxB { static constexpr int v = 4; };
-
-// global loads a thread issues before writing them to LDS
-constexpr int UNR = 8;
 
 // ------------------------------------------------------------------ x pass
 // one workgroup per (block of NL ky lines, kz); line l = ky index ky0 + l
@@ -246,11 +128,14 @@ __global__ void __launch_bounds__(256) xfft_kernel(SynthArgs a, int NL, const ty
 }
 
 // ------------------------------------------------------------------ y pass
-// one workgroup per (plane s*n + x, block of NL kz lines)
+// one workgroup per (plane s*n + x, block of NL kz lines).  nslots == 0: the
+// output is T[s][x][kz][y] for the z pass; nslots > 0: the half-synthesised
+// layout H[x][y/2][slot0 + s][kz][y%2] of the fused z-synthesis contraction
+// (each (x, y pair) tile one contiguous run over all slots).
 template <typename T>
 __global__ void __launch_bounds__(256) yfft_kernel(SynthArgs a, int NL, const typename cplx<T>::type* __restrict__ U,
                                                    const typename cplx<T>::type* __restrict__ twg,
-                                                   typename cplx<T>::type* __restrict__ Tm) {
+                                                   typename cplx<T>::type* __restrict__ Tm, int nslots, int slot0) {
   typedef typename cplx<T>::type C;
   const int n = a.n, R = a.R, P = 2 * R + 1, Q = R + 1, ls = n + 1;
   const int64_t plane = blockIdx.x;
@@ -284,10 +169,23 @@ __global__ void __launch_bounds__(256) yfft_kernel(SynthArgs a, int NL, const ty
   }
   __syncthreads();
   stockham_lds<MaxB<T>::v>(buf, W, n, ls, nl, a);
-  C* dst = Tm + (plane * Q + kz0) * n;
-  for (int t = threadIdx.x; t < nl * n; t += blockDim.x) {
-    const int l = fdiv(t, n, inv_n), y = t - l * n;
-    dst[(int64_t)l * n + y] = buf[l * ls + y];
+  if (nslots == 0) {
+    C* dst = Tm + (plane * Q + kz0) * n;
+    for (int t = threadIdx.x; t < nl * n; t += blockDim.x) {
+      const int l = fdiv(t, n, inv_n), y = t - l * n;
+      dst[(int64_t)l * n + y] = buf[l * ls + y];
+    }
+  } else {
+    // consecutive threads: (y%2, l) fastest, so each y pair writes 2 nl
+    // consecutive entries
+    const int s = (int)(plane / n), x = (int)(plane - (int64_t)s * n);
+    const float inv_nl = 1.0f / (float)nl;
+    const int64_t xbase = (int64_t)x * (n >> 1);
+    for (int t = threadIdx.x; t < nl * n; t += blockDim.x) {
+      const int j = t & 1, r = t >> 1;
+      const int yp = fdiv(r, nl, inv_nl), l = r - yp * nl;
+      Tm[(((xbase + yp) * nslots + slot0 + s) * Q + kz0 + l) * 2 + j] = buf[l * ls + 2 * yp + j];
+    }
   }
 }
 
@@ -416,7 +314,7 @@ static hipError_t launch_synth_t(const SynthArgs& a, const BinEdges& E, const Po
   hipLaunchKernelGGL(xfft_kernel<T>, dim3((P + NL - 1) / NL, Q), blk, lds, st, a, NL, (const C*)V, bins, (const C*)tw,
                      U);
   hipLaunchKernelGGL(yfft_kernel<T>, dim3((unsigned)planes, (Q + NL - 1) / NL), blk, lds, st, a, NL, (const C*)U,
-                     (const C*)tw, Tm);
+                     (const C*)tw, Tm, 0, 0);
   hipLaunchKernelGGL(zfft_kernel<T>, dim3((unsigned)planes, (n + 2 * NL - 1) / (2 * NL)), blk, lds, st, a, NL,
                      (const C*)Tm, (const C*)tw, (T*)out);
   return hipGetLastError();
@@ -426,6 +324,33 @@ hipError_t launch_synth(int prec, const SynthArgs& a, const BinEdges& E, const P
                         signed char* bins, void* tw, void* work, void* out, hipStream_t st) {
   if (prec == 0) return launch_synth_t<double>(a, E, W, D, V, bins, tw, work, out, st);
   return launch_synth_t<float>(a, E, W, D, V, bins, tw, work, out, st);
+}
+
+hipError_t launch_twiddles(int n, void* tw, hipStream_t st) {
+  hipLaunchKernelGGL(twiddle_kernel<float>, dim3((n + 255) / 256), dim3(256), 0, st, n, (float2*)tw);
+  return hipGetLastError();
+}
+
+// x and y passes only (FP32), into slots slot0 .. slot0 + nshell - 1 of the
+// half-synthesised layout H[x][y/2][nslots][R+1][2]
+hipError_t launch_synth_half(const SynthArgs& a, const BinEdges& E, const PolySpec& Wp, const void* D, void* V,
+                             signed char* bins, void* tw, void* work, void* out, int nslots, int slot0,
+                             hipStream_t st) {
+  typedef float2 C;
+  const int R = a.R, P = 2 * R + 1, Q = R + 1, n = a.n;
+  const int64_t cube = (int64_t)P * Q * P;
+  dim3 blk(256);
+  hipLaunchKernelGGL(twiddle_kernel<float>, dim3((n + 255) / 256), blk, 0, st, n, (C*)tw);
+  hipLaunchKernelGGL(cube_kernel<float>, dim3(grid_for(cube, 256, 1 << 14)), blk, 0, st, a, E, Wp, (const C*)D,
+                     (C*)V, bins);
+  const int NL = synth_lines_per_block(1, a);
+  const size_t lds = sizeof(C) * ((size_t)n + (size_t)NL * (n + 1));
+  const int64_t planes = (int64_t)a.nshell * n;
+  hipLaunchKernelGGL(xfft_kernel<float>, dim3((P + NL - 1) / NL, Q), blk, lds, st, a, NL, (const C*)V, bins,
+                     (const C*)tw, (C*)work);
+  hipLaunchKernelGGL(yfft_kernel<float>, dim3((unsigned)planes, (Q + NL - 1) / NL), blk, lds, st, a, NL,
+                     (const C*)work, (const C*)tw, (C*)out, nslots, slot0);
+  return hipGetLastError();
 }
 
 }  // namespace cnpc

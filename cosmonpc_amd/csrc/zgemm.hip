@@ -1,0 +1,223 @@
+// Fused z-synthesis + split-bf16 contraction  S_ab = sum_x A_a(x) G(x) B_b(x).
+//
+// The shells A_a, B_b are the real inverse FFTs of shell-filtered spectra
+// (clustering_estimator.py:934-970) and S_ab is the reference's
+// np.sum(G_00 * F_i * F_j) (:972).  The shell synthesis leaves them half
+// done, x and y transformed, in H[x][y/2][slot][kz][y%2] for the R + 1
+// non-zero kz.  Writing the real shells (N^3 FP32 each) and reading them back
+// in the GEMM is most of the HBM traffic of the unfused path; here the z pass
+// runs in LDS right before the MFMAs and the real shells never reach HBM.
+//
+// A workgroup (12 waves) loops over tiles (x, y pair), one contiguous run of H over all
+// slots, and handles the two rows y of a tile in turn.  For a row:
+//   1. two real shells share one complex line (two real transforms per
+//      complex FFT): line l = P + iQ with, along z, the Hermitian half
+//      spectra p, q of the two shells expanded as w[kz] = p + iq,
+//      w[n-kz] = conj(p) + i conj(q).  For B == A the pairs are (A_2l,
+//      A_2l+1); for B != A they are (A_l, B_l);
+//   2. the lines get the in-place Stockham inverse FFT in LDS (real part =
+//      first shell, imaginary part = second shell, z = 0 .. n-1);
+//   3. the row's n values of K are cut in chunks of 32 (the last one padded
+//      with zeros); wave w takes chunks w, w + nw, ...: its lanes read 8
+//      consecutive values of their A and B rows, form B * G, split hi + lo in
+//      bf16 and issue the three 16x16x32 MFMAs of each of the 2 x 2 output
+//      blocks (FP32 accumulate).
+// The per-wave accumulators are summed in a fixed order into one FP64
+// 32 x 32 partial per workgroup, and the partials in a fixed order by the
+// finalize kernel (deterministic).
+#include "lds_fft.h"
+#include "mfma_split.h"
+
+namespace cnpc {
+
+// 12 waves, 3 per SIMD: the FFT staging and the MFMA operands fit the 168
+// registers of that occupancy without spills
+constexpr int ZG_THREADS = 768;
+constexpr int ZG_MAXB = 4;  // butterflies per thread and FFT pass
+
+struct ZGemmArgs {
+  int n, Q;                 // grid, non-zero kz per line (R + 1)
+  int na, nb;               // rows of A and B (<= 32)
+  int sa_slots, sb_slots;   // slots per tile in HA / HB
+  int same;                 // B is A
+  int nlines;               // complex lines per row
+  int ntiles;               // n * n / 2
+  SynthArgs fft;            // n and its radices
+  const float4* HA;
+  const float4* HB;
+  const float* G;           // [x][y][z]
+  const float2* tw;         // e^{2 pi i j / n}
+  double* partial;          // [gridDim.x][32 * 32]
+};
+
+// half spectrum of row r (0: y, 1: y + 1) of slot s of a tile, or 0
+__device__ __forceinline__ float2 zg_half(const float4* __restrict__ T, int nvalid, int s, int kz, int Q, int r) {
+  if (s >= nvalid) return make_float2(0.f, 0.f);
+  const float4 v = T[s * Q + kz];
+  return r ? make_float2(v.z, v.w) : make_float2(v.x, v.y);
+}
+
+// 8 consecutive z values of one part of one line (0 where !ok or z >= n)
+__device__ __forceinline__ void zg_row8(const float2* buf, int ls, int line, int part, bool ok, int z0, int n,
+                                        float (&v)[8]) {
+  const float* p = (const float*)(buf + line * ls + z0) + part;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const float c = p[2 * j];
+    v[j] = (ok && z0 + j < n) ? c : 0.f;
+  }
+}
+
+__global__ void __launch_bounds__(ZG_THREADS) zgemm_bf16x3_kernel(ZGemmArgs g) {
+  const int n = g.n, Q = g.Q, R = Q - 1, ls = n + 1, half = n >> 1, L = g.nlines;
+  extern __shared__ unsigned char smem[];
+  float2* W = (float2*)smem;
+  float2* buf = W + n;
+  for (int j = threadIdx.x; j < n; j += blockDim.x) W[j] = g.tw[j];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  const int rr = lane & 15, kg = lane >> 4;
+  const int nch = (n + 31) >> 5;
+  const int nz0 = n - 2 * R - 1;
+  const float inv_z = 1.0f / (float)(nz0 > 0 ? nz0 : 1), inv_q = 1.0f / (float)Q;
+  // A row i / B row j of this lane -> (line, part); rows past na / nb read
+  // a valid line and are masked to zero
+  int la[2], pa[2], lb[2], pb[2];
+  bool va[2], vb[2];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int i = 16 * h + rr;
+    va[h] = i < g.na;
+    vb[h] = i < g.nb;
+    if (g.same) {
+      la[h] = lb[h] = i >> 1;
+      pa[h] = pb[h] = i & 1;
+    } else {
+      la[h] = lb[h] = i;
+      pa[h] = 0;
+      pb[h] = 1;
+    }
+    if (la[h] >= L) la[h] = L - 1;
+    if (lb[h] >= L) lb[h] = L - 1;
+  }
+  float acc[2][2][4];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) acc[i][j][q] = 0.f;
+  f16x8 fdh, fdl;  // unused fp16 outputs of split8
+  for (int64_t tile = blockIdx.x; tile < g.ntiles; tile += gridDim.x) {
+    const int x = (int)(tile / half), yp = (int)(tile - (int64_t)x * half);
+    const float4* TA = g.HA + tile * g.sa_slots * Q;
+    const float4* TB = g.HB + tile * g.sb_slots * Q;
+    for (int r = 0; r < 2; ++r) {
+      // modes Q .. n-Q are zero (the FFT of the previous row overwrote them)
+      for (int t = threadIdx.x; t < L * nz0; t += blockDim.x) {
+        const int l = fdiv(t, nz0, inv_z), j = t - l * nz0;
+        buf[l * ls + Q + j] = make_float2(0.f, 0.f);
+      }
+      for (int t = threadIdx.x; t < L * Q; t += blockDim.x) {
+        const int l = fdiv(t, Q, inv_q), kz = t - l * Q;
+        float2 p, q;
+        if (g.same) {
+          p = zg_half(TA, g.na, 2 * l, kz, Q, r);
+          q = zg_half(TA, g.na, 2 * l + 1, kz, Q, r);
+        } else {
+          p = zg_half(TA, g.na, l, kz, Q, r);
+          q = zg_half(TB, g.nb, l, kz, Q, r);
+        }
+        float2* Ln = buf + l * ls;
+        if (kz == 0) {  // c2r reads only the real part of the zero mode
+          Ln[0] = make_float2(p.x, q.x);
+        } else {
+          Ln[kz] = make_float2(p.x - q.y, p.y + q.x);
+          Ln[n - kz] = make_float2(p.x + q.y, q.x - p.y);
+        }
+      }
+      __syncthreads();
+      stockham_lds<ZG_MAXB>(buf, W, n, ls, L, g.fft);
+      const float* Grow = g.G + ((int64_t)x * n + 2 * yp + r) * n;
+      for (int c = w; c < nch; c += nw) {
+        const int z0 = 32 * c + 8 * kg;
+        const bool zin = z0 < n;
+        float vg[8];
+        if (z0 + 8 <= n) {
+          const float4* gp = (const float4*)(Grow + z0);
+          const float4 g0 = gp[0], g1 = gp[1];
+          vg[0] = g0.x; vg[1] = g0.y; vg[2] = g0.z; vg[3] = g0.w;
+          vg[4] = g1.x; vg[5] = g1.y; vg[6] = g1.z; vg[7] = g1.w;
+        } else {
+#pragma unroll
+          for (int j = 0; j < 8; ++j) vg[j] = 0.f;
+        }
+        const int zr = zin ? z0 : 0;  // LDS reads stay inside the line
+        bf16x8 ah[2], al[2], bh[2], bl[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          float a[8], b[8];
+          zg_row8(buf, ls, la[h], pa[h], zin && va[h], zr, n, a);
+          zg_row8(buf, ls, lb[h], pb[h], zin && vb[h], zr, n, b);
+#pragma unroll
+          for (int j = 0; j < 8; ++j) b[j] *= vg[j];
+          split8<false>(a, ah[h], al[h], fdh, fdl);
+          split8<false>(b, bh[h], bl[h], fdh, fdl);
+        }
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int j = 0; j < 2; ++j) {
+            f32x4 cacc = {acc[i][j][0], acc[i][j][1], acc[i][j][2], acc[i][j][3]};
+            cacc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], cacc, 0, 0, 0);
+            cacc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], cacc, 0, 0, 0);
+            cacc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], cacc, 0, 0, 0);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) acc[i][j][q] = cacc[q];
+          }
+      }
+      __syncthreads();  // the next row overwrites buf
+    }
+  }
+  block_reduce_store(acc, g.partial + (int64_t)blockIdx.x * 1024);
+}
+
+static int zg_lines(int na, int nb, int same) { return same ? (na + 1) / 2 : (na > nb ? na : nb); }
+
+size_t zgemm_lds_bytes(int n, int nlines) { return sizeof(float2) * ((size_t)n + (size_t)nlines * (n + 1)); }
+
+// host check: every pass of the line FFT fits the staged registers and the
+// line buffers (plus the 8 KB reduction tile) fit the LDS
+int zgemm_supported(const SynthArgs& a, int na, int nb, int same, size_t lds_limit) {
+  const int n = a.n;
+  if (n % 8 != 0 || na < 1 || na > 32 || nb < 1 || nb > 32) return 0;
+  const int L = zg_lines(na, nb, same);
+  for (int s = 0; s < a.nrad; ++s)
+    if ((int64_t)L * (n / a.rad[s]) > (int64_t)ZG_MAXB * ZG_THREADS) return 0;
+  return zgemm_lds_bytes(n, L) + 32 * 32 * sizeof(double) <= lds_limit ? 1 : 0;
+}
+
+hipError_t launch_zgemm(const SynthArgs& a, int Q, const void* HA, int na, int sa_slots, const void* HB, int nb,
+                        int sb_slots, int same, const float* G, const void* tw, int nblk, double* partial, double* C,
+                        hipStream_t st) {
+  ZGemmArgs g;
+  g.n = a.n; g.Q = Q; g.na = na; g.nb = same ? na : nb;
+  g.sa_slots = sa_slots; g.sb_slots = same ? sa_slots : sb_slots;
+  g.same = same;
+  g.nlines = zg_lines(na, g.nb, same);
+  g.ntiles = a.n * (a.n / 2);
+  g.fft = a;
+  g.HA = (const float4*)HA;
+  g.HB = (const float4*)(same ? HA : HB);
+  g.G = G;
+  g.tw = (const float2*)tw;
+  g.partial = partial;
+  const size_t lds = zgemm_lds_bytes(a.n, g.nlines);
+  // dynamic LDS beyond the default 64 KB must be allowed per kernel
+  hipError_t e = hipFuncSetAttribute((const void*)zgemm_bf16x3_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(zgemm_bf16x3_kernel, dim3(nblk), dim3(ZG_THREADS), lds, st, g);
+  return launch_gemm_finalize(partial, nblk, 0, 0, na, g.nb, 1.0, C, g.nb, st);
+}
+
+}  // namespace cnpc

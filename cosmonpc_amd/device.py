@@ -3,6 +3,7 @@ for every buffer.  Thin typed wrappers over the C ABI (include/cnpc.h); every
 call validates shapes and dtypes on the host before anything is launched.
 """
 import ctypes
+import math
 
 import numpy as np
 import torch
@@ -173,6 +174,59 @@ class GpuContext:
         check(self.lib.cnpc_shell_synth(self.h, n, float(boxsize), prec, _ptr(delta),
                                         edges.ctypes.data_as(ctypes.c_void_p), len(edges) - 1, int(bin0), int(nshell),
                                         ctypes.byref(poly), _ptr(work), _ptr(out)), "cnpc_shell_synth")
+
+    @staticmethod
+    def synth_radius(boxsize, kmax):
+        """R = ceil(kmax / kf): every shell mode has |n_d| <= R."""
+        return int(math.ceil(float(kmax) / (2.0 * math.pi / float(boxsize))))
+
+    def half_shells(self, n, R, nslots):
+        """Buffer of the half-synthesised layout [x][y/2][nslots][R+1][y%2]
+        (complex64)."""
+        return torch.empty((n, n // 2, nslots, R + 1, 2), dtype=torch.complex64, device=self.device)
+
+    def shell_synth_half(self, delta, n, boxsize, edges, bin0, nshell, poly, work, out, slot0):
+        """x and y passes of the FP32 shell synthesis of bins bin0 .. bin0 +
+        nshell - 1 into slots slot0 .. of ``out`` (from ``half_shells``)."""
+        edges = np.ascontiguousarray(edges, dtype=np.float64)
+        R = self.synth_radius(boxsize, edges[-1])
+        if delta is not None:
+            self._check_spec(delta, n, 1)
+        self._check_spec(work[:nshell], n, 1, nshell)
+        if (out.dtype != torch.complex64 or out.dim() != 5 or tuple(out.shape[:2]) != (n, n // 2)
+                or tuple(out.shape[3:]) != (R + 1, 2) or not out.is_contiguous() or not out.is_cuda):
+            raise ValueError(f"half shells must be a contiguous ({n}, {n // 2}, nslots, {R + 1}, 2) complex64 tensor")
+        check(self.lib.cnpc_shell_synth_half(self.h, n, float(boxsize), _ptr(delta),
+                                             edges.ctypes.data_as(ctypes.c_void_p), len(edges) - 1, int(bin0),
+                                             int(nshell), ctypes.byref(poly), _ptr(work), _ptr(out),
+                                             out.shape[2], int(slot0)), "cnpc_shell_synth_half")
+
+    def zgemm_supported(self, n, na, nb, same):
+        ok = ctypes.c_int(0)
+        check(self.lib.cnpc_contract_zgemm_supported(self.h, n, int(na), int(nb), int(bool(same)), ctypes.byref(ok)),
+              "cnpc_contract_zgemm_supported")
+        return bool(ok.value)
+
+    def contract_zgemm(self, n, boxsize, kmax, G, HA, na, HB=None, nb=None, nblk=0):
+        """C[a, b] = sum_x A_a G B_b over the shells in slots 0 .. na-1 of HA
+        and 0 .. nb-1 of HB (None: B = A), z pass fused (device float64
+        tensor)."""
+        if G.dtype != torch.float32 or G.numel() != n ** 3 or not G.is_contiguous():
+            raise ValueError("G must be a contiguous FP32 n^3 mesh")
+        for H in (HA,) if HB is None else (HA, HB):
+            if H.dtype != torch.complex64 or H.dim() != 5 or tuple(H.shape[:2]) != (n, n // 2):
+                raise ValueError("half shells have the wrong layout")
+        if HB is not None and HB.shape[3] != HA.shape[3]:
+            raise ValueError("HA and HB come from different k_max")
+        if self.synth_radius(boxsize, kmax) + 1 != HA.shape[3]:
+            raise ValueError("kmax does not match the half shells")
+        nb_ = na if HB is None else nb
+        C = torch.empty((na, nb_), dtype=torch.float64, device=self.device)
+        check(self.lib.cnpc_contract_zgemm(self.h, n, float(boxsize), float(kmax), _ptr(G), _ptr(HA), int(na),
+                                           HA.shape[2], _ptr(HB), int(nb_),
+                                           HA.shape[2] if HB is None else HB.shape[2], int(nblk), _ptr(C)),
+              "cnpc_contract_zgemm")
+        return C
 
     def shell_filter(self, delta, n, boxsize, prec, edges, bin0, nshell, poly, out):
         """``delta`` None filters the weight alone."""

@@ -10,6 +10,10 @@ GPU-only options (read from ``stat_attrs``; the reference ignores them):
   precision   : "fp64" (default) | "fp32"  -- carried by the painted fields
   contraction : "gemm_fp64" (fp64 default) | "pairwise" | "bf16x3"
                 (fp32 default) | "fp16x3"
+  shell_method: "synth" (default, pruned shell synthesis) | "filter"
+                (shell filter + rocFFT 3-D c2r)
+  zsynth      : "fused" (default: with bf16x3 the shells' z pass runs
+                inside the contraction) | "separate" (real shells in HBM)
 """
 import logging
 import math
@@ -116,6 +120,20 @@ class ShellBuilder:
             done += c
 
 
+    def build_half(self, delta, poly, bins, out, slot0):
+        """x and y passes of the shells of ``bins`` (consecutive) into slots
+        slot0 .. of the half-synthesised buffer ``out`` (FP32)."""
+        b0 = bins[0]
+        nb = len(bins)
+        assert list(bins) == list(range(b0, b0 + nb))
+        done = 0
+        while done < nb:
+            c = min(self.batch, nb - done)
+            self.ctx.shell_synth_half(delta, self.n, self.L, self.edges, b0 + done, c, poly, self.work, out,
+                                      slot0 + done)
+            done += c
+
+
 def _gemm_mode(contraction, prec):
     if contraction in ("bf16x3", "fp16x3"):
         if prec != PREC["fp32"]:
@@ -170,7 +188,7 @@ def _weight_components(ell, m, conj_w):
 
 
 def _config_matrix(ctx, builder, src1, src2, Gs, kinfo, ell_1, m1, ell_2, m2, sym_eq, sym_cj, mode, block_size,
-                   dvm, conj_w=False):
+                   dvm, conj_w=False, zsynth="fused"):
     """3j-free signal matrix of one magnetic configuration,
     S_ij = sum_x G F_i F'_j, with the reference's upper-triangle evaluation and
     lower fill (transpose for F' = F, conjugate for F' = +-conj F;
@@ -179,7 +197,12 @@ def _config_matrix(ctx, builder, src1, src2, Gs, kinfo, ell_1, m1, ell_2, m2, sy
     F_i = c2r(w1(khat) src1 mask_i), F'_j = c2r(w2(khat) src2 mask_j) with
     w = Y_lm (or conj Y_lm), split into real shell components; src None is a
     unit spectrum.  ``Gs`` lists (real mesh, complex factor) pairs whose sum is
-    the (possibly complex) G."""
+    the (possibly complex) G.
+
+    With the split-bf16 GEMM and ``zsynth="fused"`` the shells stay half
+    synthesised (x and y passes) and their z pass runs inside the contraction
+    (``cnpc_contract_zgemm``) whenever a block of rows fits it; otherwise the
+    real shells are built and contracted separately."""
     n, L, prec, k_edge, k_bins = kinfo
     N3 = n ** 3
     use_sym = sym_eq or sym_cj
@@ -191,8 +214,17 @@ def _config_matrix(ctx, builder, src1, src2, Gs, kinfo, ell_1, m1, ell_2, m2, sy
     else:
         comps2 = _weight_components(ell_2, m2, conj_w)
     nc1, nc2 = len(comps1), len(comps2)
+    kmax = float(k_edge[-1])
+    fused = zsynth == "fused" and mode == "bf16x3" and builder.method == "synth"
+    if fused:
+        cap = min(k_bins, 32 // max(nc1, nc2))
+        fused = cap >= 1 and ctx.zgemm_supported(n, nc1 * cap, nc2 * cap, False)
+    if fused:
+        R = ctx.synth_radius(L, kmax)
+        shell_bytes = n * n * (R + 1) * 8  # one slot of the half-synthesised layout
+    else:
+        shell_bytes = N3 * (8 if prec == 0 else 4)
     # resident shells per block: nc1 * bs (+ nc2 * bs off the diagonal)
-    shell_bytes = N3 * (8 if prec == 0 else 4)
     bs_fit = _fit_block(k_bins, shell_bytes * (nc1 + nc2), ctx)
     if use_sym and _fit_block(k_bins, shell_bytes * nc1, ctx) == k_bins:
         bs_fit = k_bins  # one diagonal block: leg 2 reuses leg 1's shells
@@ -202,23 +234,32 @@ def _config_matrix(ctx, builder, src1, src2, Gs, kinfo, ell_1, m1, ell_2, m2, sy
     else:
         sub = np.zeros(k_bins, dtype=np.complex128)
         bs = bs_fit
+    if fused:
+        bs = min(bs, cap)
+
+    def shells(src, comps, bins):
+        rows = len(comps) * len(bins)
+        if fused:
+            H = ctx.half_shells(n, R, rows)
+            for c, (poly, _) in enumerate(comps):
+                builder.build_half(src, poly, bins, H, c * len(bins))
+            return H
+        out = ctx.mesh(n, prec, rows).view(rows, N3)
+        for c, (poly, _) in enumerate(comps):
+            builder.build(src, poly, bins, out[c * len(bins):(c + 1) * len(bins)])
+        return out
+
     for bi in range(0, k_bins, bs):
         ib = list(range(bi, min(k_bins, bi + bs)))
-        R1 = ctx.mesh(n, prec, nc1 * len(ib)).view(nc1 * len(ib), N3)
-        for c, (poly, _) in enumerate(comps1):
-            builder.build(src1, poly, ib, R1[c * len(ib):(c + 1) * len(ib)])
+        R1 = shells(src1, comps1, ib)
         jstarts = [bi] if dvm == "diagonal" else list(range(bi if use_sym else 0, k_bins, bs))
         for bj in jstarts:
             jb = list(range(bj, min(k_bins, bj + bs)))
             if bj == bi and use_sym:
                 R2 = None
             else:
-                R2 = ctx.mesh(n, prec, nc2 * len(jb)).view(nc2 * len(jb), N3)
-                for c, (poly, _) in enumerate(comps2):
-                    if use_sym:
-                        builder.build(src1, comps1[c][0], jb, R2[c * len(jb):(c + 1) * len(jb)])
-                    else:
-                        builder.build(src2, poly, jb, R2[c * len(jb):(c + 1) * len(jb)])
+                # off the diagonal the symmetric case rebuilds leg 1's shells
+                R2 = shells(src1, comps1, jb) if use_sym else shells(src2, comps2, jb)
             # pairs this block contributes (reference loop order)
             want = []
             for i in ib:
@@ -235,7 +276,11 @@ def _config_matrix(ctx, builder, src1, src2, Gs, kinfo, ell_1, m1, ell_2, m2, sy
                      for c1 in range(nc1) for c2 in range(nc2)]
             C = 0j
             for G, gph in Gs:
-                C = C + gph * _contract(ctx, mode, prec, G, R1, R2, pairs)
+                if fused:
+                    Cm = ctx.contract_zgemm(n, L, kmax, G, R1, nc1 * len(ib), R2, nc2 * rows2).cpu().numpy()
+                else:
+                    Cm = _contract(ctx, mode, prec, G, R1, R2, pairs)
+                C = C + gph * Cm
             for (i, j) in want:
                 s = 0j
                 for c1 in range(nc1):
@@ -254,7 +299,7 @@ def _config_matrix(ctx, builder, src1, src2, Gs, kinfo, ell_1, m1, ell_2, m2, sy
 
 
 def _bk_signal(ctx, src1, src2, G, kinfo, mcfg, tjv, ell_1, ell_2, sym_allowed, mode, block_size, dvm,
-               conj_w=False, shell_method="synth"):
+               conj_w=False, shell_method="synth", zsynth="fused"):
     """Sum over the box magnetic configs of 3j * sum_x G F_i F'_j with the
     reference's symmetry fill and 2 Re accumulation
     (``clustering_estimator.py:867-1061``; with ``conj_w`` and null sources,
@@ -271,7 +316,7 @@ def _bk_signal(ctx, src1, src2, G, kinfo, mcfg, tjv, ell_1, ell_2, sym_allowed, 
         sym_eq = sym_allowed and same_ell and m1 == m2
         sym_cj = sym_allowed and same_ell and m1 == -m2
         sub = tjv[ss] * _config_matrix(ctx, builder, src1, src2, Gs, kinfo, ell_1, m1, ell_2, m2, sym_eq, sym_cj,
-                                       mode, block_size, dvm, conj_w)
+                                       mode, block_size, dvm, conj_w, zsynth)
         total += sub if (m1, m2, M) == (0, 0, 0) else 2 * np.real(sub)
     return total
 
@@ -332,7 +377,7 @@ def bk_box_g00(ctx, delta, n, prec):
 
 
 def bk_box_signal_auto(ctx, delta, G, n, boxsize, prec, k_edge, angu_config, contraction, dvm="full",
-                       block_size="full"):
+                       block_size="full", zsynth="fused"):
     """Signal stage of an auto-correlation box bispectrum: from the
     compensated delta_k and G_00 resident in HBM to ``total_res``
     (``clustering_estimator.py:867-1061``) for one angular configuration."""
@@ -340,7 +385,7 @@ def bk_box_signal_auto(ctx, delta, G, n, boxsize, prec, k_edge, angu_config, con
     mcfg, tjv = A.get_magnetic_configs_box(ell_1, ell_2, L)
     k_bins = len(k_edge) - 1
     return _bk_signal(ctx, delta, delta, G, (n, boxsize, prec, np.asarray(k_edge), k_bins), mcfg, tjv, ell_1,
-                      ell_2, True, _gemm_mode(contraction, prec), block_size, dvm)
+                      ell_2, True, _gemm_mode(contraction, prec), block_size, dvm, zsynth=zsynth)
 
 
 def calculate_bk_sugi_box(rfield_a, rfield_b, rfield_c, correlation_mode, stat_attrs, comm=None, **kwargs):
@@ -361,6 +406,7 @@ def calculate_bk_sugi_box(rfield_a, rfield_b, rfield_c, correlation_mode, stat_a
     n, Lbox, prec = rfield_a.n, rfield_a.boxsize, rfield_a.prec
     contraction = _gemm_mode(stat_attrs.get("contraction", "gemm_fp64" if prec == 0 else "bf16x3"), prec)
     shell_method = stat_attrs.get("shell_method", "synth")
+    zsynth = stat_attrs.get("zsynth", "fused")
 
     N_ells = (2 * ell_1 + 1) * (2 * ell_2 + 1) * (2 * L + 1)
     H_ells = A.wigner_3j(ell_1, ell_2, L, 0, 0, 0)
@@ -407,7 +453,7 @@ def calculate_bk_sugi_box(rfield_a, rfield_b, rfield_c, correlation_mode, stat_a
     G = bk_box_g00(ctx, gsrc_m, m, prec)
     kinfo = (m, Lbox, prec, k_edge, k_bins)
     total_res = grid_factor * _bk_signal(ctx, ca_m, src2_m, G, kinfo, mcfg, tjv, ell_1, ell_2, same_tracer,
-                                         contraction, block_size, dvm, shell_method=shell_method)
+                                         contraction, block_size, dvm, shell_method=shell_method, zsynth=zsynth)
     t2 = _sync_time(ctx)
     LAST_TIMINGS.update({"prepare_s": t1 - t0, "signal_s": t2 - t1, "contraction": contraction, "grid": m})
     del G, ca_m, src2_m, gsrc_m
@@ -497,7 +543,7 @@ def calculate_bk_sugi_box(rfield_a, rfield_b, rfield_c, correlation_mode, stat_a
             del spec
             tot = grid_factor * _bk_signal(ctx, None, None, S.view(-1), kinfo, mcfg, tjv, ell_1, ell_2, True,
                                            contraction, block_size, dvm, conj_w=True,
-                                           shell_method=shell_method)
+                                           shell_method=shell_method, zsynth=zsynth)
             del S
             kk = k_num.astype(np.float64)
             with np.errstate(invalid="ignore", divide="ignore"):

@@ -484,6 +484,7 @@ def calculate_bk_sugi_survey(rfield_a, rfield_b, rfield_c, correlation_mode, sta
     vpc = V / float(n) ** 3
     contraction = _gemm_mode(stat_attrs.get("contraction", "gemm_fp64" if prec == 0 else "bf16x3"), prec)
     shell_method = stat_attrs.get("shell_method", "synth")
+    zsynth = stat_attrs.get("zsynth", "fused")
     N_ells = (2 * ell_1 + 1) * (2 * ell_2 + 1) * (2 * L + 1)
     H_ells = A.wigner_3j(ell_1, ell_2, L, 0, 0, 0)
     mcfg, tjv = A.get_magnetic_configs_survey(ell_1, ell_2, L)
@@ -551,7 +552,7 @@ def calculate_bk_sugi_survey(rfield_a, rfield_b, rfield_c, correlation_mode, sta
             sym_eq = sym_tracer and same_ell and m1 == m2
             sym_cj = sym_tracer and same_ell and m1 == -m2
             sub = grid_factor * _config_matrix(ctx, builder, src1, src2, Gs, kinfo, ell_1, m1, ell_2, m2, sym_eq,
-                                               sym_cj, contraction, block_size, dvm, conj_w)
+                                               sym_cj, contraction, block_size, dvm, conj_w, zsynth)
             sub = post(tjv[ss] * sub)
             swap = (m2, m1, M)
             has_swap = sym_tracer and same_ell and swap in cfg_idx and swap != (m1, m2, M)

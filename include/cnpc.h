@@ -61,8 +61,11 @@ int cnpc_ctx_reserve(cnpc_ctx* ctx, int64_t scratch_bytes);
 int cnpc_ctx_clear_plans(cnpc_ctx* ctx);
 
 /* Paint weighted particles onto mesh0 (and the +0.5-cell shifted copy onto
- * mesh1 when interlaced).  Zero-fills the meshes first.  window: 1 ngp,
- * 2 cic, 3 tsc, 4 pcs.  boxcenter may be NULL.
+ * mesh1 when interlaced); every cell of the meshes is overwritten.  window:
+ * 1 ngp, 2 cic, 3 tsc, 4 pcs.  boxcenter may be NULL.  For nmesh % 8 == 0
+ * (16..2048) the particles are sorted by 8^3 tile and each tile is summed in
+ * LDS and written once (no global atomics; work memory grows with np in
+ * the context); other meshes use per-particle atomic scatter.
  * Replaces pm.paint in pm_painter (mesh_generator.py:194-205). */
 int cnpc_paint(cnpc_ctx* ctx, const double* pos_xyz, const double* weight, int64_t np, int nmesh, double boxsize,
                int window, int interlaced, const double* boxcenter, int prec, void* mesh0, void* mesh1);
@@ -129,13 +132,24 @@ int cnpc_shell_filter(cnpc_ctx* ctx, int nmesh, double boxsize, int prec, const 
 
 /* Pruned shell synthesis: out[s] = c2r(w(khat) delta(k) 1[k in bin bin0+s])
  * for s < nshell, the same real shells as cnpc_shell_filter followed by
- * cnpc_fft_c2r, computed as a sparse x-direction DFT over the modes inside
- * |n_d| <= R = ceil(edges[nbins] L / 2 pi) followed by a batched 2-D c2r
- * over (y, z).  work: nshell*n*n*(n/2+1) complex; out: nshell*n^3 real;
- * delta_k may be NULL (weight alone); nbins <= 127.  Replaces the shell IFFTs
- * c2r(ylm_weighted_cfield * mask_i) of clustering_estimator.py:934-970. */
+ * cnpc_fft_c2r, computed by x, y and z FFT passes restricted to the modes
+ * inside |n_d| <= R = ceil(edges[nbins] L / 2 pi).  work:
+ * nshell*n*n*(n/2+1) complex; out: nshell*n^3 real; delta_k may be NULL
+ * (weight alone); nbins <= 127; n must factor into 2, 3 and 5.  Replaces the
+ * shell IFFTs c2r(ylm_weighted_cfield * mask_i) of
+ * clustering_estimator.py:934-970. */
 int cnpc_shell_synth(cnpc_ctx* ctx, int nmesh, double boxsize, int prec, const void* delta_k, const double* edges,
                      int nbins, int bin0, int nshell, const cnpc_poly* weight, void* work, void* out);
+
+/* The x and y passes of cnpc_shell_synth (FP32 only): shells bin0 ..
+ * bin0+nshell-1 into slots slot0 .. slot0+nshell-1 of the half-synthesised
+ * layout out[x][y/2][nslots][R+1][y%2] (complex64, n*(n/2)*nslots*(R+1)*2
+ * entries), the input of cnpc_contract_zgemm.  work: nshell half spectra.
+ * n must be a multiple of 8.  Same shells as clustering_estimator.py:934-970
+ * before their z transform. */
+int cnpc_shell_synth_half(cnpc_ctx* ctx, int nmesh, double boxsize, const void* delta_k, const double* edges,
+                          int nbins, int bin0, int nshell, const cnpc_poly* weight, void* work, void* out,
+                          int nslots, int slot0);
 
 /* Pairwise plain contraction: out[p] = sum_x G A_{ia[p]} B_{ib[p]}, one full
  * streaming pass per pair (the np.sum(G_00 * F_i * F_j) of
@@ -152,6 +166,21 @@ int cnpc_contract_pairs(cnpc_ctx* ctx, int prec, int64_t ncell, const void* G, c
  * loop of clustering_estimator.py:911-981. */
 int cnpc_contract_gemm(cnpc_ctx* ctx, int mode, int64_t ncell, const void* G, const void* A, int na, const void* B,
                        int nb, int same, float sa, float sb, int nblk, double* C);
+
+/* Fused z synthesis + split-bf16 contraction: C[a][b] = sum_x A_a G B_b
+ * (device FP64 C, row-major na x nb) where A_a, B_b are the shells held in
+ * slots 0 .. na-1 of HA and 0 .. nb-1 of HB in the layout of
+ * cnpc_shell_synth_half (sa_slots / sb_slots slots per tile; kmax =
+ * edges[nbins] of that call), G a real FP32 n^3 mesh.  HB NULL (or HA)
+ * means B == A.  na, nb <= 32.  Equals cnpc_shell_synth followed by
+ * cnpc_contract_gemm mode 0 without the real shells in HBM; the block loop
+ * of clustering_estimator.py:911-981. */
+int cnpc_contract_zgemm(cnpc_ctx* ctx, int nmesh, double boxsize, double kmax, const float* G, const void* HA, int na,
+                        int sa_slots, const void* HB, int nb, int sb_slots, int nblk, double* C);
+
+/* *ok = 1 when cnpc_contract_zgemm accepts this grid and these row counts
+ * on the current device (LDS line buffers and FFT staging), else 0. */
+int cnpc_contract_zgemm_supported(cnpc_ctx* ctx, int nmesh, int na, int nb, int same, int* ok);
 
 /* Max |x| over n values (device FP32 in, host double out; synchronises).
  * Used to pick the power-of-two prescales of the fp16 split. */

@@ -17,8 +17,8 @@ def _cfgs(out):
                 apply_rsd=False, nmesh=[32] * 3, boxsize=[500.0] * 3, sampler="cic", interlaced=True,
                 compensation=True, cosmology={"h": 0.6766, "Omega0": 0.30966}, redshift_box=0.5,
                 output_dir=str(out))
-    pk = dict(base, statistic="pk", k_min=0.01, k_max=0.3, k_bins=10, poles=[0, 2, 4])
-    bk = dict(base, statistic="bk_sugi", k_min=0.02, k_max=0.14, k_bins=5, angu_config=[2, 0, 2],
+    pk = dict(base, statistic="pk", k_min=0.01, k_max=0.19, k_bins=9, poles=[0, 2, 4])
+    bk = dict(base, statistic="bk_sugi", k_min=0.02, k_max=0.1, k_bins=4, angu_config=[2, 0, 2],
               tracer_type="aaa", data_vector_mode="full", shotnoise_mode="both")
     return pk, bk
 
@@ -30,6 +30,29 @@ def _arrays(tmp_path):
         pos = catalog(max(npart, 1), 500.0, 20 + i, ngrid=32)[:npart]
         arrays[path] = (pos, weights(npart, seed=30 + i))
     return arrays
+
+
+def _assert_close(got, want, what):
+    """Result entries agree: floating arrays to 1e-10 of their max |.|,
+    floats to 1e-12, containers recursively, everything else exactly."""
+    if isinstance(want, dict):
+        assert sorted(got) == sorted(want), what
+        for k in want:
+            _assert_close(got[k], want[k], f"{what}.{k}")
+    elif isinstance(want, (list, tuple)):
+        assert len(got) == len(want), what
+        for i, (g, w) in enumerate(zip(got, want)):
+            _assert_close(g, w, f"{what}[{i}]")
+    elif isinstance(want, np.ndarray) and want.dtype.kind in "fc":
+        assert got.shape == want.shape, what
+        if np.any(want != 0):
+            assert rel_to_max(got, want) < 1e-10, what
+        else:
+            np.testing.assert_array_equal(got, want, err_msg=what)
+    elif isinstance(want, (float, np.floating)):
+        assert got == pytest.approx(want, rel=1e-12, nan_ok=True), what
+    else:
+        np.testing.assert_equal(got, want, err_msg=what)
 
 
 def test_run_mocks_matches_run_stats(gpu_ctx, tmp_path):
@@ -49,14 +72,8 @@ def test_run_mocks_matches_run_stats(gpu_ctx, tmp_path):
             ref = run_stats(cfg)
             assert sorted(ref) == sorted(res)
             for key, val in ref.items():
-                if isinstance(val, np.ndarray) and val.dtype.kind in "fc" and val.size and np.any(val != 0):
-                    assert rel_to_max(res[key], val) < 1e-10, (path, cfg["statistic"], key)
-                elif isinstance(val, np.ndarray):
-                    np.testing.assert_array_equal(res[key], val)
-                elif isinstance(val, float):
-                    assert res[key] == pytest.approx(val, rel=1e-12), key
-                else:
-                    assert res[key] == val, key
+                if key != "output_dir":
+                    _assert_close(res[key], val, f"{os.path.basename(path)} {cfg['statistic']} {key}")
     names = sorted(os.listdir(tmp_path / "batched"))
     assert names == sorted(os.listdir(tmp_path / "single"))
     again = run_mocks([pk, bk], paths, batch=2, arrays=arrays)

@@ -27,12 +27,14 @@ def _log(msg):
 
 @pytest.mark.parametrize("sampler", WINDOWS)
 @pytest.mark.parametrize("prec", [0, 1])
-def test_paint_matches_oracle(gpu_ctx, sampler, prec):
+@pytest.mark.parametrize("n", [32, 36])
+def test_paint_matches_oracle(gpu_ctx, sampler, prec, n):
+    """n = 32: sorted LDS-tiled paint; n = 36: scatter (atomic) paint."""
     import torch
 
     C, _, _ = _oracle()
-    _log(f"paint {sampler} prec={prec}")
-    n, L = 32, 500.0
+    _log(f"paint {sampler} prec={prec} n={n}")
+    L = 500.0
     pos = catalog(5000, L, 3, ngrid=32)
     w = weights(len(pos), seed=4)
     ctx = gpu_ctx
@@ -61,6 +63,32 @@ def test_paint_wraps_and_empty(gpu_ctx):
     e = torch.zeros((0, 3), dtype=torch.float64, device="cuda")
     ctx.paint(e, torch.zeros(0, dtype=torch.float64, device="cuda"), n, L, "cic", False, 0, m0)
     assert m0.abs().max().item() == 0.0
+
+
+@pytest.mark.parametrize("sampler", WINDOWS)
+def test_paint_tiled_matches_scatter(gpu_ctx, sampler):
+    """Tiled paint (cnpc_paint at n % 8 == 0) == scatter paint (the batch
+    entry with one catalog) on a dense clump inside one tile, particles on
+    tile and box edges and a uniform background; FP64 to 1e-13 of max."""
+    import torch
+
+    n, L = 64, 640.0
+    rng = np.random.default_rng(5)
+    clump = 200.0 + rng.random((3000, 3)) * 4.0           # one 8^3 tile (80 Mpc/h)
+    edges = np.array([[0.0, 0.0, 0.0], [639.999, 639.999, 639.999], [80.0, 160.0, 239.99], [79.999, 80.0, 0.0]])
+    pos = np.concatenate([clump, edges, rng.random((20000, 3)) * L])
+    w = 0.5 + rng.random(len(pos))
+    ctx = gpu_ctx
+    pd, wd = torch.from_numpy(pos).cuda(), torch.from_numpy(w).cuda()
+    for prec in (0, 1):
+        a0, a1 = ctx.mesh(n, prec), ctx.mesh(n, prec)
+        ctx.paint(pd, wd, n, L, sampler, True, prec, a0, a1)
+        b0, b1 = ctx.mesh(n, prec, 1), ctx.mesh(n, prec, 1)
+        ctx.paint_batch(pd, wd, [0, len(pos)], n, L, sampler, True, prec, b0, b1)
+        tol = 1e-13 if prec == 0 else 2e-6
+        for a, b in ((a0, b0[0]), (a1, b1[0])):
+            a, b = a.double().cpu().numpy(), b.double().cpu().numpy()
+            assert np.max(np.abs(a - b)) <= tol * np.max(np.abs(b)), (sampler, prec)
 
 
 def test_fft_roundtrip_and_normalisation(gpu_ctx):
@@ -126,7 +154,7 @@ def test_pk_box_cross_matches_oracle(gpu_ctx):
 
 
 def _bk_case(gpu_ctx, angu, dvm, contraction, prec=0, sampler="tsc", interlaced=True, n=32, L=300.0,
-             k=(0.03, 0.15, 6), npart=30000, seed=11, block_size=None, shell_method="synth"):
+             k=(0.03, 0.15, 6), npart=30000, seed=11, block_size=None, shell_method="synth", zsynth="fused"):
     # run_task requires 2 k_max < k_Nyq (task_executor.py:192-196); the
     # half-spectrum path relies on it, so every case respects it
     assert 2 * k[1] < np.pi * n / L
@@ -134,11 +162,11 @@ def _bk_case(gpu_ctx, angu, dvm, contraction, prec=0, sampler="tsc", interlaced=
     from cosmonpc_amd.mesh import paint_field
 
     _, _, B = _oracle()
-    _log(f"bk {angu} {dvm} {contraction} prec={prec} {sampler} n={n} {shell_method}")
+    _log(f"bk {angu} {dvm} {contraction} prec={prec} {sampler} n={n} {shell_method} {zsynth}")
     pos = catalog(npart, L, seed, ngrid=n)
     w = np.ones(len(pos))
     attrs, fa, _, _ = B.get_mesh_box({"a": (pos, w)}, [n] * 3, [L] * 3, sampler, interlaced)
-    extra = {"shell_method": shell_method}
+    extra = {"shell_method": shell_method, "zsynth": zsynth}
     if block_size is not None:
         extra["block_size"] = block_size
     sa = dict(attrs)
@@ -249,6 +277,80 @@ def test_shell_synth_matches_filter_c2r(gpu_ctx, prec, n, L, kmax):
             ah, bh = a.cpu().numpy(), b.cpu().numpy()
             tol = 1e-12 if prec == 0 else 2e-5
             assert np.max(np.abs(ah - bh)) <= tol * np.max(np.abs(bh)), (ell, m, src is None)
+
+
+@pytest.mark.parametrize("n,L,kmax", [(40, 500.0, 0.12), (48, 600.0, 0.13), (80, 1000.0, 0.15)])
+def test_zgemm_fused_matches_fp64_contraction(gpu_ctx, n, L, kmax):
+    """Fused z synthesis + split-bf16 GEMM (half-synthesised shells in slots)
+    against the FP64 real shells contracted by the FP64 GEMM: B == A with an
+    odd row count (a lone last line), B != A with different weights and row
+    counts, slots beyond the rows, a null spectrum.  The bar is the accuracy
+    of the unfused split-bf16 path on the same inputs (FP32 real shells, the
+    GEMM on them): the fused error must stay within twice it (or 1e-6 of
+    max |C|); an indexing mistake gives O(1) errors."""
+    import torch
+
+    from cosmonpc_amd import angular as A
+
+    ctx = gpu_ctx
+    rng = np.random.default_rng(n + 7)
+    x64 = torch.from_numpy(rng.standard_normal((n, n, n))).cuda()
+    D64, D32 = ctx.spectrum(n, 0), ctx.spectrum(n, 1)
+    ctx.r2c(x64, D64, n, 0)
+    ctx.r2c(x64.float().contiguous(), D32, n, 1)
+    G64 = torch.from_numpy(1.0 + 0.5 * rng.standard_normal(n ** 3)).cuda()
+    G32 = G64.float().contiguous()
+    edges = np.linspace(0.01, kmax, 8)
+    nbins = 7
+    R = ctx.synth_radius(L, kmax)
+    polys = {0: A.shell_components(0, 0)[0][0], 2: A.shell_components(2, 0)[0][0]}
+
+    def real64(src, poly, rows):
+        out = ctx.mesh(n, 0, rows)
+        ctx.shell_synth(src, n, L, 0, edges, 0, rows, poly, ctx.spectrum(n, 0, rows), out)
+        return out.view(rows, n ** 3)
+
+    def real32(src, poly, rows):
+        out = ctx.mesh(n, 1, rows)
+        ctx.shell_synth(src, n, L, 1, edges, 0, rows, poly, ctx.spectrum(n, 1, rows), out)
+        return out.view(rows, n ** 3)
+
+    def half32(src, poly, rows, nslots):
+        H = ctx.half_shells(n, R, nslots)
+        work = ctx.spectrum(n, 1, 4)
+        for b0 in range(0, rows, 4):
+            c = min(4, rows - b0)
+            ctx.shell_synth_half(src, n, L, edges, b0, c, poly, work, H, b0)
+        return H
+
+    cases = [("same odd rows", 0, 7, None, None, 9), ("A != B", 2, 6, 0, 5, 6), ("null spectrum", 0, 5, None, None, 5)]
+    for what, la, na, lb, nb, nslots in cases:
+        src64, src32 = (None, None) if what == "null spectrum" else (D64, D32)
+        A64, A32 = real64(src64, polys[la], na), real32(src32, polys[la], na)
+        HA = half32(src32, polys[la], na, nslots)
+        if lb is None:
+            ref = ctx.contract_gemm("gemm_fp64", G64, A64).cpu().numpy()
+            sep = ctx.contract_gemm("bf16x3", G32, A32).cpu().numpy()
+            got = ctx.contract_zgemm(n, L, kmax, G32, HA, na).cpu().numpy()
+        else:
+            B64, B32 = real64(src64, polys[lb], nb), real32(src32, polys[lb], nb)
+            HB = half32(src32, polys[lb], nb, nb + 1)
+            ref = ctx.contract_gemm("gemm_fp64", G64, A64, B64).cpu().numpy()
+            sep = ctx.contract_gemm("bf16x3", G32, A32, B32).cpu().numpy()
+            got = ctx.contract_zgemm(n, L, kmax, G32, HA, na, HB, nb).cpu().numpy()
+        scale = np.max(np.abs(ref))
+        err, err_sep = np.max(np.abs(got - ref)) / scale, np.max(np.abs(sep - ref)) / scale
+        _log(f"zgemm n={n} {what}: rel err fused {err:.2e}, separate {err_sep:.2e}")
+        assert err <= max(2 * err_sep, 1e-6), (what, err, err_sep)
+
+
+def test_bk_box_fused_and_separate_z_agree(gpu_ctx):
+    """The estimator with the fused z pass equals the one with real shells in
+    HBM (both split-bf16) to 1e-6 of max|B| and the FP64 oracle to 1e-4."""
+    got_f, ref = _bk_case(gpu_ctx, (2, 0, 2), "full", "bf16x3", prec=1)
+    got_s, _ = _bk_case(gpu_ctx, (2, 0, 2), "full", "bf16x3", prec=1, zsynth="separate")
+    assert rel_to_max(got_f["Bk_raw"], got_s["Bk_raw"]) < 1e-6
+    assert rel_to_max(got_f["Bk_raw"], ref["Bk_raw"]) < 1e-4
 
 
 def test_bk_box_filter_method_matches_oracle(gpu_ctx):
