@@ -234,12 +234,13 @@ def kernel_roofline(ctx, delta, n, L, k_edge, torch, np, A):
 
     # production path: x and y synthesis passes into the half-synthesised
     # layout, then the z pass fused into the split-bf16 GEMM
-    t_half = []
+    t_half, kz = [], np.zeros(K, dtype=np.int32)
     for b0 in range(0, K, batch):
         c = min(batch, K - b0)
         t_half.append((timed(lambda: ctx.shell_synth_half(delta, n, L, k_edge, b0, c, poly, work, H, b0)), c))
-    ctx.contract_zgemm(n, L, kmax, G, H, K)
-    t_zgemm = timed(lambda: ctx.contract_zgemm(n, L, kmax, G, H, K), 3)
+        kz[b0:b0 + c] = ctx.shell_synth_half(delta, n, L, k_edge, b0, c, poly, work, H, b0)
+    ctx.contract_zgemm(n, L, kmax, G, H, K, kz)
+    t_zgemm = timed(lambda: ctx.contract_zgemm(n, L, kmax, G, H, K, kz), 3)
     # the separate path: real shells (cube, x, y and z FFT kernels) + GEMM
     t_synth = []
     for b0 in range(0, K, batch):
@@ -263,8 +264,9 @@ def kernel_roofline(ctx, delta, n, L, k_edge, torch, np, A):
     # spectra read + meshes written (c2r); delta read + spectra written
     # (filter); every shell and G read once (GEMM)
     b_synth = batch * N3 * 4
-    b_half = batch * n * n * (R + 1) * 8
-    b_zgemm = K * n * n * (R + 1) * 8 + N3 * 4
+    # written / read kz rows only (each shell holds kz < its own radius)
+    b_half = int(np.mean([kz[b0:b0 + batch].sum() for b0 in range(0, K - batch + 1, batch)])) * n * n * 8
+    b_zgemm = int(kz.sum()) * n * n * 8 + N3 * 4
     b_c2r = batch * (half * 8 + N3 * 4)
     b_filter = half * 8 + batch * half * 8
     b_gemm = (K + 1) * N3 * 4
@@ -384,12 +386,13 @@ def bench_kernels(args):
     del shells
     R = ctx.synth_radius(L, kmax)
     H = ctx.half_shells(m, R, K)
-    out["shell_synth_half_batch4"] = (timed(lambda: ctx.shell_synth_half(dm, m, L, k_edge, 8, 4, poly, work, H, 8)),
-                                      4 * m * m * (R + 1) * 8)
+    kz = np.zeros(K, dtype=np.int32)
     for b0 in range(0, K, 4):
-        ctx.shell_synth_half(dm, m, L, k_edge, b0, min(4, K - b0), poly, work, H, b0)
-    out["zgemm_bf16x3"] = (timed(lambda: ctx.contract_zgemm(m, L, kmax, G, H, K)),
-                           K * m * m * (R + 1) * 8 + m ** 3 * 4)
+        kz[b0:b0 + 4] = ctx.shell_synth_half(dm, m, L, k_edge, b0, min(4, K - b0), poly, work, H, b0)
+    out["shell_synth_half_batch4"] = (timed(lambda: ctx.shell_synth_half(dm, m, L, k_edge, 8, 4, poly, work, H, 8)),
+                                      int(kz[8:12].sum()) * m * m * 8)
+    out["zgemm_bf16x3"] = (timed(lambda: ctx.contract_zgemm(m, L, kmax, G, H, K, kz)),
+                           int(kz.sum()) * m * m * 8 + m ** 3 * 4)
     del H
     src = torch.empty(1 << 28, dtype=torch.float32, device=dev)
     dst = torch.empty_like(src)

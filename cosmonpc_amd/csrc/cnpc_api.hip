@@ -372,6 +372,15 @@ static int factor_radices(int n, SynthArgs& a) {
 // every mode with |k| < kmax has |n_d| <= R
 static int synth_radius(double kmax, double boxsize) { return (int)std::ceil(kmax / (2.0 * M_PI / boxsize)); }
 
+// per-shell radii of a synthesis call: shell s holds |k| < edges[bin0+s+1],
+// so |n_d| <= ceil(edges / kf) (one more for rounding), at most R
+static void shell_radii(SynthArgs& a, const double* edges, double boxsize) {
+  for (int s = 0; s < a.nshell; ++s) {
+    const int r = synth_radius(edges[a.bin0 + s + 1], boxsize) + 1;
+    a.rs[s] = r < a.R ? r : a.R;
+  }
+}
+
 int cnpc_shell_synth(cnpc_ctx* ctx, int n, double boxsize, int prec, const void* D, const double* edges, int nbins,
                      int bin0, int nshell, const cnpc_poly* weight, void* work, void* out) {
   REQUIRE(ctx, "null context");
@@ -389,6 +398,8 @@ int cnpc_shell_synth(cnpc_ctx* ctx, int n, double boxsize, int prec, const void*
   SynthArgs a;
   std::memset(&a, 0, sizeof(a));
   a.n = n; a.R = R; a.bin0 = bin0; a.nshell = nshell; a.kf = kf;
+  REQUIRE(nshell <= CNPC_SYNTH_MAXS, "shell_synth: too many shells in one call");
+  shell_radii(a, edges, boxsize);
   REQUIRE(factor_radices(n, a), "shell_synth: nmesh must factor into 2, 3 and 5");
   REQUIRE(synth_lines_per_block(prec, a) >= 1, "shell_synth: mesh too large for the LDS line buffers");
   const size_t cs = prec == 0 ? 16 : 8;
@@ -410,7 +421,8 @@ int cnpc_shell_synth(cnpc_ctx* ctx, int n, double boxsize, int prec, const void*
 }
 
 int cnpc_shell_synth_half(cnpc_ctx* ctx, int n, double boxsize, const void* D, const double* edges, int nbins,
-                          int bin0, int nshell, const cnpc_poly* weight, void* work, void* out, int nslots, int slot0) {
+                          int bin0, int nshell, const cnpc_poly* weight, void* work, void* out, int nslots, int slot0,
+                          int* kz_rows) {
   REQUIRE(ctx, "null context");
   REQUIRE(n >= 8 && n % 8 == 0 && boxsize > 0, "shell_synth_half: nmesh must be a multiple of 8");
   REQUIRE(nbins >= 1 && nbins <= 127 && edges, "shell_synth_half: nbins must be in [1, 127]");
@@ -425,6 +437,10 @@ int cnpc_shell_synth_half(cnpc_ctx* ctx, int n, double boxsize, const void* D, c
   SynthArgs a;
   std::memset(&a, 0, sizeof(a));
   a.n = n; a.R = R; a.bin0 = bin0; a.nshell = nshell; a.kf = 2.0 * M_PI / boxsize;
+  REQUIRE(nshell <= CNPC_SYNTH_MAXS, "shell_synth_half: too many shells in one call");
+  shell_radii(a, edges, boxsize);
+  if (kz_rows)
+    for (int s = 0; s < nshell; ++s) kz_rows[s] = a.rs[s] + 1;
   REQUIRE(factor_radices(n, a), "shell_synth_half: nmesh must factor into 2, 3 and 5");
   REQUIRE(synth_lines_per_block(1, a) >= 1, "shell_synth_half: mesh too large for the LDS line buffers");
   const size_t P = 2 * (size_t)R + 1, Q = (size_t)R + 1, cube = P * Q * P;
@@ -443,7 +459,8 @@ int cnpc_shell_synth_half(cnpc_ctx* ctx, int n, double boxsize, const void* D, c
 }
 
 int cnpc_contract_zgemm(cnpc_ctx* ctx, int n, double boxsize, double kmax, const float* G, const void* HA, int na,
-                        int sa_slots, const void* HB, int nb, int sb_slots, int nblk, double* C) {
+                        int sa_slots, const int* kz_a, const void* HB, int nb, int sb_slots, const int* kz_b,
+                        int nblk, double* C) {
   REQUIRE(ctx, "null context");
   REQUIRE(n >= 8 && n % 8 == 0 && boxsize > 0 && kmax > 0, "contract_zgemm: nmesh must be a multiple of 8");
   REQUIRE(G && HA && C, "contract_zgemm: null buffers");
@@ -469,14 +486,23 @@ int cnpc_contract_zgemm(cnpc_ctx* ctx, int n, double boxsize, double kmax, const
   CHECK_HIP(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
   REQUIRE(zgemm_supported(a, na, nb, same, (size_t)lds_max),
           "contract_zgemm: grid too large for the LDS line buffers");
-  if (nblk <= 0) nblk = 256;
+  if (nblk <= 0) {  // one resident workgroup per CU
+    CHECK_HIP(hipDeviceGetAttribute(&nblk, hipDeviceAttributeMultiprocessorCount, dev));
+  }
   if (nblk > 4096) nblk = 4096;
   const size_t pbytes = (size_t)nblk * 1024 * 8;
   if (int rc = ensure_scratch(ctx, pbytes + (size_t)n * 8)) return rc;
   char* base = (char*)ctx->scratch;
   CHECK_HIP(launch_twiddles(n, base + pbytes, ctx->stream));
-  CHECK_HIP(launch_zgemm(a, R + 1, HA, na, sa_slots, same ? HA : HB, nb, sb_slots, same, G, base + pbytes, nblk,
-                         (double*)base, C, ctx->stream));
+  int qa[32], qb[32];
+  for (int s = 0; s < 32; ++s) {
+    qa[s] = (kz_a && s < na) ? kz_a[s] : R + 1;
+    qb[s] = (!same && kz_b && s < nb) ? kz_b[s] : (same ? qa[s] : R + 1);
+    REQUIRE(qa[s] >= 0 && qa[s] <= R + 1 && qb[s] >= 0 && qb[s] <= R + 1,
+            "contract_zgemm: kz rows per slot must be in [0, R + 1]");
+  }
+  CHECK_HIP(launch_zgemm(a, R + 1, HA, na, sa_slots, qa, same ? HA : HB, nb, sb_slots, qb, same, G, base + pbytes,
+                         nblk, (double*)base, C, ctx->stream));
   return 0;
 }
 

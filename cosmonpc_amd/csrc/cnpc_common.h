@@ -197,6 +197,8 @@ struct ShellArgs {
 
 #define CNPC_MAX_RADICES 24
 
+#define CNPC_SYNTH_MAXS 128
+
 struct SynthArgs {
   int n;        // grid
   int R;        // |n_d| <= R holds every shell mode
@@ -205,6 +207,7 @@ struct SynthArgs {
   double kf;
   int nrad;     // mixed-radix factorisation of n (radices 2, 3, 4, 5)
   int rad[CNPC_MAX_RADICES];
+  int rs[CNPC_SYNTH_MAXS];  // per shell of the call: its modes have |n_d| <= rs[s] <= R
 };
 
 // ---------------------------------------------------------------- survey (full grids)
@@ -280,9 +283,9 @@ hipError_t launch_synth_half(const SynthArgs& a, const BinEdges& E, const PolySp
 hipError_t launch_twiddles(int n, void* tw, hipStream_t st);
 size_t zgemm_lds_bytes(int n, int nlines);
 int zgemm_supported(const SynthArgs& a, int na, int nb, int same, size_t lds_limit);
-hipError_t launch_zgemm(const SynthArgs& a, int Q, const void* HA, int na, int sa_slots, const void* HB, int nb,
-                        int sb_slots, int same, const float* G, const void* tw, int nblk, double* partial, double* C,
-                        hipStream_t st);
+hipError_t launch_zgemm(const SynthArgs& a, int Q, const void* HA, int na, int sa_slots, const int* qa,
+                        const void* HB, int nb, int sb_slots, const int* qb, int same, const float* G, const void* tw,
+                        int nblk, double* partial, double* C, hipStream_t st);
 hipError_t launch_gemm_finalize(const double* partial, int nblk, int a0, int b0, int na, int nb, double inv_scale,
                                 double* C, int ldc, hipStream_t st);
 hipError_t launch_synth(int prec, const SynthArgs& a, const BinEdges& E, const PolySpec& W, const void* D, void* V,

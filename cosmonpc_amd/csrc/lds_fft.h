@@ -3,8 +3,8 @@
 //
 // The transforms are the unnormalised inverse e^{+i k.x} of the reference's
 // pmesh c2r.  Lines are padded to n+1 entries so that accesses striding over
-// lines spread over the LDS banks; twiddles come from one table
-// W[j] = e^{2 pi i j / n} in LDS.
+// lines spread over the LDS banks; the twiddles of all passes sit in one LDS
+// table of n - 1 entries (build_pass_twiddles).
 #pragma once
 #include "cnpc_common.h"
 
@@ -28,40 +28,114 @@ __device__ __forceinline__ int fdiv(int a, int b, float inv_b) {
 }
 
 // ------------------------------------------------------------------ LDS FFT
-// cos / sin of 2 pi j / R for the small DFTs
-template <int RR> struct Roots;
-template <> struct Roots<2> {
-  static constexpr double c[2] = {1.0, -1.0};
-  static constexpr double s[2] = {0.0, 0.0};
+// R-point inverse DFTs v[b] = sum_q u[q] e^{+2 pi i q b / R} in registers,
+// written out for R = 2, 3, 4, 5 with the fewest multiplications
+template <typename C>
+__device__ __forceinline__ C cadd(C a, C b) {
+  C r;
+  r.x = a.x + b.x;
+  r.y = a.y + b.y;
+  return r;
+}
+template <typename C>
+__device__ __forceinline__ C csub(C a, C b) {
+  C r;
+  r.x = a.x - b.x;
+  r.y = a.y - b.y;
+  return r;
+}
+// a + i s d  and  a - i s d
+template <typename C, typename T>
+__device__ __forceinline__ void pm_i(C a, C d, T s, C& plus, C& minus) {
+  plus.x = a.x - s * d.y;
+  plus.y = a.y + s * d.x;
+  minus.x = a.x + s * d.y;
+  minus.y = a.y - s * d.x;
+}
+
+template <int RR> struct SmallDft;
+template <> struct SmallDft<2> {
+  template <typename C> __device__ __forceinline__ static void run(C (&u)[2]) {
+    const C a = u[0], b = u[1];
+    u[0] = cadd(a, b);
+    u[1] = csub(a, b);
+  }
 };
-template <> struct Roots<3> {
-  static constexpr double c[3] = {1.0, -0.5, -0.5};
-  static constexpr double s[3] = {0.0, 0.86602540378443865, -0.86602540378443865};
+template <> struct SmallDft<3> {
+  template <typename C> __device__ __forceinline__ static void run(C (&u)[3]) {
+    typedef decltype(C().x) T;
+    const T s = (T)0.86602540378443865;
+    const C t = cadd(u[1], u[2]), d = csub(u[1], u[2]);
+    C m;
+    m.x = u[0].x - (T)0.5 * t.x;
+    m.y = u[0].y - (T)0.5 * t.y;
+    u[0] = cadd(u[0], t);
+    pm_i(m, d, s, u[1], u[2]);
+  }
 };
-template <> struct Roots<4> {
-  static constexpr double c[4] = {1.0, 0.0, -1.0, 0.0};
-  static constexpr double s[4] = {0.0, 1.0, 0.0, -1.0};
+template <> struct SmallDft<4> {
+  template <typename C> __device__ __forceinline__ static void run(C (&u)[4]) {
+    typedef decltype(C().x) T;
+    const C a = cadd(u[0], u[2]), b = csub(u[0], u[2]);
+    const C c = cadd(u[1], u[3]), d = csub(u[1], u[3]);
+    u[0] = cadd(a, c);
+    u[2] = csub(a, c);
+    pm_i(b, d, (T)1, u[1], u[3]);
+  }
 };
-template <> struct Roots<5> {
-  static constexpr double c[5] = {1.0, 0.30901699437494742, -0.80901699437494742, -0.80901699437494742,
-                                  0.30901699437494742};
-  static constexpr double s[5] = {0.0, 0.95105651629515357, 0.58778525229247313, -0.58778525229247313,
-                                  -0.95105651629515357};
+template <> struct SmallDft<5> {
+  template <typename C> __device__ __forceinline__ static void run(C (&u)[5]) {
+    typedef decltype(C().x) T;
+    const T c1 = (T)0.30901699437494742, c2 = (T)-0.80901699437494742;
+    const T s1 = (T)0.95105651629515357, s2 = (T)0.58778525229247313;
+    const C t1 = cadd(u[1], u[4]), t2 = cadd(u[2], u[3]);
+    const C d1 = csub(u[1], u[4]), d2 = csub(u[2], u[3]);
+    C a1, a2, b1, b2;
+    a1.x = u[0].x + c1 * t1.x + c2 * t2.x;
+    a1.y = u[0].y + c1 * t1.y + c2 * t2.y;
+    a2.x = u[0].x + c2 * t1.x + c1 * t2.x;
+    a2.y = u[0].y + c2 * t1.y + c1 * t2.y;
+    b1.x = s1 * d1.x + s2 * d2.x;
+    b1.y = s1 * d1.y + s2 * d2.y;
+    b2.x = s2 * d1.x - s1 * d2.x;
+    b2.y = s2 * d1.y - s1 * d2.y;
+    u[0] = cadd(u[0], cadd(t1, t2));
+    pm_i(a1, b1, (T)1, u[1], u[4]);
+    pm_i(a2, b2, (T)1, u[2], u[3]);
+  }
 };
+
+// Twiddles of every pass in pass order, pass s (radix R, stride p) at offset
+// off_s = sum_{t<s} (R_t - 1) p_t: entry (r - 1) p + k = e^{2 pi i r k/(p R)}
+// for r in [1, R), k in [0, p).  Consecutive butterflies (consecutive k)
+// read consecutive entries, so the twiddle reads are free of bank conflicts.
+// The table has n - 1 entries.
+template <typename C>
+__device__ void build_pass_twiddles(C* __restrict__ Wt, const C* __restrict__ twg, int n, const SynthArgs& a) {
+  int off = 0, p = 1;
+  for (int s = 0; s < a.nrad; ++s) {
+    const int R = a.rad[s], cnt = (R - 1) * p, tstep = n / (p * R);
+    for (int j = threadIdx.x; j < cnt; j += blockDim.x) {
+      const int r = j / p + 1, k = j - (r - 1) * p;
+      Wt[off + j] = twg[r * k * tstep];
+    }
+    off += cnt;
+    p *= R;
+  }
+}
 
 // One in-place pass of the mixed-radix Stockham inverse FFT (e^{+2 pi i})
 // over nl lines of length n, line l at buf + l*ls.  Butterfly i in [0, n/R)
-// reads buf[i + r n/R], multiplies by e^{2 pi i r k/(p R)} (k = i mod p), does
-// an R-point DFT and writes buf[(i/p) p R + k + r p].  All reads are staged in
-// registers before the barrier, so one buffer suffices; a thread owns at most
-// MAXB butterflies (the host sizes nl accordingly).
+// reads buf[i + r n/R], multiplies by e^{2 pi i r k/(p R)} (k = i mod p;
+// none in the first pass, p = 1), does an R-point DFT and writes
+// buf[(i/p) p R + k + r p].  All reads are staged in registers before the
+// barrier, so one buffer suffices; a thread owns at most MAXB butterflies
+// (the host sizes nl accordingly).  Tw: this pass's twiddles.
 template <int RR, int MAXB, typename C>
-__device__ __forceinline__ void stockham_pass(C* __restrict__ buf, const C* __restrict__ W, int n, int ls, int nl,
+__device__ __forceinline__ void stockham_pass(C* __restrict__ buf, const C* __restrict__ Tw, int n, int ls, int nl,
                                               int p) {
-  typedef decltype(C().x) T;
   const int Tn = n / RR;
   const int pR = p * RR;
-  const int tstep = n / pR;
   const int total = nl * Tn;
   const float inv_Tn = 1.0f / (float)Tn, inv_p = 1.0f / (float)p;
   C u[MAXB][RR];
@@ -75,9 +149,10 @@ __device__ __forceinline__ void stockham_pass(C* __restrict__ buf, const C* __re
       const int q = fdiv(i, p, inv_p), k = i - q * p;
       const C* s = buf + l * ls;
 #pragma unroll
-      for (int r = 0; r < RR; ++r) {
-        u[it][r] = s[i + r * Tn];
-        if (r) u[it][r] = cmul(u[it][r], W[r * k * tstep]);
+      for (int r = 0; r < RR; ++r) u[it][r] = s[i + r * Tn];
+      if (p > 1) {
+#pragma unroll
+        for (int r = 1; r < RR; ++r) u[it][r] = cmul(u[it][r], Tw[(r - 1) * p + k]);
       }
       dsto[it] = l * ls + q * pR + k;
     }
@@ -86,38 +161,30 @@ __device__ __forceinline__ void stockham_pass(C* __restrict__ buf, const C* __re
 #pragma unroll
   for (int it = 0; it < MAXB; ++it) {
     if (dsto[it] >= 0) {
+      SmallDft<RR>::run(u[it]);
       C* d = buf + dsto[it];
 #pragma unroll
-      for (int b = 0; b < RR; ++b) {
-        C v = u[it][0];
-#pragma unroll
-        for (int q = 1; q < RR; ++q) {
-          const int j = (q * b) % RR;
-          const T wc = (T)Roots<RR>::c[j], ws = (T)Roots<RR>::s[j];
-          v.x += u[it][q].x * wc - u[it][q].y * ws;
-          v.y += u[it][q].x * ws + u[it][q].y * wc;
-        }
-        d[b * p] = v;
-      }
+      for (int b = 0; b < RR; ++b) d[b * p] = u[it][b];
     }
   }
   __syncthreads();
 }
 
 template <int MAXB, typename C>
-__device__ __forceinline__ void stockham_single(C* buf, const C* W, int n, int ls, int nl, int R, int p) {
-  if (R == 4) stockham_pass<4, MAXB>(buf, W, n, ls, nl, p);
-  else if (R == 2) stockham_pass<2, MAXB>(buf, W, n, ls, nl, p);
-  else if (R == 3) stockham_pass<3, MAXB>(buf, W, n, ls, nl, p);
-  else stockham_pass<5, MAXB>(buf, W, n, ls, nl, p);
+__device__ __forceinline__ void stockham_single(C* buf, const C* Tw, int n, int ls, int nl, int R, int p) {
+  if (R == 4) stockham_pass<4, MAXB>(buf, Tw, n, ls, nl, p);
+  else if (R == 2) stockham_pass<2, MAXB>(buf, Tw, n, ls, nl, p);
+  else if (R == 3) stockham_pass<3, MAXB>(buf, Tw, n, ls, nl, p);
+  else stockham_pass<5, MAXB>(buf, Tw, n, ls, nl, p);
 }
 
-// Whole inverse FFT of nl lines.
+// Whole inverse FFT of nl lines; Wt from build_pass_twiddles.
 template <int MAXB, typename C>
-__device__ void stockham_lds(C* buf, const C* W, int n, int ls, int nl, const SynthArgs& a) {
-  int p = 1;
+__device__ void stockham_lds(C* buf, const C* Wt, int n, int ls, int nl, const SynthArgs& a) {
+  int p = 1, off = 0;
   for (int s = 0; s < a.nrad; ++s) {
-    stockham_single<MAXB>(buf, W, n, ls, nl, a.rad[s], p);
+    stockham_single<MAXB>(buf, Wt + off, n, ls, nl, a.rad[s], p);
+    off += (a.rad[s] - 1) * p;
     p *= a.rad[s];
   }
 }

@@ -21,8 +21,8 @@
 // workgroup.  Lines are padded to N+1 entries so that accesses striding over
 // lines spread over the LDS banks, and the odd radices run first, where the
 // Stockham writes stride by the radix.  All transforms are the unnormalised
-// inverse e^{+i k.x} of the reference's pmesh c2r.  Twiddles come from one
-// table W[j] = e^{2 pi i j / N} in LDS.
+// inverse e^{+i k.x} of the reference's pmesh c2r.  Each workgroup lays the
+// twiddles of all passes out pass by pass in LDS (conflict-free reads).
 #include "lds_fft.h"
 
 namespace cnpc {
@@ -79,11 +79,15 @@ __global__ void __launch_bounds__(256) xfft_kernel(SynthArgs a, int NL, const ty
   extern __shared__ unsigned char smem[];
   C* W = (C*)smem;
   C* buf = W + n;
-  for (int j = threadIdx.x; j < n; j += blockDim.x) W[j] = twg[j];
+  build_pass_twiddles(W, twg, n, a);
   const float inv_n = 1.0f / (float)n, inv_nl = 1.0f / (float)nl;
   const int64_t lbase = ((int64_t)kz * P + ky0) * P;
   for (int s = 0; s < a.nshell; ++s) {
     const signed char want = (signed char)(a.bin0 + s);
+    // lines with kz or every ky of the block beyond the shell's radius are
+    // zero for this shell: skipped (the y pass never reads them)
+    const int rs = a.rs[s];
+    if (kz > rs || ky0 - R > rs || ky0 + nl - 1 - R < -rs) continue;
     // loads issued UNR at a time before the LDS writes (latency hiding)
     for (int t0 = threadIdx.x; t0 < nl * n; t0 += UNR * blockDim.x) {
       C v[UNR];
@@ -141,10 +145,14 @@ __global__ void __launch_bounds__(256) yfft_kernel(SynthArgs a, int NL, const ty
   const int64_t plane = blockIdx.x;
   const int kz0 = blockIdx.y * NL;
   const int nl = min(NL, Q - kz0);
+  // kz lines beyond the shell's radius are zero: not computed, not written
+  // (the z pass and the fused contraction read kz <= rs only)
+  const int rs = a.rs[(int)(plane / n)];
+  if (kz0 > rs) return;
   extern __shared__ unsigned char smem[];
   C* W = (C*)smem;
   C* buf = W + n;
-  for (int j = threadIdx.x; j < n; j += blockDim.x) W[j] = twg[j];
+  build_pass_twiddles(W, twg, n, a);
   const float inv_n = 1.0f / (float)n;
   const C* src = U + (plane * Q + kz0) * P;
   for (int t0 = threadIdx.x; t0 < nl * n; t0 += UNR * blockDim.x) {
@@ -160,7 +168,7 @@ __global__ void __launch_bounds__(256) yfft_kernel(SynthArgs a, int NL, const ty
         const int l = fdiv(t, n, inv_n), y = t - l * n;
         const int f = y < (n >> 1) ? y : y - n;
         dst[u] = l * ls + y;
-        if (f >= -R && f <= R) v[u] = src[l * P + f + R];
+        if (f >= -rs && f <= rs && kz0 + l <= rs) v[u] = src[l * P + f + R];
       }
     }
 #pragma unroll
@@ -205,19 +213,21 @@ __global__ void __launch_bounds__(256) zfft_kernel(SynthArgs a, int NL, const ty
   extern __shared__ unsigned char smem[];
   C* W = (C*)smem;
   C* buf = W + n;
-  for (int j = threadIdx.x; j < n; j += blockDim.x) W[j] = twg[j];
-  // zero the modes Q .. n-Q of every line (the rest is written below)
-  const int nz0 = n - 2 * R - 1;
+  build_pass_twiddles(W, twg, n, a);
+  // this shell's modes have kz <= rs: zero the modes rs+1 .. n-rs-1 of every
+  // line (the rest is written below)
+  const int rs = a.rs[(int)(plane / n)], qs = rs + 1;
+  const int nz0 = n - 2 * rs - 1;
   const float inv_z = 1.0f / (float)(nz0 > 0 ? nz0 : 1), inv_nl = 1.0f / (float)nl;
   for (int t = threadIdx.x; t < nl * nz0; t += blockDim.x) {
     const int l = fdiv(t, nz0, inv_z), j = t - l * nz0;
     C z;
     z.x = (T)0;
     z.y = (T)0;
-    buf[l * ls + Q + j] = z;
+    buf[l * ls + qs + j] = z;
   }
   constexpr int UZ = UNR / 2;
-  for (int t0 = threadIdx.x; t0 < nl * Q; t0 += UZ * blockDim.x) {
+  for (int t0 = threadIdx.x; t0 < nl * qs; t0 += UZ * blockDim.x) {
     C A[UZ], B[UZ];
     int kzs[UZ], loff[UZ];
 #pragma unroll
@@ -226,7 +236,7 @@ __global__ void __launch_bounds__(256) zfft_kernel(SynthArgs a, int NL, const ty
       kzs[u] = -1;
       loff[u] = 0;
       A[u].x = A[u].y = B[u].x = B[u].y = (T)0;
-      if (t < nl * Q) {
+      if (t < nl * qs) {
         const int kz = fdiv(t, nl, inv_nl), l = t - kz * nl;
         const int ya = y0 + 2 * l, yb = ya + 1;
         const C* row = Tm + (plane * Q + kz) * n;

@@ -8,20 +8,23 @@
 // in the GEMM is most of the HBM traffic of the unfused path; here the z pass
 // runs in LDS right before the MFMAs and the real shells never reach HBM.
 //
-// A workgroup (12 waves) loops over tiles (x, y pair), one contiguous run of H over all
-// slots, and handles the two rows y of a tile in turn.  For a row:
+// A workgroup loops over tiles (x, y pair), one contiguous run of H over all
+// slots, and stages the tile's two G rows in LDS.  Then, per FFT round:
 //   1. two real shells share one complex line (two real transforms per
 //      complex FFT): line l = P + iQ with, along z, the Hermitian half
 //      spectra p, q of the two shells expanded as w[kz] = p + iq,
 //      w[n-kz] = conj(p) + i conj(q).  For B == A the pairs are (A_2l,
-//      A_2l+1); for B != A they are (A_l, B_l);
+//      A_2l+1) and one round holds both rows; for B != A they are (A_l, B_l)
+//      and each row is a round;
 //   2. the lines get the in-place Stockham inverse FFT in LDS (real part =
 //      first shell, imaginary part = second shell, z = 0 .. n-1);
-//   3. the row's n values of K are cut in chunks of 32 (the last one padded
+//   3. each row's n values of K are cut in chunks of 32 (the last one padded
 //      with zeros); wave w takes chunks w, w + nw, ...: its lanes read 8
 //      consecutive values of their A and B rows, form B * G, split hi + lo in
 //      bf16 and issue the three 16x16x32 MFMAs of each of the 2 x 2 output
 //      blocks (FP32 accumulate).
+// Slot s holds kz rows below q[s] only (its own shell radius); the rest of
+// its line is zero.
 // The per-wave accumulators are summed in a fixed order into one FP64
 // 32 x 32 partial per workgroup, and the partials in a fixed order by the
 // finalize kernel (deterministic).
@@ -31,7 +34,8 @@
 namespace cnpc {
 
 // 12 waves, 3 per SIMD: the FFT staging and the MFMA operands fit the 168
-// registers of that occupancy without spills
+// registers of that occupancy without spills; one workgroup per CU (about
+// 100 KB of LDS at n = 400)
 constexpr int ZG_THREADS = 768;
 constexpr int ZG_MAXB = 4;  // butterflies per thread and FFT pass
 
@@ -42,6 +46,7 @@ struct ZGemmArgs {
   int same;                 // B is A
   int nlines;               // complex lines per row
   int ntiles;               // n * n / 2
+  int qa[32], qb[32];       // kz rows held per slot (the rest is zero)
   SynthArgs fft;            // n and its radices
   const float4* HA;
   const float4* HB;
@@ -49,13 +54,6 @@ struct ZGemmArgs {
   const float2* tw;         // e^{2 pi i j / n}
   double* partial;          // [gridDim.x][32 * 32]
 };
-
-// half spectrum of row r (0: y, 1: y + 1) of slot s of a tile, or 0
-__device__ __forceinline__ float2 zg_half(const float4* __restrict__ T, int nvalid, int s, int kz, int Q, int r) {
-  if (s >= nvalid) return make_float2(0.f, 0.f);
-  const float4 v = T[s * Q + kz];
-  return r ? make_float2(v.z, v.w) : make_float2(v.x, v.y);
-}
 
 // 8 consecutive z values of one part of one line (0 where !ok or z >= n)
 __device__ __forceinline__ void zg_row8(const float2* buf, int ls, int line, int part, bool ok, int z0, int n,
@@ -68,37 +66,29 @@ __device__ __forceinline__ void zg_row8(const float2* buf, int ls, int line, int
   }
 }
 
-__global__ void __launch_bounds__(ZG_THREADS) zgemm_bf16x3_kernel(ZGemmArgs g) {
+// One round = the lines of one FFT and the MFMA chunks they feed.  B == A:
+// one round per tile, both rows (line r * Lh + l holds row r of shells 2l,
+// 2l+1), 2 * nchr chunks.  B != A: one round per row (line l holds A_l and
+// B_l), nchr chunks.
+template <int THREADS>
+__global__ void __launch_bounds__(THREADS) zgemm_bf16x3_kernel(ZGemmArgs g) {
   const int n = g.n, Q = g.Q, R = Q - 1, ls = n + 1, half = n >> 1, L = g.nlines;
   extern __shared__ unsigned char smem[];
   float2* W = (float2*)smem;
-  float2* buf = W + n;
-  for (int j = threadIdx.x; j < n; j += blockDim.x) W[j] = g.tw[j];
+  float* Gs = (float*)(W + n);          // G rows y, y + 1 of the tile
+  float2* buf = (float2*)(Gs + 2 * n);
+  build_pass_twiddles(W, g.tw, n, g.fft);
+  __shared__ int qa[32], qb[32];  // kz rows per slot (read by lane-varying index)
+  if (threadIdx.x < 32) {
+    qa[threadIdx.x] = g.qa[threadIdx.x];
+    qb[threadIdx.x] = g.qb[threadIdx.x];
+  }
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
   const int rr = lane & 15, kg = lane >> 4;
-  const int nch = (n + 31) >> 5;
+  const int nchr = (n + 31) >> 5;      // chunks per row
+  const int Lh = (g.na + 1) >> 1;      // lines per row when B == A
   const int nz0 = n - 2 * R - 1;
   const float inv_z = 1.0f / (float)(nz0 > 0 ? nz0 : 1), inv_q = 1.0f / (float)Q;
-  // A row i / B row j of this lane -> (line, part); rows past na / nb read
-  // a valid line and are masked to zero
-  int la[2], pa[2], lb[2], pb[2];
-  bool va[2], vb[2];
-#pragma unroll
-  for (int h = 0; h < 2; ++h) {
-    const int i = 16 * h + rr;
-    va[h] = i < g.na;
-    vb[h] = i < g.nb;
-    if (g.same) {
-      la[h] = lb[h] = i >> 1;
-      pa[h] = pb[h] = i & 1;
-    } else {
-      la[h] = lb[h] = i;
-      pa[h] = 0;
-      pb[h] = 1;
-    }
-    if (la[h] >= L) la[h] = L - 1;
-    if (lb[h] >= L) lb[h] = L - 1;
-  }
   float acc[2][2][4];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
@@ -107,59 +97,87 @@ __global__ void __launch_bounds__(ZG_THREADS) zgemm_bf16x3_kernel(ZGemmArgs g) {
 #pragma unroll
       for (int q = 0; q < 4; ++q) acc[i][j][q] = 0.f;
   f16x8 fdh, fdl;  // unused fp16 outputs of split8
+  const int rounds = g.same ? 1 : 2;
   for (int64_t tile = blockIdx.x; tile < g.ntiles; tile += gridDim.x) {
     const int x = (int)(tile / half), yp = (int)(tile - (int64_t)x * half);
     const float4* TA = g.HA + tile * g.sa_slots * Q;
     const float4* TB = g.HB + tile * g.sb_slots * Q;
-    for (int r = 0; r < 2; ++r) {
-      // modes Q .. n-Q are zero (the FFT of the previous row overwrote them)
+    for (int r0 = 0; r0 < rounds; ++r0) {
+      // modes Q .. n-Q are zero (the previous FFT overwrote them)
       for (int t = threadIdx.x; t < L * nz0; t += blockDim.x) {
         const int l = fdiv(t, nz0, inv_z), j = t - l * nz0;
         buf[l * ls + Q + j] = make_float2(0.f, 0.f);
       }
-      for (int t = threadIdx.x; t < L * Q; t += blockDim.x) {
-        const int l = fdiv(t, Q, inv_q), kz = t - l * Q;
-        float2 p, q;
-        if (g.same) {
-          p = zg_half(TA, g.na, 2 * l, kz, Q, r);
-          q = zg_half(TA, g.na, 2 * l + 1, kz, Q, r);
-        } else {
-          p = zg_half(TA, g.na, l, kz, Q, r);
-          q = zg_half(TB, g.nb, l, kz, Q, r);
+      if (r0 == 0) {
+        const float* Grow = g.G + ((int64_t)x * n + 2 * yp) * n;
+        for (int t = threadIdx.x; t < 2 * n; t += blockDim.x) Gs[t] = Grow[t];
+      }
+      if (g.same) {
+        // (l, kz): shells 2l, 2l+1, both rows -> lines l and Lh + l
+        for (int t = threadIdx.x; t < Lh * Q; t += blockDim.x) {
+          const int l = fdiv(t, Q, inv_q), kz = t - l * Q;
+          const int s0 = 2 * l, s1 = s0 + 1;
+          float4 a = make_float4(0.f, 0.f, 0.f, 0.f), b = a;
+          if (s0 < g.na && kz < qa[s0]) a = TA[s0 * Q + kz];
+          if (s1 < g.na && kz < qa[s1]) b = TA[s1 * Q + kz];
+#pragma unroll
+          for (int r = 0; r < 2; ++r) {
+            const float2 p = r ? make_float2(a.z, a.w) : make_float2(a.x, a.y);
+            const float2 q = r ? make_float2(b.z, b.w) : make_float2(b.x, b.y);
+            float2* Ln = buf + (r * Lh + l) * ls;
+            if (kz == 0) {  // c2r reads only the real part of the zero mode
+              Ln[0] = make_float2(p.x, q.x);
+            } else {
+              Ln[kz] = make_float2(p.x - q.y, p.y + q.x);
+              Ln[n - kz] = make_float2(p.x + q.y, q.x - p.y);
+            }
+          }
         }
-        float2* Ln = buf + l * ls;
-        if (kz == 0) {  // c2r reads only the real part of the zero mode
-          Ln[0] = make_float2(p.x, q.x);
-        } else {
-          Ln[kz] = make_float2(p.x - q.y, p.y + q.x);
-          Ln[n - kz] = make_float2(p.x + q.y, q.x - p.y);
+      } else {
+        for (int t = threadIdx.x; t < L * Q; t += blockDim.x) {
+          const int l = fdiv(t, Q, inv_q), kz = t - l * Q;
+          float4 a = make_float4(0.f, 0.f, 0.f, 0.f), b = a;
+          if (l < g.na && kz < qa[l]) a = TA[l * Q + kz];
+          if (l < g.nb && kz < qb[l]) b = TB[l * Q + kz];
+          const float2 p = r0 ? make_float2(a.z, a.w) : make_float2(a.x, a.y);
+          const float2 q = r0 ? make_float2(b.z, b.w) : make_float2(b.x, b.y);
+          float2* Ln = buf + l * ls;
+          if (kz == 0) {
+            Ln[0] = make_float2(p.x, q.x);
+          } else {
+            Ln[kz] = make_float2(p.x - q.y, p.y + q.x);
+            Ln[n - kz] = make_float2(p.x + q.y, q.x - p.y);
+          }
         }
       }
       __syncthreads();
       stockham_lds<ZG_MAXB>(buf, W, n, ls, L, g.fft);
-      const float* Grow = g.G + ((int64_t)x * n + 2 * yp + r) * n;
-      for (int c = w; c < nch; c += nw) {
-        const int z0 = 32 * c + 8 * kg;
+      const int nchunk = g.same ? 2 * nchr : nchr;
+      for (int c = w; c < nchunk; c += nw) {
+        const int r = g.same ? (c >= nchr ? 1 : 0) : r0;
+        const int z0 = 32 * (c - (g.same && r ? nchr : 0)) + 8 * kg;
         const bool zin = z0 < n;
-        float vg[8];
-        if (z0 + 8 <= n) {
-          const float4* gp = (const float4*)(Grow + z0);
-          const float4 g0 = gp[0], g1 = gp[1];
-          vg[0] = g0.x; vg[1] = g0.y; vg[2] = g0.z; vg[3] = g0.w;
-          vg[4] = g1.x; vg[5] = g1.y; vg[6] = g1.z; vg[7] = g1.w;
-        } else {
-#pragma unroll
-          for (int j = 0; j < 8; ++j) vg[j] = 0.f;
-        }
         const int zr = zin ? z0 : 0;  // LDS reads stay inside the line
+        float vg[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) vg[j] = (zin && z0 + j < n) ? Gs[r * n + zr + j] : 0.f;
         bf16x8 ah[2], al[2], bh[2], bl[2];
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
+          const int i = 16 * h + rr;
           float a[8], b[8];
-          zg_row8(buf, ls, la[h], pa[h], zin && va[h], zr, n, a);
-          zg_row8(buf, ls, lb[h], pb[h], zin && vb[h], zr, n, b);
+          if (g.same) {
+            const int line = r * Lh + (i >> 1 < Lh ? i >> 1 : Lh - 1);
+            zg_row8(buf, ls, line, i & 1, zin && i < g.na, zr, n, a);
 #pragma unroll
-          for (int j = 0; j < 8; ++j) b[j] *= vg[j];
+            for (int j = 0; j < 8; ++j) b[j] = a[j] * vg[j];
+          } else {
+            const int line = i < L ? i : L - 1;
+            zg_row8(buf, ls, line, 0, zin && i < g.na, zr, n, a);
+            zg_row8(buf, ls, line, 1, zin && i < g.nb, zr, n, b);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) b[j] *= vg[j];
+          }
           split8<false>(a, ah[h], al[h], fdh, fdl);
           split8<false>(b, bh[h], bl[h], fdh, fdl);
         }
@@ -175,15 +193,19 @@ __global__ void __launch_bounds__(ZG_THREADS) zgemm_bf16x3_kernel(ZGemmArgs g) {
             for (int q = 0; q < 4; ++q) acc[i][j][q] = cacc[q];
           }
       }
-      __syncthreads();  // the next row overwrites buf
+      __syncthreads();  // the next round overwrites buf (and Gs)
     }
   }
   block_reduce_store(acc, g.partial + (int64_t)blockIdx.x * 1024);
 }
 
-static int zg_lines(int na, int nb, int same) { return same ? (na + 1) / 2 : (na > nb ? na : nb); }
+// lines of one FFT round: B == A both rows (2 * ceil(na / 2)), else one row
+static int zg_lines(int na, int nb, int same) { return same ? 2 * ((na + 1) / 2) : (na > nb ? na : nb); }
 
-size_t zgemm_lds_bytes(int n, int nlines) { return sizeof(float2) * ((size_t)n + (size_t)nlines * (n + 1)); }
+// twiddles, the tile's two G rows and the lines
+size_t zgemm_lds_bytes(int n, int nlines) {
+  return sizeof(float2) * (size_t)n + sizeof(float) * 2 * (size_t)n + sizeof(float2) * (size_t)nlines * (n + 1);
+}
 
 // host check: every pass of the line FFT fits the staged registers and the
 // line buffers (plus the 8 KB reduction tile) fit the LDS
@@ -196,15 +218,19 @@ int zgemm_supported(const SynthArgs& a, int na, int nb, int same, size_t lds_lim
   return zgemm_lds_bytes(n, L) + 32 * 32 * sizeof(double) <= lds_limit ? 1 : 0;
 }
 
-hipError_t launch_zgemm(const SynthArgs& a, int Q, const void* HA, int na, int sa_slots, const void* HB, int nb,
-                        int sb_slots, int same, const float* G, const void* tw, int nblk, double* partial, double* C,
-                        hipStream_t st) {
+hipError_t launch_zgemm(const SynthArgs& a, int Q, const void* HA, int na, int sa_slots, const int* qa,
+                        const void* HB, int nb, int sb_slots, const int* qb, int same, const float* G, const void* tw,
+                        int nblk, double* partial, double* C, hipStream_t st) {
   ZGemmArgs g;
   g.n = a.n; g.Q = Q; g.na = na; g.nb = same ? na : nb;
   g.sa_slots = sa_slots; g.sb_slots = same ? sa_slots : sb_slots;
   g.same = same;
   g.nlines = zg_lines(na, g.nb, same);
   g.ntiles = a.n * (a.n / 2);
+  for (int s = 0; s < 32; ++s) {
+    g.qa[s] = qa[s];
+    g.qb[s] = qb[s];
+  }
   g.fft = a;
   g.HA = (const float4*)HA;
   g.HB = (const float4*)(same ? HA : HB);
@@ -213,10 +239,10 @@ hipError_t launch_zgemm(const SynthArgs& a, int Q, const void* HA, int na, int s
   g.partial = partial;
   const size_t lds = zgemm_lds_bytes(a.n, g.nlines);
   // dynamic LDS beyond the default 64 KB must be allowed per kernel
-  hipError_t e = hipFuncSetAttribute((const void*)zgemm_bf16x3_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     (int)lds);
+  hipError_t e = hipFuncSetAttribute((const void*)zgemm_bf16x3_kernel<ZG_THREADS>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(zgemm_bf16x3_kernel, dim3(nblk), dim3(ZG_THREADS), lds, st, g);
+  hipLaunchKernelGGL(zgemm_bf16x3_kernel<ZG_THREADS>, dim3(nblk), dim3(ZG_THREADS), lds, st, g);
   return launch_gemm_finalize(partial, nblk, 0, 0, na, g.nb, 1.0, C, g.nb, st);
 }
 

@@ -187,7 +187,8 @@ class GpuContext:
 
     def shell_synth_half(self, delta, n, boxsize, edges, bin0, nshell, poly, work, out, slot0):
         """x and y passes of the FP32 shell synthesis of bins bin0 .. bin0 +
-        nshell - 1 into slots slot0 .. of ``out`` (from ``half_shells``)."""
+        nshell - 1 into slots slot0 .. of ``out`` (from ``half_shells``).
+        Returns the kz rows written per shell (numpy int32)."""
         edges = np.ascontiguousarray(edges, dtype=np.float64)
         R = self.synth_radius(boxsize, edges[-1])
         if delta is not None:
@@ -196,10 +197,13 @@ class GpuContext:
         if (out.dtype != torch.complex64 or out.dim() != 5 or tuple(out.shape[:2]) != (n, n // 2)
                 or tuple(out.shape[3:]) != (R + 1, 2) or not out.is_contiguous() or not out.is_cuda):
             raise ValueError(f"half shells must be a contiguous ({n}, {n // 2}, nslots, {R + 1}, 2) complex64 tensor")
+        rows = np.zeros(int(nshell), dtype=np.int32)
         check(self.lib.cnpc_shell_synth_half(self.h, n, float(boxsize), _ptr(delta),
                                              edges.ctypes.data_as(ctypes.c_void_p), len(edges) - 1, int(bin0),
                                              int(nshell), ctypes.byref(poly), _ptr(work), _ptr(out),
-                                             out.shape[2], int(slot0)), "cnpc_shell_synth_half")
+                                             out.shape[2], int(slot0), rows.ctypes.data_as(ctypes.c_void_p)),
+              "cnpc_shell_synth_half")
+        return rows
 
     def zgemm_supported(self, n, na, nb, same):
         ok = ctypes.c_int(0)
@@ -207,10 +211,13 @@ class GpuContext:
               "cnpc_contract_zgemm_supported")
         return bool(ok.value)
 
-    def contract_zgemm(self, n, boxsize, kmax, G, HA, na, HB=None, nb=None, nblk=0):
+    def contract_zgemm(self, n, boxsize, kmax, G, HA, na, kz_a, HB=None, nb=None, kz_b=None, nblk=0):
         """C[a, b] = sum_x A_a G B_b over the shells in slots 0 .. na-1 of HA
         and 0 .. nb-1 of HB (None: B = A), z pass fused (device float64
-        tensor)."""
+        tensor).  kz_a / kz_b: kz rows per slot as ``shell_synth_half``
+        returned them (the rows above are not written by the synthesis)."""
+        if kz_a is None or (HB is not None and kz_b is None):
+            raise ValueError("contract_zgemm needs the kz rows of every slot")
         if G.dtype != torch.float32 or G.numel() != n ** 3 or not G.is_contiguous():
             raise ValueError("G must be a contiguous FP32 n^3 mesh")
         for H in (HA,) if HB is None else (HA, HB):
@@ -221,10 +228,22 @@ class GpuContext:
         if self.synth_radius(boxsize, kmax) + 1 != HA.shape[3]:
             raise ValueError("kmax does not match the half shells")
         nb_ = na if HB is None else nb
+
+        def rows(kz, count):
+            if kz is None:
+                return None
+            kz = np.ascontiguousarray(kz, dtype=np.int32)
+            if len(kz) < count:
+                raise ValueError("kz rows must cover every slot used")
+            return kz
+
+        ka, kb = rows(kz_a, na), rows(kz_b, nb_) if HB is not None else None
         C = torch.empty((na, nb_), dtype=torch.float64, device=self.device)
         check(self.lib.cnpc_contract_zgemm(self.h, n, float(boxsize), float(kmax), _ptr(G), _ptr(HA), int(na),
-                                           HA.shape[2], _ptr(HB), int(nb_),
-                                           HA.shape[2] if HB is None else HB.shape[2], int(nblk), _ptr(C)),
+                                           HA.shape[2], None if ka is None else ka.ctypes.data_as(ctypes.c_void_p),
+                                           _ptr(HB), int(nb_), HA.shape[2] if HB is None else HB.shape[2],
+                                           None if kb is None else kb.ctypes.data_as(ctypes.c_void_p), int(nblk),
+                                           _ptr(C)),
               "cnpc_contract_zgemm")
         return C
 

@@ -122,16 +122,19 @@ class ShellBuilder:
 
     def build_half(self, delta, poly, bins, out, slot0):
         """x and y passes of the shells of ``bins`` (consecutive) into slots
-        slot0 .. of the half-synthesised buffer ``out`` (FP32)."""
+        slot0 .. of the half-synthesised buffer ``out`` (FP32); returns the
+        kz rows written per shell."""
         b0 = bins[0]
         nb = len(bins)
         assert list(bins) == list(range(b0, b0 + nb))
         done = 0
+        rows = []
         while done < nb:
             c = min(self.batch, nb - done)
-            self.ctx.shell_synth_half(delta, self.n, self.L, self.edges, b0 + done, c, poly, self.work, out,
-                                      slot0 + done)
+            rows.append(self.ctx.shell_synth_half(delta, self.n, self.L, self.edges, b0 + done, c, poly, self.work,
+                                                  out, slot0 + done))
             done += c
+        return np.concatenate(rows)
 
 
 def _gemm_mode(contraction, prec):
@@ -241,9 +244,8 @@ def _config_matrix(ctx, builder, src1, src2, Gs, kinfo, ell_1, m1, ell_2, m2, sy
         rows = len(comps) * len(bins)
         if fused:
             H = ctx.half_shells(n, R, rows)
-            for c, (poly, _) in enumerate(comps):
-                builder.build_half(src, poly, bins, H, c * len(bins))
-            return H
+            kz = [builder.build_half(src, poly, bins, H, c * len(bins)) for c, (poly, _) in enumerate(comps)]
+            return H, np.concatenate(kz)
         out = ctx.mesh(n, prec, rows).view(rows, N3)
         for c, (poly, _) in enumerate(comps):
             builder.build(src, poly, bins, out[c * len(bins):(c + 1) * len(bins)])
@@ -277,7 +279,9 @@ def _config_matrix(ctx, builder, src1, src2, Gs, kinfo, ell_1, m1, ell_2, m2, sy
             C = 0j
             for G, gph in Gs:
                 if fused:
-                    Cm = ctx.contract_zgemm(n, L, kmax, G, R1, nc1 * len(ib), R2, nc2 * rows2).cpu().numpy()
+                    Cm = ctx.contract_zgemm(n, L, kmax, G, R1[0], nc1 * len(ib), R1[1],
+                                            None if R2 is None else R2[0], nc2 * rows2,
+                                            None if R2 is None else R2[1]).cpu().numpy()
                 else:
                     Cm = _contract(ctx, mode, prec, G, R1, R2, pairs)
                 C = C + gph * Cm

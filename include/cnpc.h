@@ -144,12 +144,14 @@ int cnpc_shell_synth(cnpc_ctx* ctx, int nmesh, double boxsize, int prec, const v
 /* The x and y passes of cnpc_shell_synth (FP32 only): shells bin0 ..
  * bin0+nshell-1 into slots slot0 .. slot0+nshell-1 of the half-synthesised
  * layout out[x][y/2][nslots][R+1][y%2] (complex64, n*(n/2)*nslots*(R+1)*2
- * entries), the input of cnpc_contract_zgemm.  work: nshell half spectra.
+ * entries), the input of cnpc_contract_zgemm.  A shell's modes have
+ * kz < kz_rows[s] (its own radius): only those rows of its slot are written.
+ * kz_rows: host array of nshell ints, or NULL.  work: nshell half spectra.
  * n must be a multiple of 8.  Same shells as clustering_estimator.py:934-970
  * before their z transform. */
 int cnpc_shell_synth_half(cnpc_ctx* ctx, int nmesh, double boxsize, const void* delta_k, const double* edges,
                           int nbins, int bin0, int nshell, const cnpc_poly* weight, void* work, void* out,
-                          int nslots, int slot0);
+                          int nslots, int slot0, int* kz_rows);
 
 /* Pairwise plain contraction: out[p] = sum_x G A_{ia[p]} B_{ib[p]}, one full
  * streaming pass per pair (the np.sum(G_00 * F_i * F_j) of
@@ -171,12 +173,14 @@ int cnpc_contract_gemm(cnpc_ctx* ctx, int mode, int64_t ncell, const void* G, co
  * (device FP64 C, row-major na x nb) where A_a, B_b are the shells held in
  * slots 0 .. na-1 of HA and 0 .. nb-1 of HB in the layout of
  * cnpc_shell_synth_half (sa_slots / sb_slots slots per tile; kmax =
- * edges[nbins] of that call), G a real FP32 n^3 mesh.  HB NULL (or HA)
- * means B == A.  na, nb <= 32.  Equals cnpc_shell_synth followed by
- * cnpc_contract_gemm mode 0 without the real shells in HBM; the block loop
- * of clustering_estimator.py:911-981. */
+ * edges[nbins] of that call), G a real FP32 n^3 mesh.  kz_a / kz_b: the
+ * kz_rows that call reported for each slot (host arrays; NULL = all R+1
+ * rows).  HB NULL (or HA) means B == A.  na, nb <= 32.  Equals
+ * cnpc_shell_synth followed by cnpc_contract_gemm mode 0 without the real
+ * shells in HBM; the block loop of clustering_estimator.py:911-981. */
 int cnpc_contract_zgemm(cnpc_ctx* ctx, int nmesh, double boxsize, double kmax, const float* G, const void* HA, int na,
-                        int sa_slots, const void* HB, int nb, int sb_slots, int nblk, double* C);
+                        int sa_slots, const int* kz_a, const void* HB, int nb, int sb_slots, const int* kz_b,
+                        int nblk, double* C);
 
 /* *ok = 1 when cnpc_contract_zgemm accepts this grid and these row counts
  * on the current device (LDS line buffers and FFT staging), else 0. */

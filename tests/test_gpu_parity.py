@@ -318,26 +318,28 @@ def test_zgemm_fused_matches_fp64_contraction(gpu_ctx, n, L, kmax):
     def half32(src, poly, rows, nslots):
         H = ctx.half_shells(n, R, nslots)
         work = ctx.spectrum(n, 1, 4)
+        kz = np.zeros(rows, dtype=np.int32)
         for b0 in range(0, rows, 4):
             c = min(4, rows - b0)
-            ctx.shell_synth_half(src, n, L, edges, b0, c, poly, work, H, b0)
-        return H
+            kz[b0:b0 + c] = ctx.shell_synth_half(src, n, L, edges, b0, c, poly, work, H, b0)
+        assert np.all(np.diff(kz) >= 0) and kz[-1] <= R + 1
+        return H, kz
 
     cases = [("same odd rows", 0, 7, None, None, 9), ("A != B", 2, 6, 0, 5, 6), ("null spectrum", 0, 5, None, None, 5)]
     for what, la, na, lb, nb, nslots in cases:
         src64, src32 = (None, None) if what == "null spectrum" else (D64, D32)
         A64, A32 = real64(src64, polys[la], na), real32(src32, polys[la], na)
-        HA = half32(src32, polys[la], na, nslots)
+        HA, kza = half32(src32, polys[la], na, nslots)
         if lb is None:
             ref = ctx.contract_gemm("gemm_fp64", G64, A64).cpu().numpy()
             sep = ctx.contract_gemm("bf16x3", G32, A32).cpu().numpy()
-            got = ctx.contract_zgemm(n, L, kmax, G32, HA, na).cpu().numpy()
+            got = ctx.contract_zgemm(n, L, kmax, G32, HA, na, kza).cpu().numpy()
         else:
             B64, B32 = real64(src64, polys[lb], nb), real32(src32, polys[lb], nb)
-            HB = half32(src32, polys[lb], nb, nb + 1)
+            HB, kzb = half32(src32, polys[lb], nb, nb + 1)
             ref = ctx.contract_gemm("gemm_fp64", G64, A64, B64).cpu().numpy()
             sep = ctx.contract_gemm("bf16x3", G32, A32, B32).cpu().numpy()
-            got = ctx.contract_zgemm(n, L, kmax, G32, HA, na, HB, nb).cpu().numpy()
+            got = ctx.contract_zgemm(n, L, kmax, G32, HA, na, kza, HB, nb, kzb).cpu().numpy()
         scale = np.max(np.abs(ref))
         err, err_sep = np.max(np.abs(got - ref)) / scale, np.max(np.abs(sep - ref)) / scale
         _log(f"zgemm n={n} {what}: rel err fused {err:.2e}, separate {err_sep:.2e}")
