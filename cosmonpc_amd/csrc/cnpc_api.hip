@@ -481,13 +481,15 @@ int cnpc_contract_zgemm(cnpc_ctx* ctx, int n, double boxsize, double kmax, const
   std::memset(&a, 0, sizeof(a));
   a.n = n; a.R = R;
   REQUIRE(factor_radices(n, a), "contract_zgemm: nmesh must factor into 2, 3 and 5");
+  merge_radices(a);
   int dev = 0, lds_max = 0;
   CHECK_HIP(hipGetDevice(&dev));
   CHECK_HIP(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
   REQUIRE(zgemm_supported(a, na, nb, same, (size_t)lds_max),
           "contract_zgemm: grid too large for the LDS line buffers");
-  if (nblk <= 0) {  // one resident workgroup per CU
+  if (nblk <= 0) {  // as many workgroups as stay resident
     CHECK_HIP(hipDeviceGetAttribute(&nblk, hipDeviceAttributeMultiprocessorCount, dev));
+    nblk *= zgemm_blocks_per_cu(same);
   }
   if (nblk > 4096) nblk = 4096;
   const size_t pbytes = (size_t)nblk * 1024 * 8;
@@ -514,6 +516,7 @@ int cnpc_contract_zgemm_supported(cnpc_ctx* ctx, int n, int na, int nb, int same
   std::memset(&a, 0, sizeof(a));
   a.n = n;
   if (!factor_radices(n, a)) return 0;
+  merge_radices(a);
   int dev = 0, lds_max = 0;
   CHECK_HIP(hipGetDevice(&dev));
   CHECK_HIP(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));

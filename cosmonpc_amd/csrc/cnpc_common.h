@@ -276,6 +276,24 @@ hipError_t launch_pair_weights(int64_t np, const double* W, const double* WF, in
 hipError_t launch_radecz(int64_t np, const double* ra, const double* dec, const double* z, const RadeczArgs& a,
                          double* pos, hipStream_t st);
 
+// merge adjacent radices into the composite ones (16, 20, 25) the wide LDS
+// FFT passes handle (fewer passes, fewer barriers)
+inline void merge_radices(SynthArgs& a) {
+  int k = 0;
+  for (int s = 0; s < a.nrad; ++s) {
+    const int m = (s + 1 < a.nrad) ? a.rad[s] * a.rad[s + 1] : 0;
+    if (m == 16 || m == 20 || m == 25) {
+      a.rad[k++] = m;
+      ++s;
+    } else {
+      a.rad[k++] = a.rad[s];
+    }
+  }
+  a.nrad = k;
+}
+
+inline int wide_radix(int R) { return R == 16 || R == 20 || R == 25; }
+
 int synth_lines_per_block(int prec, const SynthArgs& a);
 hipError_t launch_synth_half(const SynthArgs& a, const BinEdges& E, const PolySpec& W, const void* D, void* V,
                              signed char* bins, void* tw, void* work, void* out, int nslots, int slot0,
@@ -283,6 +301,7 @@ hipError_t launch_synth_half(const SynthArgs& a, const BinEdges& E, const PolySp
 hipError_t launch_twiddles(int n, void* tw, hipStream_t st);
 size_t zgemm_lds_bytes(int n, int nlines);
 int zgemm_supported(const SynthArgs& a, int na, int nb, int same, size_t lds_limit);
+int zgemm_blocks_per_cu(int same);
 hipError_t launch_zgemm(const SynthArgs& a, int Q, const void* HA, int na, int sa_slots, const int* qa,
                         const void* HB, int nb, int sb_slots, const int* qb, int same, const float* G, const void* tw,
                         int nblk, double* partial, double* C, hipStream_t st);

@@ -105,6 +105,97 @@ template <> struct SmallDft<5> {
   }
 };
 
+// cos / sin of 2 pi j / N at compile time (range-reduced Taylor series,
+// double precision), for the internal twiddles of the composite DFTs
+constexpr double ct_pi = 3.14159265358979323846;
+constexpr double ct_sin_red(double x) {  // |x| <= pi/4
+  double t = x, s = x;
+  for (int k = 1; k < 14; ++k) {
+    t *= -x * x / ((2.0 * k) * (2.0 * k + 1.0));
+    s += t;
+  }
+  return s;
+}
+constexpr double ct_cos_red(double x) {
+  double t = 1.0, s = 1.0;
+  for (int k = 1; k < 14; ++k) {
+    t *= -x * x / ((2.0 * k - 1.0) * (2.0 * k));
+    s += t;
+  }
+  return s;
+}
+// e^{2 pi i j / N}: octant reduction on j / N
+constexpr double ct_cos(int j, int N) {
+  const int m = ((j % N) + N) % N;
+  const int o = (8 * m) / N;  // octant
+  const double x = 2.0 * ct_pi * m / N;
+  return o == 0 || o == 7 ? ct_cos_red(o == 0 ? x : x - 2.0 * ct_pi)
+       : o == 1 || o == 2 ? -ct_sin_red(x - 0.5 * ct_pi)
+       : o == 3 || o == 4 ? -ct_cos_red(x - ct_pi)
+                          : ct_sin_red(x - 1.5 * ct_pi);
+}
+constexpr double ct_sin(int j, int N) {
+  const int m = ((j % N) + N) % N;
+  const int o = (8 * m) / N;
+  const double x = 2.0 * ct_pi * m / N;
+  return o == 0 || o == 7 ? ct_sin_red(o == 0 ? x : x - 2.0 * ct_pi)
+       : o == 1 || o == 2 ? ct_cos_red(x - 0.5 * ct_pi)
+       : o == 3 || o == 4 ? -ct_sin_red(x - ct_pi)
+                          : -ct_cos_red(x - 1.5 * ct_pi);
+}
+
+// e^{2 pi i j / N} for j < N as compile-time tables
+template <int N> struct TwTable {
+  double c[N], s[N];
+  constexpr TwTable() : c(), s() {
+    for (int j = 0; j < N; ++j) {
+      c[j] = ct_cos(j, N);
+      s[j] = ct_sin(j, N);
+    }
+  }
+};
+template <int N> struct TwConst { static constexpr TwTable<N> t{}; };
+
+// Composite DFT of N = N1 N2 points in registers (Cooley-Tukey): with
+// q = N2 n1 + n2 and k = k1 + N1 k2, X[k] = sum_n2 w_N^{n2 k1} w_N2^{n2 k2}
+// sum_n1 u[q] w_N1^{n1 k1}.
+template <int N1, int N2> struct CompositeDft {
+  template <typename C> __device__ __forceinline__ static void run(C (&u)[N1 * N2]) {
+    typedef decltype(C().x) T;
+    C y[N2][N1];
+#pragma unroll
+    for (int n2 = 0; n2 < N2; ++n2) {
+      C a[N1];
+#pragma unroll
+      for (int n1 = 0; n1 < N1; ++n1) a[n1] = u[N2 * n1 + n2];
+      SmallDft<N1>::run(a);
+#pragma unroll
+      for (int k1 = 0; k1 < N1; ++k1) {
+        if (n2 * k1 == 0) {
+          y[n2][k1] = a[k1];
+        } else {
+          C w;
+          w.x = (T)TwConst<N1 * N2>::t.c[n2 * k1];
+          w.y = (T)TwConst<N1 * N2>::t.s[n2 * k1];
+          y[n2][k1] = cmul(a[k1], w);
+        }
+      }
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < N1; ++k1) {
+      C b[N2];
+#pragma unroll
+      for (int n2 = 0; n2 < N2; ++n2) b[n2] = y[n2][k1];
+      SmallDft<N2>::run(b);
+#pragma unroll
+      for (int k2 = 0; k2 < N2; ++k2) u[k1 + N1 * k2] = b[k2];
+    }
+  }
+};
+template <> struct SmallDft<16> : CompositeDft<4, 4> {};
+template <> struct SmallDft<20> : CompositeDft<4, 5> {};
+template <> struct SmallDft<25> : CompositeDft<5, 5> {};
+
 // Twiddles of every pass in pass order, pass s (radix R, stride p) at offset
 // off_s = sum_{t<s} (R_t - 1) p_t: entry (r - 1) p + k = e^{2 pi i r k/(p R)}
 // for r in [1, R), k in [0, p).  Consecutive butterflies (consecutive k)
@@ -178,12 +269,27 @@ __device__ __forceinline__ void stockham_single(C* buf, const C* Tw, int n, int 
   else stockham_pass<5, MAXB>(buf, Tw, n, ls, nl, p);
 }
 
-// Whole inverse FFT of nl lines; Wt from build_pass_twiddles.
+// the same with the composite radices 16, 20 and 25 as well (one such
+// butterfly per thread: the host checks nl * n / R <= blockDim.x)
 template <int MAXB, typename C>
+__device__ __forceinline__ void stockham_single_wide(C* buf, const C* Tw, int n, int ls, int nl, int R, int p) {
+  if (R == 16) stockham_pass<16, 1>(buf, Tw, n, ls, nl, p);
+  else if (R == 20) stockham_pass<20, 1>(buf, Tw, n, ls, nl, p);
+  else if (R == 25) stockham_pass<25, 1>(buf, Tw, n, ls, nl, p);
+  else stockham_single<MAXB>(buf, Tw, n, ls, nl, R, p);
+}
+
+
+// Whole inverse FFT of nl lines; Wt from build_pass_twiddles.  WIDE admits
+// the composite radices in a.rad.
+template <int MAXB, bool WIDE = false, typename C>
 __device__ void stockham_lds(C* buf, const C* Wt, int n, int ls, int nl, const SynthArgs& a) {
   int p = 1, off = 0;
   for (int s = 0; s < a.nrad; ++s) {
-    stockham_single<MAXB>(buf, Wt + off, n, ls, nl, a.rad[s], p);
+    if (WIDE)
+      stockham_single_wide<MAXB>(buf, Wt + off, n, ls, nl, a.rad[s], p);
+    else
+      stockham_single<MAXB>(buf, Wt + off, n, ls, nl, a.rad[s], p);
     off += (a.rad[s] - 1) * p;
     p *= a.rad[s];
   }

@@ -36,8 +36,14 @@ namespace cnpc {
 // 12 waves, 3 per SIMD: the FFT staging and the MFMA operands fit the 168
 // registers of that occupancy without spills; one workgroup per CU (about
 // 100 KB of LDS at n = 400)
-constexpr int ZG_THREADS = 768;
-constexpr int ZG_MAXB = 4;  // butterflies per thread and FFT pass
+#ifndef ZG_THREADS_DEF
+#define ZG_THREADS_DEF 768
+#endif
+#ifndef ZG_MAXB_DEF
+#define ZG_MAXB_DEF 4
+#endif
+constexpr int ZG_THREADS = ZG_THREADS_DEF;
+constexpr int ZG_MAXB = ZG_MAXB_DEF;  // butterflies per thread and FFT pass
 
 struct ZGemmArgs {
   int n, Q;                 // grid, non-zero kz per line (R + 1)
@@ -66,11 +72,12 @@ __device__ __forceinline__ void zg_row8(const float2* buf, int ls, int line, int
   }
 }
 
-// One round = the lines of one FFT and the MFMA chunks they feed.  B == A:
-// one round per tile, both rows (line r * Lh + l holds row r of shells 2l,
-// 2l+1), 2 * nchr chunks.  B != A: one round per row (line l holds A_l and
-// B_l), nchr chunks.
-template <int THREADS>
+// One round = the lines of one FFT and the MFMA chunks they feed.  B == A
+// with BR: one round per tile, both rows (line r * Lh + l holds row r of
+// shells 2l, 2l+1), 2 * nchr chunks; B == A without BR: one round per row
+// (line l holds shells 2l, 2l+1 of that row).  B != A: one round per row
+// (line l holds A_l and B_l), nchr chunks.
+template <int THREADS, bool BR>
 __global__ void __launch_bounds__(THREADS) zgemm_bf16x3_kernel(ZGemmArgs g) {
   const int n = g.n, Q = g.Q, R = Q - 1, ls = n + 1, half = n >> 1, L = g.nlines;
   extern __shared__ unsigned char smem[];
@@ -97,7 +104,7 @@ __global__ void __launch_bounds__(THREADS) zgemm_bf16x3_kernel(ZGemmArgs g) {
 #pragma unroll
       for (int q = 0; q < 4; ++q) acc[i][j][q] = 0.f;
   f16x8 fdh, fdl;  // unused fp16 outputs of split8
-  const int rounds = g.same ? 1 : 2;
+  const int rounds = (g.same && BR) ? 1 : 2;
   for (int64_t tile = blockIdx.x; tile < g.ntiles; tile += gridDim.x) {
     const int x = (int)(tile / half), yp = (int)(tile - (int64_t)x * half);
     const float4* TA = g.HA + tile * g.sa_slots * Q;
@@ -113,7 +120,8 @@ __global__ void __launch_bounds__(THREADS) zgemm_bf16x3_kernel(ZGemmArgs g) {
         for (int t = threadIdx.x; t < 2 * n; t += blockDim.x) Gs[t] = Grow[t];
       }
       if (g.same) {
-        // (l, kz): shells 2l, 2l+1, both rows -> lines l and Lh + l
+        // (l, kz): shells 2l, 2l+1; both rows -> lines l and Lh + l (BR),
+        // else row r0 -> line l
         for (int t = threadIdx.x; t < Lh * Q; t += blockDim.x) {
           const int l = fdiv(t, Q, inv_q), kz = t - l * Q;
           const int s0 = 2 * l, s1 = s0 + 1;
@@ -121,10 +129,11 @@ __global__ void __launch_bounds__(THREADS) zgemm_bf16x3_kernel(ZGemmArgs g) {
           if (s0 < g.na && kz < qa[s0]) a = TA[s0 * Q + kz];
           if (s1 < g.na && kz < qa[s1]) b = TA[s1 * Q + kz];
 #pragma unroll
-          for (int r = 0; r < 2; ++r) {
+          for (int rb = 0; rb < (BR ? 2 : 1); ++rb) {
+            const int r = BR ? rb : r0;
             const float2 p = r ? make_float2(a.z, a.w) : make_float2(a.x, a.y);
             const float2 q = r ? make_float2(b.z, b.w) : make_float2(b.x, b.y);
-            float2* Ln = buf + (r * Lh + l) * ls;
+            float2* Ln = buf + ((BR ? r * Lh : 0) + l) * ls;
             if (kz == 0) {  // c2r reads only the real part of the zero mode
               Ln[0] = make_float2(p.x, q.x);
             } else {
@@ -151,11 +160,11 @@ __global__ void __launch_bounds__(THREADS) zgemm_bf16x3_kernel(ZGemmArgs g) {
         }
       }
       __syncthreads();
-      stockham_lds<ZG_MAXB>(buf, W, n, ls, L, g.fft);
-      const int nchunk = g.same ? 2 * nchr : nchr;
+      stockham_lds<ZG_MAXB, true>(buf, W, n, ls, L, g.fft);
+      const int nchunk = (g.same && BR) ? 2 * nchr : nchr;
       for (int c = w; c < nchunk; c += nw) {
-        const int r = g.same ? (c >= nchr ? 1 : 0) : r0;
-        const int z0 = 32 * (c - (g.same && r ? nchr : 0)) + 8 * kg;
+        const int r = (g.same && BR) ? (c >= nchr ? 1 : 0) : r0;
+        const int z0 = 32 * (c - ((g.same && BR && r) ? nchr : 0)) + 8 * kg;
         const bool zin = z0 < n;
         const int zr = zin ? z0 : 0;  // LDS reads stay inside the line
         float vg[8];
@@ -167,7 +176,7 @@ __global__ void __launch_bounds__(THREADS) zgemm_bf16x3_kernel(ZGemmArgs g) {
           const int i = 16 * h + rr;
           float a[8], b[8];
           if (g.same) {
-            const int line = r * Lh + (i >> 1 < Lh ? i >> 1 : Lh - 1);
+            const int line = (BR ? r * Lh : 0) + (i >> 1 < Lh ? i >> 1 : Lh - 1);
             zg_row8(buf, ls, line, i & 1, zin && i < g.na, zr, n, a);
 #pragma unroll
             for (int j = 0; j < 8; ++j) b[j] = a[j] * vg[j];
@@ -199,8 +208,18 @@ __global__ void __launch_bounds__(THREADS) zgemm_bf16x3_kernel(ZGemmArgs g) {
   block_reduce_store(acc, g.partial + (int64_t)blockIdx.x * 1024);
 }
 
-// lines of one FFT round: B == A both rows (2 * ceil(na / 2)), else one row
-static int zg_lines(int na, int nb, int same) { return same ? 2 * ((na + 1) / 2) : (na > nb ? na : nb); }
+// Launch shapes.  B == A: 2 * ceil(na/2) lines per round (both rows) with
+// 768 threads, one workgroup per CU, unless ZG_SAME_PER_ROW picks per-row
+// rounds of ceil(na/2) lines with 384 threads, two workgroups per CU.
+// B != A: max(na, nb) lines per round, 768 threads.
+#ifndef ZG_SAME_PER_ROW
+#define ZG_SAME_PER_ROW 0
+#endif
+static int zg_lines(int na, int nb, int same) {
+  if (same) return ZG_SAME_PER_ROW ? (na + 1) / 2 : 2 * ((na + 1) / 2);
+  return na > nb ? na : nb;
+}
+static int zg_threads(int same) { return (same && ZG_SAME_PER_ROW) ? 384 : ZG_THREADS; }
 
 // twiddles, the tile's two G rows and the lines
 size_t zgemm_lds_bytes(int n, int nlines) {
@@ -214,8 +233,21 @@ int zgemm_supported(const SynthArgs& a, int na, int nb, int same, size_t lds_lim
   if (n % 8 != 0 || na < 1 || na > 32 || nb < 1 || nb > 32) return 0;
   const int L = zg_lines(na, nb, same);
   for (int s = 0; s < a.nrad; ++s)
-    if ((int64_t)L * (n / a.rad[s]) > (int64_t)ZG_MAXB * ZG_THREADS) return 0;
+    if ((int64_t)L * (n / a.rad[s]) > (int64_t)(wide_radix(a.rad[s]) ? 1 : ZG_MAXB) * zg_threads(same)) return 0;
   return zgemm_lds_bytes(n, L) + 32 * 32 * sizeof(double) <= lds_limit ? 1 : 0;
+}
+
+// workgroups per CU the launch shape allows
+int zgemm_blocks_per_cu(int same) { return (same && ZG_SAME_PER_ROW) ? 2 : 1; }
+
+template <int T, bool BR>
+static hipError_t zg_launch(const ZGemmArgs& g, int nblk, size_t lds, hipStream_t st) {
+  // dynamic LDS beyond the default 64 KB must be allowed per kernel
+  hipError_t e = hipFuncSetAttribute((const void*)zgemm_bf16x3_kernel<T, BR>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((zgemm_bf16x3_kernel<T, BR>), dim3(nblk), dim3(T), lds, st, g);
+  return hipGetLastError();
 }
 
 hipError_t launch_zgemm(const SynthArgs& a, int Q, const void* HA, int na, int sa_slots, const int* qa,
@@ -238,11 +270,14 @@ hipError_t launch_zgemm(const SynthArgs& a, int Q, const void* HA, int na, int s
   g.tw = (const float2*)tw;
   g.partial = partial;
   const size_t lds = zgemm_lds_bytes(a.n, g.nlines);
-  // dynamic LDS beyond the default 64 KB must be allowed per kernel
-  hipError_t e = hipFuncSetAttribute((const void*)zgemm_bf16x3_kernel<ZG_THREADS>,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipError_t e;
+  if (same && ZG_SAME_PER_ROW)
+    e = zg_launch<384, false>(g, nblk, lds, st);
+  else if (same)
+    e = zg_launch<ZG_THREADS, true>(g, nblk, lds, st);
+  else
+    e = zg_launch<ZG_THREADS, false>(g, nblk, lds, st);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(zgemm_bf16x3_kernel<ZG_THREADS>, dim3(nblk), dim3(ZG_THREADS), lds, st, g);
   return launch_gemm_finalize(partial, nblk, 0, 0, na, g.nb, 1.0, C, g.nb, st);
 }
 
