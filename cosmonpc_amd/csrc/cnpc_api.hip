@@ -401,6 +401,7 @@ int cnpc_shell_synth(cnpc_ctx* ctx, int n, double boxsize, int prec, const void*
   REQUIRE(nshell <= CNPC_SYNTH_MAXS, "shell_synth: too many shells in one call");
   shell_radii(a, edges, boxsize);
   REQUIRE(factor_radices(n, a), "shell_synth: nmesh must factor into 2, 3 and 5");
+  if (prec == 1) merge_radices(a);  // the FP32 passes take the composite radices
   REQUIRE(synth_lines_per_block(prec, a) >= 1, "shell_synth: mesh too large for the LDS line buffers");
   const size_t cs = prec == 0 ? 16 : 8;
   const size_t P = 2 * (size_t)R + 1, Q = (size_t)R + 1, cube = P * Q * P;
@@ -442,6 +443,7 @@ int cnpc_shell_synth_half(cnpc_ctx* ctx, int n, double boxsize, const void* D, c
   if (kz_rows)
     for (int s = 0; s < nshell; ++s) kz_rows[s] = a.rs[s] + 1;
   REQUIRE(factor_radices(n, a), "shell_synth_half: nmesh must factor into 2, 3 and 5");
+  merge_radices(a);
   REQUIRE(synth_lines_per_block(1, a) >= 1, "shell_synth_half: mesh too large for the LDS line buffers");
   const size_t P = 2 * (size_t)R + 1, Q = (size_t)R + 1, cube = P * Q * P;
   // work holds U (nshell n P Q complex), at most nshell half spectra

@@ -121,7 +121,7 @@ __global__ void __launch_bounds__(256) xfft_kernel(SynthArgs a, int NL, const ty
       }
     }
     __syncthreads();
-    stockham_lds<MaxB<T>::v>(buf, W, n, ls, nl, a);
+    stockham_lds<MaxB<T>::v, sizeof(T) == 4>(buf, W, n, ls, nl, a);
     // U[s][x][kz][ky]: consecutive threads take consecutive ky of one x
     for (int t = threadIdx.x; t < nl * n; t += blockDim.x) {
       const int x = fdiv(t, nl, inv_nl), l = t - x * nl;
@@ -176,7 +176,7 @@ __global__ void __launch_bounds__(256) yfft_kernel(SynthArgs a, int NL, const ty
       if (dst[u] >= 0) buf[dst[u]] = v[u];
   }
   __syncthreads();
-  stockham_lds<MaxB<T>::v>(buf, W, n, ls, nl, a);
+  stockham_lds<MaxB<T>::v, sizeof(T) == 4>(buf, W, n, ls, nl, a);
   if (nslots == 0) {
     C* dst = Tm + (plane * Q + kz0) * n;
     for (int t = threadIdx.x; t < nl * n; t += blockDim.x) {
@@ -263,7 +263,7 @@ __global__ void __launch_bounds__(256) zfft_kernel(SynthArgs a, int NL, const ty
     }
   }
   __syncthreads();
-  stockham_lds<MaxB<T>::v>(buf, W, n, ls, nl, a);
+  stockham_lds<MaxB<T>::v, sizeof(T) == 4>(buf, W, n, ls, nl, a);
   const float inv_n = 1.0f / (float)n;
   T* dst = out + (plane * n + y0) * n;
   for (int t = threadIdx.x; t < nrows * n; t += blockDim.x) {
@@ -299,7 +299,8 @@ int synth_lines_per_block(int prec, const SynthArgs& a) {
   int nl = (int)((budget - (size_t)n * cs) / ((size_t)(n + 1) * cs));
   const int maxb = prec == 0 ? MaxB<double>::v : MaxB<float>::v;
   for (int s = 0; s < a.nrad; ++s) {
-    const int lim = maxb * 256 * a.rad[s] / n;  // nl * n / R butterflies <= maxb * 256
+    // nl * n / R butterflies <= maxb * 256 (one per thread for the composite radices)
+    const int lim = (wide_radix(a.rad[s]) ? 1 : maxb) * 256 * a.rad[s] / n;
     if (nl > lim) nl = lim;
   }
   return nl > 16 ? 16 : nl;
