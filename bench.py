@@ -287,9 +287,22 @@ def kernel_roofline(ctx, delta, n, L, k_edge, torch, np, A):
     dom = max(("shell_synth_half_batch4", "zgemm_bf16x3"), key=lambda k: stages[k]["total_ms"])
     ach = stages[dom]["GBps"]
     roof = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
-            "traffic": None, "kernel": dom, "algorithmic_bytes_per_launch": stages[dom]["bytes"]}
+            "traffic": measured_traffic(dom), "kernel": dom, "algorithmic_bytes_per_launch": stages[dom]["bytes"]}
     del work, shells, G, H
     return {"roofline": roof, "stages": stages}
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of ``kernel`` as measured by the rocprofv3
+    FETCH_SIZE / WRITE_SIZE runs summarised in profiles/ (None if absent)."""
+    path = os.path.join(ROOT, "profiles", "kernels_512_hbm_traffic_pmc.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        parts = line.split()
+        if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
+            return float(parts[2])
+    return None
 
 
 def cpu_baseline(n, L, k_edge, npairs):
