@@ -143,6 +143,14 @@ def main():
     torch.cuda.synchronize()
     t_sep = time.perf_counter() - t0
     log(f"bf16x3 separate z pass: {t_sep * 1e3:.1f} ms/step -> {bins_per_step / t_sep:.1f} bins/s")
+    # the FP8 split variant (three e4m3 pieces, six products; separate z pass)
+    signal_step(d32, 1, "fp8x6")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res_fp8 = signal_step(d32, 1, "fp8x6")
+    torch.cuda.synchronize()
+    t_fp8 = time.perf_counter() - t0
+    log(f"fp8x6 (separate z pass): {t_fp8 * 1e3:.1f} ms/step -> {bins_per_step / t_fp8:.1f} bins/s")
 
     # ---------------------------------------------------------- per-kernel roofline
     m, nmax = contraction_grid(n, L, kmax, "bf16x3")
@@ -151,7 +159,8 @@ def main():
     del dm32
     log(f"kernels (grid {m}^3): {json.dumps(roof['stages'])}")
 
-    extra = {"separate_z_ms_per_step": t_sep * 1e3, "separate_z_bins_per_s": bins_per_step / t_sep}
+    extra = {"separate_z_ms_per_step": t_sep * 1e3, "separate_z_bins_per_s": bins_per_step / t_sep,
+             "fp8x6_ms_per_step": t_fp8 * 1e3, "fp8x6_bins_per_s": bins_per_step / t_fp8}
     if not args.no_fp64:
         paint_field(ctx, pos_d, w_d, n, L, sampler, True, 0)
         torch.cuda.synchronize()
@@ -174,10 +183,11 @@ def main():
             log(f"fp64 {mode}: {dt * 1e3:.1f} ms/step -> {bins_per_step / dt:.1f} bins/s")
         ref = res["pairwise"]
         extra["speedup_vs_fp64_plain"] = extra["pairwise_ms_per_step"] / (t_step * 1e3)
-        for a, r64, rb, rs, rg in zip(angus, ref, res_bf16, res_sep, res["gemm_fp64"]):
+        for a, r64, rb, rs, r8, rg in zip(angus, ref, res_bf16, res_sep, res_fp8, res["gemm_fp64"]):
             tag = "".join(map(str, a))
             extra[f"err_bf16x3_B{tag}"] = float(np.max(np.abs(rb - r64)) / np.max(np.abs(r64)))
             extra[f"err_bf16x3_separate_B{tag}"] = float(np.max(np.abs(rs - r64)) / np.max(np.abs(r64)))
+            extra[f"err_fp8x6_B{tag}"] = float(np.max(np.abs(r8 - r64)) / np.max(np.abs(r64)))
             extra[f"err_gemm_fp64_B{tag}"] = float(np.max(np.abs(rg - r64)) / np.max(np.abs(r64)))
         del d64
     cpu = None if args.no_cpu else cpu_baseline(n, L, k_edge, npairs)

@@ -311,7 +311,9 @@ int cnpc_prepare_spectrum(cnpc_ctx* ctx, int n, double boxsize, int prec, const 
   a.n = n; a.interlaced = interlaced ? 1 : 0; a.comp = comp; a.p = window;
   a.kf = 2.0 * M_PI / boxsize; a.H = boxsize / n; a.scale = scale; a.sub0 = sub0;
   a.zero0 = zero_k0 ? 1 : 0; a.pad = 0;
-  CHECK_HIP(launch_prepare(prec, a, c1, c2, out, ctx->stream));
+  // per-axis tables: n doubles (compensation) + n complex doubles (phase)
+  if (int rc = ensure_scratch(ctx, (size_t)(n + 2) * 24)) return rc;
+  CHECK_HIP(launch_prepare(prec, a, c1, c2, out, ctx->scratch, ctx->stream));
   return 0;
 }
 
@@ -583,7 +585,7 @@ int cnpc_contract_pairs(cnpc_ctx* ctx, int prec, int64_t ncell, const void* G, c
 int cnpc_contract_gemm(cnpc_ctx* ctx, int mode, int64_t ncell, const void* G, const void* A, int na, const void* B,
                        int nb, int same, float sa, float sb, int nblk, double* C) {
   REQUIRE(ctx, "null context");
-  REQUIRE(mode >= 0 && mode <= 2, "contract_gemm: mode must be 0 (bf16x3), 1 (fp16x3) or 2 (fp64)");
+  REQUIRE(mode >= 0 && mode <= 3, "contract_gemm: mode must be 0 (bf16x3), 1 (fp16x3), 2 (fp64) or 3 (fp8x6)");
   REQUIRE(ncell > 0 && na >= 1 && nb >= 1 && G && A && B && C, "contract_gemm: bad arguments");
   REQUIRE(ncell % 64 == 0, "contract_gemm: ncell must be a multiple of 64");
   REQUIRE(!same || (A == B && na == nb), "contract_gemm: same requires A == B and na == nb");

@@ -320,7 +320,8 @@ hipError_t launch_paint(const double* pos, const double* w, int64_t np, int n, d
 hipError_t launch_paint_batch(const double* pos, const double* w, const int64_t* off, int nb, int64_t np_max, int n,
                               double boxsize, int order, const double* center, int ncopy, int prec, void* m0, void* m1,
                               hipStream_t st);
-hipError_t launch_prepare(int prec, const PrepareArgs& a, const void* c1, const void* c2, void* out, hipStream_t st);
+hipError_t launch_prepare(int prec, const PrepareArgs& a, const void* c1, const void* c2, void* out, void* tables,
+                          hipStream_t st);
 hipError_t launch_bin(int prec, const BinArgs& a, const BinEdges& E, const PolySet& P, const void* A, const void* B,
                       double* out, hipStream_t st);
 hipError_t launch_shell_filter(int prec, const ShellArgs& a, const BinEdges& E, const PolySpec& W, const void* D,

@@ -165,6 +165,94 @@ __global__ void __launch_bounds__(256) gemm_x3_kernel(GemmArgs g) {
   block_reduce_store(acc, g.partial + (int64_t)blockIdx.x * 1024);
 }
 
+// ------------------------------------------------------------- gemm_fp8x6
+// FP8 (e4m3) variant: each prescaled FP32 value x (|x| <= 2^7) is split into
+// three fp8 pieces x = p0 + p1 2^-4 + p2 2^-8 (each piece the fp8 rounding of
+// the remaining residual, scaled by 2^4 to stay in range), and
+// A B ~ sum_{i+j<=2} p_i q_j 2^{-4(i+j)}: six 16x16x32 fp8 MFMAs into three
+// FP32 accumulators (orders 0, 1, 2), combined per K-chunk.
+__device__ __forceinline__ void split8_fp8(const float (&v)[8], long (&p)[3]) {
+  int w[3][2];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    float r[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) r[j] = v[4 * h + j];
+#pragma unroll
+    for (int piece = 0; piece < 3; ++piece) {
+      int lo = __builtin_amdgcn_cvt_pk_fp8_f32(r[0], r[1], 0, false);
+      int pk = __builtin_amdgcn_cvt_pk_fp8_f32(r[2], r[3], lo, true);
+      w[piece][h] = pk;
+      if (piece < 2) {  // residuals of the fp8 roundings, scaled by 2^4
+        r[0] = (r[0] - __builtin_amdgcn_cvt_f32_fp8(pk, 0)) * 16.0f;
+        r[1] = (r[1] - __builtin_amdgcn_cvt_f32_fp8(pk, 1)) * 16.0f;
+        r[2] = (r[2] - __builtin_amdgcn_cvt_f32_fp8(pk, 2)) * 16.0f;
+        r[3] = (r[3] - __builtin_amdgcn_cvt_f32_fp8(pk, 3)) * 16.0f;
+      }
+    }
+  }
+#pragma unroll
+  for (int piece = 0; piece < 3; ++piece)
+    p[piece] = (long)(((unsigned long)(unsigned)w[piece][1] << 32) | (unsigned)w[piece][0]);
+}
+
+__global__ void __launch_bounds__(256) gemm_fp8x6_kernel(GemmArgs g) {
+  const float* __restrict__ G = (const float*)g.G;
+  const float* __restrict__ A = (const float*)g.A;
+  const float* __restrict__ B = (const float*)g.B;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int rr = lane & 15, kg = lane >> 4;
+  int ra[2], rb[2];
+  for (int h = 0; h < 2; ++h) {
+    ra[h] = min(g.a0 + 16 * h + rr, g.na - 1);
+    rb[h] = min(g.b0 + 16 * h + rr, g.nb - 1);
+  }
+  const float* pa[2] = {A + ra[0] * g.lda, A + ra[1] * g.lda};
+  const float* pb[2] = {B + rb[0] * g.ldb, B + rb[1] * g.ldb};
+  float acc[2][2][4];
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 2; ++j)
+      for (int q = 0; q < 4; ++q) acc[i][j][q] = 0.f;
+  const int64_t kbeg = (int64_t)blockIdx.x * g.kper;
+  int64_t kend = kbeg + g.kper;
+  if (kend > g.K) kend = g.K;
+  for (int64_t k0 = kbeg + 32 * w; k0 < kend; k0 += 128) {
+    const int64_t ko = k0 + 8 * kg;
+    float va[2][8], vb[2][8], vg[8];
+    for (int j = 0; j < 8; ++j) {
+      const int64_t kk = ko + j;
+      const bool ok = kk < kend;
+      vg[j] = ok ? G[kk] : 0.f;
+      for (int h = 0; h < 2; ++h) {
+        va[h][j] = ok ? pa[h][kk] : 0.f;
+        vb[h][j] = ok ? (g.same ? va[h][j] : pb[h][kk]) : 0.f;
+      }
+    }
+    long ap[2][3], bp[2][3];
+    for (int h = 0; h < 2; ++h) {
+      float ta[8], tb[8];
+      for (int j = 0; j < 8; ++j) {
+        ta[j] = va[h][j] * g.sa;
+        tb[j] = vb[h][j] * vg[j] * g.sb;
+      }
+      split8_fp8(ta, ap[h]);
+      split8_fp8(tb, bp[h]);
+    }
+    for (int i = 0; i < 2; ++i)
+      for (int j = 0; j < 2; ++j) {
+        const f32x4 z = {0.f, 0.f, 0.f, 0.f};
+        f32x4 c0 = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(ap[i][0], bp[j][0], z, 0, 0, 0);
+        f32x4 c1 = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(ap[i][0], bp[j][1], z, 0, 0, 0);
+        c1 = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(ap[i][1], bp[j][0], c1, 0, 0, 0);
+        f32x4 c2 = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(ap[i][0], bp[j][2], z, 0, 0, 0);
+        c2 = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(ap[i][1], bp[j][1], c2, 0, 0, 0);
+        c2 = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(ap[i][2], bp[j][0], c2, 0, 0, 0);
+        for (int q = 0; q < 4; ++q) acc[i][j][q] += c0[q] + c1[q] * 0.0625f + c2[q] * 0.00390625f;
+      }
+  }
+  block_reduce_store(acc, g.partial + (int64_t)blockIdx.x * 1024);
+}
+
 // ------------------------------------------------------------- gemm_f64
 __global__ void __launch_bounds__(256) gemm_f64_kernel(GemmArgs g) {
   const double* __restrict__ G = (const double*)g.G;
@@ -263,7 +351,7 @@ hipError_t launch_gemm_finalize(const double* partial, int nblk, int a0, int b0,
   return hipGetLastError();
 }
 
-// mode: 0 bf16x3, 1 f16x3 (FP32 storage); 2 f64 (FP64 storage)
+// mode: 0 bf16x3, 1 f16x3, 3 fp8x6 (FP32 storage); 2 f64 (FP64 storage)
 hipError_t launch_gemm(int mode, const void* G, const void* A, int64_t lda, int na, const void* B, int64_t ldb, int nb,
                        int same, int64_t K, float sa, float sb, int nblk, double* partial, double* C, hipStream_t st) {
   const int step = (mode == 2) ? 64 : 128;  // K per workgroup iteration
@@ -280,6 +368,8 @@ hipError_t launch_gemm(int mode, const void* G, const void* A, int64_t lda, int 
         hipLaunchKernelGGL(gemm_x3_kernel<false>, dim3(nblk_eff), dim3(256), 0, st, g);
       else if (mode == 1)
         hipLaunchKernelGGL(gemm_x3_kernel<true>, dim3(nblk_eff), dim3(256), 0, st, g);
+      else if (mode == 3)
+        hipLaunchKernelGGL(gemm_fp8x6_kernel, dim3(nblk_eff), dim3(256), 0, st, g);
       else
         hipLaunchKernelGGL(gemm_f64_kernel, dim3(nblk_eff), dim3(256), 0, st, g);
       const double inv = 1.0 / ((double)sa * (double)sb);

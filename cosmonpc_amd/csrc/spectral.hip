@@ -16,32 +16,52 @@ namespace cnpc {
 // ------------------------------------------------------------- prepare
 // Interlace combine, scaling and the compensation comp_factor() codes.
 
+// One axis' share of the compensation and of the interlacing phase, for
+// every index i (wrapped frequency n_d): comp(n_d) with comp_factor =
+// comp(nx) comp(ny) comp(nz), and e^{i/2 n_d kf H}, the factors of the
+// phase e^{+i/2 sum_d k_d H} (mesh_generator.py:212-218).
+__global__ void prep_axis_kernel(PrepareArgs a, double* __restrict__ comp1d, double2* __restrict__ ph1d) {
+  const int n = a.n;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int nd = wrap_freq(i, n);
+    comp1d[i] = comp_factor(a.comp, a.p, nd, 0, 0, n);
+    double sn, cs;
+    sincos(0.5 * ((double)nd * a.kf * a.H), &sn, &cs);
+    ph1d[i] = make_double2(cs, sn);
+  }
+}
+
+// one workgroup per (ix, iy) row of the half spectrum; the per-axis factors
+// come from the tables above (cached), so the kernel streams at HBM rate
 template <typename T>
-__global__ void __launch_bounds__(256) prepare_kernel(PrepareArgs a, const typename cplx<T>::type* __restrict__ c1,
+__global__ void __launch_bounds__(256) prepare_kernel(PrepareArgs a, const double* __restrict__ comp1d,
+                                                      const double2* __restrict__ ph1d,
+                                                      const typename cplx<T>::type* __restrict__ c1,
                                                       const typename cplx<T>::type* __restrict__ c2,
                                                       typename cplx<T>::type* __restrict__ out) {
   const int n = a.n, nzh = n / 2 + 1;
-  const int64_t total = (int64_t)n * n * nzh;
-  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (int64_t)gridDim.x * blockDim.x) {
-    const int iz = (int)(idx % nzh);
-    const int64_t r = idx / nzh;
-    const int iy = (int)(r % n), ix = (int)(r / n);
-    const int nx = wrap_freq(ix, n), ny = wrap_freq(iy, n), nz = wrap_freq(iz, n);
+  const int64_t row = blockIdx.x;
+  const int ix = (int)(row / n), iy = (int)(row - (int64_t)ix * n);
+  const double cxy = comp1d[ix] * comp1d[iy];
+  const double2 px = ph1d[ix], py = ph1d[iy];
+  const double pxr = px.x * py.x - px.y * py.y, pxi = px.x * py.y + px.y * py.x;
+  const bool row0 = (ix == 0 && iy == 0);
+  for (int iz = threadIdx.x; iz < nzh; iz += blockDim.x) {
+    const int64_t idx = row * nzh + iz;
     double re = (double)c1[idx].x, im = (double)c1[idx].y;
     if (a.interlaced) {
       // c = c1/2 + c2/2 * exp(+i/2 sum_d k_d H)   (mesh_generator.py:212-218)
-      const double kH = (double)nx * a.kf * a.H + (double)ny * a.kf * a.H + (double)nz * a.kf * a.H;
-      double sn, cs;
-      sincos(0.5 * kH, &sn, &cs);
+      const double2 pz = ph1d[iz];
+      const double cs = pxr * pz.x - pxi * pz.y, sn = pxr * pz.y + pxi * pz.x;
       const double r2 = (double)c2[idx].x, i2 = (double)c2[idx].y;
       re = re * 0.5 + 0.5 * (r2 * cs - i2 * sn);
       im = im * 0.5 + 0.5 * (r2 * sn + i2 * cs);
     }
     re *= a.scale;
     im *= a.scale;
-    const bool k0 = (nx == 0 && ny == 0 && nz == 0);
+    const bool k0 = row0 && iz == 0;
     if (k0) re -= a.sub0;
-    const double f = comp_factor(a.comp, a.p, nx, ny, nz, n);
+    const double f = cxy * comp1d[iz];
     re *= f;
     im *= f;
     if (k0 && a.zero0) { re = 0.0; im = 0.0; }
@@ -52,13 +72,18 @@ __global__ void __launch_bounds__(256) prepare_kernel(PrepareArgs a, const typen
   }
 }
 
-hipError_t launch_prepare(int prec, const PrepareArgs& a, const void* c1, const void* c2, void* out, hipStream_t st) {
-  const int64_t total = (int64_t)a.n * a.n * (a.n / 2 + 1);
-  dim3 blk(256), grd(grid_for(total, 256, 1 << 16));
+hipError_t launch_prepare(int prec, const PrepareArgs& a, const void* c1, const void* c2, void* out, void* tables,
+                          hipStream_t st) {
+  double* comp1d = (double*)tables;
+  double2* ph1d = (double2*)(comp1d + ((a.n + 1) & ~1));
+  hipLaunchKernelGGL(prep_axis_kernel, dim3((a.n + 255) / 256), dim3(256), 0, st, a, comp1d, ph1d);
+  const int64_t rows = (int64_t)a.n * a.n;
   if (prec == 0)
-    hipLaunchKernelGGL(prepare_kernel<double>, grd, blk, 0, st, a, (const double2*)c1, (const double2*)c2, (double2*)out);
+    hipLaunchKernelGGL(prepare_kernel<double>, dim3((unsigned)rows), dim3(256), 0, st, a, comp1d, ph1d,
+                       (const double2*)c1, (const double2*)c2, (double2*)out);
   else
-    hipLaunchKernelGGL(prepare_kernel<float>, grd, blk, 0, st, a, (const float2*)c1, (const float2*)c2, (float2*)out);
+    hipLaunchKernelGGL(prepare_kernel<float>, dim3((unsigned)rows), dim3(256), 0, st, a, comp1d, ph1d,
+                       (const float2*)c1, (const float2*)c2, (float2*)out);
   return hipGetLastError();
 }
 

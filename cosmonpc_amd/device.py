@@ -15,7 +15,7 @@ WINDOW = {"ngp": 1, "cic": 2, "tsc": 3, "pcs": 4}
 PREC = {"fp64": 0, "fp32": 1}
 REAL_DTYPE = {0: torch.float64, 1: torch.float32}
 CPLX_DTYPE = {0: torch.complex128, 1: torch.complex64}
-GEMM_MODE = {"bf16x3": 0, "fp16x3": 1, "gemm_fp64": 2}
+GEMM_MODE = {"bf16x3": 0, "fp16x3": 1, "gemm_fp64": 2, "fp8x6": 3}
 
 
 def _ptr(t):

@@ -9,7 +9,7 @@ fill, magnetic-config sums, analytic S3 and normalisation).
 GPU-only options (read from ``stat_attrs``; the reference ignores them):
   precision   : "fp64" (default) | "fp32"  -- carried by the painted fields
   contraction : "gemm_fp64" (fp64 default) | "pairwise" | "bf16x3"
-                (fp32 default) | "fp16x3"
+                (fp32 default) | "fp16x3" | "fp8x6"
   shell_method: "synth" (default, pruned shell synthesis) | "filter"
                 (shell filter + rocFFT 3-D c2r)
   zsynth      : "fused" (default: with bf16x3 the shells' z pass runs
@@ -138,14 +138,14 @@ class ShellBuilder:
 
 
 def _gemm_mode(contraction, prec):
-    if contraction in ("bf16x3", "fp16x3"):
+    if contraction in ("bf16x3", "fp16x3", "fp8x6"):
         if prec != PREC["fp32"]:
             raise ValueError(f"contraction '{contraction}' needs precision 'fp32'")
     elif contraction == "gemm_fp64":
         if prec != PREC["fp64"]:
             raise ValueError("contraction 'gemm_fp64' needs precision 'fp64'")
     elif contraction != "pairwise":
-        raise ValueError("contraction must be one of 'pairwise', 'gemm_fp64', 'bf16x3', 'fp16x3'")
+        raise ValueError("contraction must be one of 'pairwise', 'gemm_fp64', 'bf16x3', 'fp16x3', 'fp8x6'")
     return contraction
 
 
@@ -167,10 +167,13 @@ def _contract(ctx, mode, prec, G, R1, R2, pairs):
         C = np.zeros((nr, nc))
         C[ia, ib] = vals
         return C
-    if mode == "fp16x3":
-        sa = _pow2_scale(ctx.absmax(R1))
+    if mode in ("fp16x3", "fp8x6"):
+        # power-of-two prescales: fp16 halves stay normal below 2^14, the fp8
+        # pieces (e4m3) need |values| <= 2^7
+        target = 2.0 ** 14 if mode == "fp16x3" else 2.0 ** 7
+        sa = _pow2_scale(ctx.absmax(R1), target)
         sg = ctx.absmax(G)
-        sb = _pow2_scale(sg * ctx.absmax(R1 if R2 is None else R2))
+        sb = _pow2_scale(sg * ctx.absmax(R1 if R2 is None else R2), target)
         return ctx.contract_gemm(mode, G, R1, R2, sa, sb).cpu().numpy()
     return ctx.contract_gemm(mode, G, R1, R2).cpu().numpy()
 

@@ -21,7 +21,7 @@ O(k_bins^2) bookkeeping.
 GPU-only options (from ``stat_attrs`` / kwargs; the reference ignores them):
   precision   : "fp64" (default) | "fp32"
   contraction : "gemm_fp64" (fp64 default) | "pairwise" | "bf16x3" (fp32
-                default) | "fp16x3"
+                default) | "fp16x3" | "fp8x6"
 """
 import logging
 import math

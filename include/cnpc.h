@@ -161,7 +161,8 @@ int cnpc_contract_pairs(cnpc_ctx* ctx, int prec, int64_t ncell, const void* G, c
                         int npairs, const int32_t* ia, const int32_t* ib, double* out);
 
 /* GEMM contraction C[a][b] = sum_x A_a G B_b for a < na, b < nb (device FP64
- * C, row-major na x nb).  mode: 0 split-bf16 x3, 1 split-fp16 x3 (FP32
+ * C, row-major na x nb).  mode: 0 split-bf16 x3, 1 split-fp16 x3, 3 split-fp8
+ * (three e4m3 pieces, six products; prescaled |values| <= 2^7) (FP32
  * storage, sa/sb power-of-two prescales), 2 FP64 MFMA (FP64 storage).
  * same != 0 declares B == A (na == nb).  nblk: workgroups (0 = default).
  * Same sum as cnpc_contract_pairs for all pairs at once; the whole block

@@ -69,7 +69,8 @@ def test_paint_wraps_and_empty(gpu_ctx):
 def test_paint_tiled_matches_scatter(gpu_ctx, sampler):
     """Tiled paint (cnpc_paint at n % 8 == 0) == scatter paint (the batch
     entry with one catalog) on a dense clump inside one tile, particles on
-    tile and box edges and a uniform background; FP64 to 1e-13 of max."""
+    tile and box edges and a uniform background; FP64 to 1e-13 of max, FP32
+    to 5e-5 (summation order of ~3000 weights per cell)."""
     import torch
 
     n, L = 64, 640.0
@@ -85,7 +86,9 @@ def test_paint_tiled_matches_scatter(gpu_ctx, sampler):
         ctx.paint(pd, wd, n, L, sampler, True, prec, a0, a1)
         b0, b1 = ctx.mesh(n, prec, 1), ctx.mesh(n, prec, 1)
         ctx.paint_batch(pd, wd, [0, len(pos)], n, L, sampler, True, prec, b0, b1)
-        tol = 1e-13 if prec == 0 else 2e-6
+        # FP32: the two paths add the ~3000 clump weights of a cell in different
+        # orders (atomics), so they agree to ~N eps, not to eps
+        tol = 1e-13 if prec == 0 else 5e-5
         for a, b in ((a0, b0[0]), (a1, b1[0])):
             a, b = a.double().cpu().numpy(), b.double().cpu().numpy()
             assert np.max(np.abs(a - b)) <= tol * np.max(np.abs(b)), (sampler, prec)
@@ -366,13 +369,16 @@ def test_bk_box_blocked_matches_full(gpu_ctx):
 
 
 @pytest.mark.parametrize("angu", [(0, 0, 0), (2, 0, 2)])
-@pytest.mark.parametrize("contraction", ["bf16x3", "fp16x3"])
+@pytest.mark.parametrize("contraction", ["bf16x3", "fp16x3", "fp8x6"])
 def test_bk_box_fp32_split_error(gpu_ctx, angu, contraction):
+    """Split-precision contractions against the FP64 oracle, relative to
+    max|B|: 1e-4 for the bf16 / fp16 splits (three 8-11 bit pieces' worth);
+    1e-3 for the fp8 split (three 4-bit pieces)."""
     got, ref = _bk_case(gpu_ctx, angu, "full", contraction, prec=1, n=48, L=800.0, k=(0.01, 0.09, 8),
                         npart=60000)
     err = rel_to_max(got["Bk_raw"], ref["Bk_raw"])
     _log(f"{contraction} {angu} error rel max|B| = {err:.3e}")
-    assert err < 1e-4
+    assert err < (1e-3 if contraction == "fp8x6" else 1e-4)
 
 
 def test_gemm_modes_agree_with_pairwise(gpu_ctx):
