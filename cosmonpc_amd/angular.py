@@ -243,8 +243,24 @@ def get_valid_k3_bins(k1, k2, k_min, k_max, kbin):
     return np.logical_and(c >= abs(k1 - k2), c <= (k1 + k2))
 
 
+_Q_CACHE = {}
+
+
 def get_q_ells_eff(i, j, k_eff, ell_1, ell_2, L, k3_bins, k3_eff):
-    """q_{l1 l2 L}(k1, k2, k3) at effective k (``math_evaluator.py:592-616``)."""
+    """q_{l1 l2 L}(k1, k2, k3) at effective k (``math_evaluator.py:592-616``).
+    Memoised on its arguments: k_eff depends only on the grid and the bins,
+    so every mock of a campaign asks for the same values."""
+    key = (i, j, ell_1, ell_2, L, k3_bins, np.asarray(k_eff, dtype=np.float64).tobytes(),
+           np.asarray(k3_eff, dtype=np.float64).tobytes())
+    hit = _Q_CACHE.get(key)
+    if hit is None:
+        if len(_Q_CACHE) > 65536:
+            _Q_CACHE.clear()
+        hit = _Q_CACHE[key] = _q_ells_eff(i, j, k_eff, ell_1, ell_2, L, k3_bins, k3_eff)
+    return hit.copy()
+
+
+def _q_ells_eff(i, j, k_eff, ell_1, ell_2, L, k3_bins, k3_eff):
     q = np.zeros(k3_bins)
     lmin = min(ell_1, ell_2)
     for xx in range(-lmin, lmin + 1):

@@ -202,30 +202,73 @@ __global__ void __launch_bounds__(256) paint_key_kernel(const double* __restrict
   }
 }
 
-// exclusive prefix sum of count[0 .. m) into start[0 .. m], one workgroup of
-// 1024 threads (each thread a contiguous chunk)
-__global__ void __launch_bounds__(1024) paint_scan_kernel(const int* __restrict__ count, int m,
-                                                          int* __restrict__ start) {
+// exclusive prefix sum of count[0 .. m) into start[0 .. m] in three steps:
+// each workgroup scans 1024 counts (4 per thread) and leaves its total in
+// bsum; one workgroup scans the totals; each workgroup adds its offset
+constexpr int SCAN_TILE = 1024;
+
+__global__ void __launch_bounds__(256) paint_scan_tiles_kernel(const int* __restrict__ count, int m,
+                                                               int* __restrict__ start, int* __restrict__ bsum) {
+  __shared__ int part[256];
+  const int t = threadIdx.x, b = blockIdx.x;
+  const int i0 = b * SCAN_TILE + 4 * t;
+  int v[4], s = 0;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    v[j] = (i0 + j < m) ? count[i0 + j] : 0;
+    s += v[j];
+  }
+  part[t] = s;
+  __syncthreads();
+  for (int off = 1; off < 256; off <<= 1) {
+    const int x = t >= off ? part[t - off] : 0;
+    __syncthreads();
+    part[t] += x;
+    __syncthreads();
+  }
+  int run = part[t] - s;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    if (i0 + j < m) start[i0 + j] = run;
+    run += v[j];
+  }
+  if (t == 255) bsum[b] = part[255];
+}
+
+// in-place exclusive scan of bsum[0 .. nb) (one workgroup of 1024 threads,
+// contiguous chunks); bsum[nb] = total
+__global__ void __launch_bounds__(1024) paint_scan_top_kernel(int* __restrict__ bsum, int nb) {
   __shared__ int part[1024];
   const int t = threadIdx.x;
-  const int chunk = (m + 1023) / 1024;
-  const int b = t * chunk, e = min(m, b + chunk);
+  const int chunk = (nb + 1023) / 1024;
+  const int b = t * chunk, e = min(nb, b + chunk);
   int s = 0;
-  for (int i = b; i < e; ++i) s += count[i];
+  for (int i = b; i < e; ++i) s += bsum[i];
   part[t] = s;
   __syncthreads();
   for (int off = 1; off < 1024; off <<= 1) {
-    const int v = t >= off ? part[t - off] : 0;
+    const int x = t >= off ? part[t - off] : 0;
     __syncthreads();
-    part[t] += v;
+    part[t] += x;
     __syncthreads();
   }
-  int run = part[t] - s;  // exclusive prefix of this chunk
+  int run = part[t] - s;
   for (int i = b; i < e; ++i) {
-    start[i] = run;
-    run += count[i];
+    const int c = bsum[i];
+    bsum[i] = run;
+    run += c;
   }
-  if (t == 1023) start[m] = part[1023];
+  if (t == 1023) bsum[nb] = part[1023];
+}
+
+__global__ void __launch_bounds__(256) paint_scan_add_kernel(int* __restrict__ start, int m,
+                                                             const int* __restrict__ bsum, int nb) {
+  const int b = blockIdx.x, off = bsum[b];
+  for (int j = threadIdx.x; j < SCAN_TILE; j += blockDim.x) {
+    const int i = b * SCAN_TILE + j;
+    if (i < m) start[i] += off;
+  }
+  if (b == 0 && threadIdx.x == 0) start[m] = bsum[nb];
 }
 
 __global__ void __launch_bounds__(256) paint_scatter_kernel(const double* __restrict__ pos,
@@ -320,9 +363,9 @@ __global__ void __launch_bounds__(PAINT_TILE_THREADS) paint_tile_kernel(
 int paint_tiled_ok(int n) { return n % PT == 0 && n >= 2 * PT && n <= 2048; }
 
 size_t paint_tiled_work_bytes(int64_t np, int n) {
-  const int64_t nt = n / PT, m = nt * nt * nt;
+  const int64_t nt = n / PT, m = nt * nt * nt, nb = (m + SCAN_TILE - 1) / SCAN_TILE;
   auto al = [](size_t b) { return (b + 255) / 256 * 256; };
-  return al(np * 4) + 3 * al((m + 1) * 4) + al(np * 24) + al(np * 8);
+  return al(np * 4) + 3 * al((m + 1) * 4) + al((nb + 1) * 4) + al(np * 24) + al(np * 8);
 }
 
 template <typename T, int ORDER>
@@ -350,6 +393,9 @@ static hipError_t launch_paint_tiled_t(const double* pos, const double* w, int64
   base += al((size_t)(m + 1) * 4);
   int* start = (int*)base;
   base += al((size_t)(m + 1) * 4);
+  const int nb = (m + SCAN_TILE - 1) / SCAN_TILE;
+  int* bsum = (int*)base;
+  base += al((size_t)(nb + 1) * 4);
   double* spos = (double*)base;
   base += al(np * 24);
   double* sw = (double*)base;
@@ -365,7 +411,9 @@ static hipError_t launch_paint_tiled_t(const double* pos, const double* w, int64
       default: return hipErrorInvalidValue;
     }
   }
-  hipLaunchKernelGGL(paint_scan_kernel, dim3(1), dim3(1024), 0, st, count, m, start);
+  hipLaunchKernelGGL(paint_scan_tiles_kernel, dim3(nb), dim3(256), 0, st, count, m, start, bsum);
+  hipLaunchKernelGGL(paint_scan_top_kernel, dim3(1), dim3(1024), 0, st, bsum, nb);
+  hipLaunchKernelGGL(paint_scan_add_kernel, dim3(nb), dim3(256), 0, st, start, m, bsum, nb);
   if (np > 0)
     hipLaunchKernelGGL(paint_scatter_kernel, grd, blk, 0, st, pos, w, np, key, start, cursor, spos, sw);
   switch (order) {

@@ -101,6 +101,30 @@ class ShellBuilder:
             method = "filter"
         self.method = method
         self.work = ctx.spectrum(n, prec, batch)
+        self._tuned = False
+
+    def _tune_batch(self, delta, poly, bins, out, slot0):
+        """Shells per x/y synthesis launch: 4 or 8, timed per shell on the
+        first run of at least 8 bins (the autotuner caches the choice per
+        grid, radius and bin count)."""
+        self._tuned = True
+        if len(bins) < 8:
+            return
+        from . import autotune
+
+        shape = (self.n, int(math.ceil(self.edges[-1] / (2 * math.pi / self.L))), len(self.edges) - 1)
+        best = autotune.lookup("synth_half_batch", shape)
+        work8 = None
+        if best is None:
+            work8 = self.ctx.spectrum(self.n, 1, 8)
+
+            def run(c):
+                self.ctx.shell_synth_half(delta, self.n, self.L, self.edges, bins[0], c, poly, work8, out, slot0)
+
+            best = autotune.choose("synth_half_batch", shape, [4, 8], run, units=lambda c: c)
+        if best != self.batch:
+            self.batch = best
+            self.work = work8 if (best == 8 and work8 is not None) else self.ctx.spectrum(self.n, self.prec, best)
 
     def build(self, delta, poly, bins, out):
         """out[r] = shell of bins[r] (bins must be consecutive)."""
@@ -127,6 +151,8 @@ class ShellBuilder:
         b0 = bins[0]
         nb = len(bins)
         assert list(bins) == list(range(b0, b0 + nb))
+        if not self._tuned:
+            self._tune_batch(delta, poly, bins, out, slot0)
         done = 0
         rows = []
         while done < nb:
@@ -175,7 +201,16 @@ def _contract(ctx, mode, prec, G, R1, R2, pairs):
         sg = ctx.absmax(G)
         sb = _pow2_scale(sg * ctx.absmax(R1 if R2 is None else R2), target)
         return ctx.contract_gemm(mode, G, R1, R2, sa, sb).cpu().numpy()
-    return ctx.contract_gemm(mode, G, R1, R2).cpu().numpy()
+    return ctx.contract_gemm(mode, G, R1, R2, nblk=_gemm_blocks(ctx, mode, G, R1, R2)).cpu().numpy()
+
+
+def _gemm_blocks(ctx, mode, G, R1, R2):
+    """Split-K workgroups of the separate-path GEMM, autotuned per shape."""
+    from . import autotune
+
+    shape = (mode, G.numel(), R1.shape[0], 0 if R2 is None else R2.shape[0])
+    return autotune.choose("gemm_blocks", shape, [2048, 1024, 4096],
+                           lambda c: ctx.contract_gemm(mode, G, R1, R2, nblk=c))
 
 
 def _fit_block(k_bins, bytes_per_bin, ctx, reserve=0.25):

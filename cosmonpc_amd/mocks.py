@@ -121,6 +121,7 @@ def run_mocks(configs, catalogs, batch=16, skip_existing=True, arrays=None):
                 cfg = dict(c)
                 cfg["catalogs"] = {"data_a": path}
                 cfg["fields"] = (attrs, field)
+                cfg["gc_collect"] = False
                 res.append(run_task(**cfg))
             results[path] = res
         torch.cuda.synchronize(ctx.device)

@@ -75,12 +75,15 @@ def check_task(statistic, correlation_mode, geometry, **kwargs):
 def run_task(statistic, correlation_mode, geometry, catalogs, **kwargs):
     """``task_executor.py:11-328``.  ``kwargs`` may carry ``arrays`` (catalog
     arrays instead of files), the GPU options ``precision`` /
-    ``contraction``, and ``fields`` = (mesh attrs, field_a) of an already
-    painted box catalog (auto-correlation; used by the batched mock driver)."""
+    ``contraction``, ``fields`` = (mesh attrs, field_a) of an already
+    painted box catalog (auto-correlation; used by the batched mock driver)
+    and ``gc_collect`` (default True: the reference's gc pass after the pk
+    estimator, ``task_executor.py:115-116``)."""
     logging.basicConfig(level=logging.INFO, format="%(asctime)s - %(module)s - %(levelname)s - %(message)s")
     logging.info(f"Running {correlation_mode} {statistic} task with catalogs:{catalogs},\n")
     arrays = kwargs.pop("arrays", None)
     painted = kwargs.pop("fields", None)
+    collect = kwargs.pop("gc_collect", True)  # the batched mock driver skips the per-task gc pass
     if painted is not None and (geometry != "box-like" or correlation_mode != "auto"):
         raise ValueError("pre-painted fields are supported for box-like auto-correlation only")
     precision = kwargs.get("precision", "fp64")
@@ -119,7 +122,8 @@ def run_task(statistic, correlation_mode, geometry, catalogs, **kwargs):
         else:
             pk_res = calculate_power_spectrum_box(fa, fb, correlation_mode, stat_attrs, **kwargs)
         del fa, fb
-        gc.collect()
+        if collect:
+            gc.collect()
         pk_res.update(stat_attrs)
         if geometry == "box-like":
             for key in ("z_range", "scheme"):
@@ -163,7 +167,6 @@ def run_task(statistic, correlation_mode, geometry, catalogs, **kwargs):
         else:
             bk_res = calculate_bk_sugi_survey(fa, fb, fc, correlation_mode, stat_attrs, **kwargs)
         del fa, fb, fc
-        gc.collect()
         bk_res.update(stat_attrs)
         if geometry == "box-like":
             drop = ("z_range", "comp_weight_plan", "scheme", "alpha_scheme", "normalization_scheme")

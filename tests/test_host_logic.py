@@ -221,3 +221,24 @@ def test_run_mocks_validation_and_resume(tmp_path):
         np.save(out / f"bk_sugi_res_diagonal_000_auto_aaa_mocks_{os.path.splitext(os.path.basename(m))[0]}.npy", {})
     assert run_mocks([pk, bk], mocks) == {}
     assert LAST_TIMINGS["mocks"] == 0
+
+
+def test_autotune_cache_and_disable(tmp_path, monkeypatch):
+    """Disabled tuning returns the first candidate without running anything;
+    cached choices are looked up per shape and persist through the JSON
+    cache file."""
+    import json
+
+    from cosmonpc_amd import autotune
+
+    monkeypatch.setattr(autotune, "_CHOICES", {})
+    monkeypatch.setattr(autotune, "_LOADED", False)
+    path = tmp_path / "tune.json"
+    path.write_text(json.dumps({"gemm_blocks:(1, 2)": 4096}))
+    monkeypatch.setenv("CNPC_TUNE_CACHE", str(path))
+    monkeypatch.setenv("CNPC_TUNE", "0")
+    ran = []
+    assert autotune.lookup("gemm_blocks", (1, 2)) == 4096
+    assert autotune.choose("gemm_blocks", (1, 2), [2048, 1024], ran.append) == 4096
+    assert autotune.choose("synth_half_batch", (400, 99, 30), [4, 8], ran.append) == 4
+    assert ran == [] and autotune.lookup("synth_half_batch", (400, 99, 30)) is None
