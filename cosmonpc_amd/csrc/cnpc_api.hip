@@ -425,7 +425,7 @@ int cnpc_shell_synth(cnpc_ctx* ctx, int n, double boxsize, int prec, const void*
 
 int cnpc_shell_synth_half(cnpc_ctx* ctx, int n, double boxsize, const void* D, const double* edges, int nbins,
                           int bin0, int nshell, const cnpc_poly* weight, void* work, void* out, int nslots, int slot0,
-                          int* kz_rows) {
+                          int* kz_rows, int reuse_cube) {
   REQUIRE(ctx, "null context");
   REQUIRE(n >= 8 && n % 8 == 0 && boxsize > 0, "shell_synth_half: nmesh must be a multiple of 8");
   REQUIRE(nbins >= 1 && nbins <= 127 && edges, "shell_synth_half: nbins must be in [1, 127]");
@@ -451,6 +451,8 @@ int cnpc_shell_synth_half(cnpc_ctx* ctx, int n, double boxsize, const void* D, c
   // work holds U (nshell n P Q complex), at most nshell half spectra
   REQUIRE(P * Q <= (size_t)n * (n / 2 + 1), "shell_synth_half: k_max too large for the work area");
   const size_t vbytes = (cube * 8 + 255) / 256 * 256, bbytes = (cube + 255) / 256 * 256;
+  REQUIRE(!reuse_cube || ctx->synth_bytes >= vbytes + bbytes + (size_t)n * 8,
+          "shell_synth_half: reuse_cube without a previous cube of this shape");
   if (int rc = ensure_synth(ctx, vbytes + bbytes + (size_t)n * 8)) return rc;
   BinEdges E;
   fill_edges(E, edges, nbins);
@@ -458,7 +460,7 @@ int cnpc_shell_synth_half(cnpc_ctx* ctx, int n, double boxsize, const void* D, c
   to_spec(W, *weight);
   char* base = (char*)ctx->synth;
   CHECK_HIP(launch_synth_half(a, E, W, D, base, (signed char*)(base + vbytes), base + vbytes + bbytes, work, out,
-                              nslots, slot0, ctx->stream));
+                              nslots, slot0, reuse_cube ? 1 : 0, ctx->stream));
   return 0;
 }
 

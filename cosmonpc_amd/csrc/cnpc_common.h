@@ -297,7 +297,7 @@ inline int wide_radix(int R) { return R == 16 || R == 20 || R == 25; }
 int synth_lines_per_block(int prec, const SynthArgs& a);
 hipError_t launch_synth_half(const SynthArgs& a, const BinEdges& E, const PolySpec& W, const void* D, void* V,
                              signed char* bins, void* tw, void* work, void* out, int nslots, int slot0,
-                             hipStream_t st);
+                             int reuse_cube, hipStream_t st);
 hipError_t launch_twiddles(int n, void* tw, hipStream_t st);
 size_t zgemm_lds_bytes(int n, int nlines);
 int zgemm_supported(const SynthArgs& a, int na, int nb, int same, size_t lds_limit);

@@ -222,7 +222,10 @@ __device__ void build_pass_twiddles(C* __restrict__ Wt, const C* __restrict__ tw
 // buf[(i/p) p R + k + r p].  All reads are staged in registers before the
 // barrier, so one buffer suffices; a thread owns at most MAXB butterflies
 // (the host sizes nl accordingly).  Tw: this pass's twiddles.
-template <int RR, int MAXB, typename C>
+// PLANAR: write the pass's output as two planes per line, real parts at
+// floats [0, n) and imaginary parts at floats [ls, ls + n) of the line's
+// 2 ls floats (the consumer then reads runs of one part with vector loads).
+template <int RR, int MAXB, bool PLANAR = false, typename C>
 __device__ __forceinline__ void stockham_pass(C* __restrict__ buf, const C* __restrict__ Tw, int n, int ls, int nl,
                                               int p) {
   const int Tn = n / RR;
@@ -230,11 +233,12 @@ __device__ __forceinline__ void stockham_pass(C* __restrict__ buf, const C* __re
   const int total = nl * Tn;
   const float inv_Tn = 1.0f / (float)Tn, inv_p = 1.0f / (float)p;
   C u[MAXB][RR];
-  int dsto[MAXB];
+  int dsto[MAXB], dline[MAXB];
 #pragma unroll
   for (int it = 0; it < MAXB; ++it) {
     const int t = threadIdx.x + it * blockDim.x;
     dsto[it] = -1;
+    dline[it] = 0;
     if (t < total) {
       const int l = fdiv(t, Tn, inv_Tn), i = t - l * Tn;
       const int q = fdiv(i, p, inv_p), k = i - q * p;
@@ -245,7 +249,8 @@ __device__ __forceinline__ void stockham_pass(C* __restrict__ buf, const C* __re
 #pragma unroll
         for (int r = 1; r < RR; ++r) u[it][r] = cmul(u[it][r], Tw[(r - 1) * p + k]);
       }
-      dsto[it] = l * ls + q * pR + k;
+      dsto[it] = q * pR + k;
+      dline[it] = l * ls;
     }
   }
   __syncthreads();
@@ -253,9 +258,19 @@ __device__ __forceinline__ void stockham_pass(C* __restrict__ buf, const C* __re
   for (int it = 0; it < MAXB; ++it) {
     if (dsto[it] >= 0) {
       SmallDft<RR>::run(u[it]);
-      C* d = buf + dsto[it];
+      if (PLANAR) {
+        typedef decltype(C().x) T;
+        T* f = (T*)buf + 2 * dline[it] + dsto[it];
 #pragma unroll
-      for (int b = 0; b < RR; ++b) d[b * p] = u[it][b];
+        for (int b = 0; b < RR; ++b) {
+          f[b * p] = u[it][b].x;
+          f[ls + b * p] = u[it][b].y;
+        }
+      } else {
+        C* d = buf + dline[it] + dsto[it];
+#pragma unroll
+        for (int b = 0; b < RR; ++b) d[b * p] = u[it][b];
+      }
     }
   }
   __syncthreads();
@@ -277,6 +292,33 @@ __device__ __forceinline__ void stockham_single_wide(C* buf, const C* Tw, int n,
   else if (R == 20) stockham_pass<20, 1>(buf, Tw, n, ls, nl, p);
   else if (R == 25) stockham_pass<25, 1>(buf, Tw, n, ls, nl, p);
   else stockham_single<MAXB>(buf, Tw, n, ls, nl, R, p);
+}
+
+// a last pass writing planar output (see stockham_pass)
+template <int MAXB, typename C>
+__device__ __forceinline__ void stockham_last_planar(C* buf, const C* Tw, int n, int ls, int nl, int R, int p) {
+  if (R == 16) stockham_pass<16, 1, true>(buf, Tw, n, ls, nl, p);
+  else if (R == 20) stockham_pass<20, 1, true>(buf, Tw, n, ls, nl, p);
+  else if (R == 25) stockham_pass<25, 1, true>(buf, Tw, n, ls, nl, p);
+  else if (R == 4) stockham_pass<4, MAXB, true>(buf, Tw, n, ls, nl, p);
+  else if (R == 2) stockham_pass<2, MAXB, true>(buf, Tw, n, ls, nl, p);
+  else if (R == 3) stockham_pass<3, MAXB, true>(buf, Tw, n, ls, nl, p);
+  else stockham_pass<5, MAXB, true>(buf, Tw, n, ls, nl, p);
+}
+
+// Whole inverse FFT of nl lines (composite radices admitted), the last pass
+// writing planar lines.
+template <int MAXB, typename C>
+__device__ void stockham_lds_planar(C* buf, const C* Wt, int n, int ls, int nl, const SynthArgs& a) {
+  int p = 1, off = 0;
+  for (int s = 0; s < a.nrad; ++s) {
+    if (s + 1 == a.nrad)
+      stockham_last_planar<MAXB>(buf, Wt + off, n, ls, nl, a.rad[s], p);
+    else
+      stockham_single_wide<MAXB>(buf, Wt + off, n, ls, nl, a.rad[s], p);
+    off += (a.rad[s] - 1) * p;
+    p *= a.rad[s];
+  }
 }
 
 

@@ -346,14 +346,16 @@ hipError_t launch_twiddles(int n, void* tw, hipStream_t st) {
 // half-synthesised layout H[x][y/2][nslots][R+1][2]
 hipError_t launch_synth_half(const SynthArgs& a, const BinEdges& E, const PolySpec& Wp, const void* D, void* V,
                              signed char* bins, void* tw, void* work, void* out, int nslots, int slot0,
-                             hipStream_t st) {
+                             int reuse_cube, hipStream_t st) {
   typedef float2 C;
   const int R = a.R, P = 2 * R + 1, Q = R + 1, n = a.n;
   const int64_t cube = (int64_t)P * Q * P;
   dim3 blk(256);
-  hipLaunchKernelGGL(twiddle_kernel<float>, dim3((n + 255) / 256), blk, 0, st, n, (C*)tw);
-  hipLaunchKernelGGL(cube_kernel<float>, dim3(grid_for(cube, 256, 1 << 14)), blk, 0, st, a, E, Wp, (const C*)D,
-                     (C*)V, bins);
+  if (!reuse_cube) {  // twiddles, w(khat) delta(k) and bins of the cube: shared by every shell
+    hipLaunchKernelGGL(twiddle_kernel<float>, dim3((n + 255) / 256), blk, 0, st, n, (C*)tw);
+    hipLaunchKernelGGL(cube_kernel<float>, dim3(grid_for(cube, 256, 1 << 14)), blk, 0, st, a, E, Wp, (const C*)D,
+                       (C*)V, bins);
+  }
   const int NL = synth_lines_per_block(1, a);
   const size_t lds = sizeof(C) * ((size_t)n + (size_t)NL * (n + 1));
   const int64_t planes = (int64_t)a.nshell * n;

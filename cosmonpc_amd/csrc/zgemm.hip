@@ -61,15 +61,14 @@ struct ZGemmArgs {
   double* partial;          // [gridDim.x][32 * 32]
 };
 
-// 8 consecutive z values of one part of one line (0 where !ok or z >= n)
-__device__ __forceinline__ void zg_row8(const float2* buf, int ls, int line, int part, bool ok, int z0, int n,
+// 8 consecutive z values of one part of one planar line (two 16-byte LDS
+// loads), 0 where !ok
+__device__ __forceinline__ void zg_row8(const float2* buf, int ls, int line, int part, bool ok, int z0,
                                         float (&v)[8]) {
-  const float* p = (const float*)(buf + line * ls + z0) + part;
-#pragma unroll
-  for (int j = 0; j < 8; ++j) {
-    const float c = p[2 * j];
-    v[j] = (ok && z0 + j < n) ? c : 0.f;
-  }
+  const float4* p = (const float4*)((const float*)buf + 2 * line * ls + part * ls + z0);
+  const float4 a = p[0], b = p[1];
+  v[0] = ok ? a.x : 0.f; v[1] = ok ? a.y : 0.f; v[2] = ok ? a.z : 0.f; v[3] = ok ? a.w : 0.f;
+  v[4] = ok ? b.x : 0.f; v[5] = ok ? b.y : 0.f; v[6] = ok ? b.z : 0.f; v[7] = ok ? b.w : 0.f;
 }
 
 // One round = the lines of one FFT and the MFMA chunks they feed.  B == A
@@ -79,7 +78,8 @@ __device__ __forceinline__ void zg_row8(const float2* buf, int ls, int line, int
 // (line l holds A_l and B_l), nchr chunks.
 template <int THREADS, bool BR>
 __global__ void __launch_bounds__(THREADS) zgemm_bf16x3_kernel(ZGemmArgs g) {
-  const int n = g.n, Q = g.Q, R = Q - 1, ls = n + 1, half = n >> 1, L = g.nlines;
+  // lines of n + 4 complex: 16-byte aligned planes for the planar last pass
+  const int n = g.n, Q = g.Q, R = Q - 1, ls = n + 4, half = n >> 1, L = g.nlines;
   extern __shared__ unsigned char smem[];
   float2* W = (float2*)smem;
   float* Gs = (float*)(W + n);          // G rows y, y + 1 of the tile
@@ -160,16 +160,20 @@ __global__ void __launch_bounds__(THREADS) zgemm_bf16x3_kernel(ZGemmArgs g) {
         }
       }
       __syncthreads();
-      stockham_lds<ZG_MAXB, true>(buf, W, n, ls, L, g.fft);
+      stockham_lds_planar<ZG_MAXB>(buf, W, n, ls, L, g.fft);
       const int nchunk = (g.same && BR) ? 2 * nchr : nchr;
       for (int c = w; c < nchunk; c += nw) {
         const int r = (g.same && BR) ? (c >= nchr ? 1 : 0) : r0;
         const int z0 = 32 * (c - ((g.same && BR && r) ? nchr : 0)) + 8 * kg;
-        const bool zin = z0 < n;
+        const bool zin = z0 < n;      // n % 8 == 0: a group of 8 is all in or all out
         const int zr = zin ? z0 : 0;  // LDS reads stay inside the line
         float vg[8];
-#pragma unroll
-        for (int j = 0; j < 8; ++j) vg[j] = (zin && z0 + j < n) ? Gs[r * n + zr + j] : 0.f;
+        {
+          const float4* gp = (const float4*)(Gs + r * n + zr);
+          const float4 g0 = gp[0], g1 = gp[1];
+          vg[0] = zin ? g0.x : 0.f; vg[1] = zin ? g0.y : 0.f; vg[2] = zin ? g0.z : 0.f; vg[3] = zin ? g0.w : 0.f;
+          vg[4] = zin ? g1.x : 0.f; vg[5] = zin ? g1.y : 0.f; vg[6] = zin ? g1.z : 0.f; vg[7] = zin ? g1.w : 0.f;
+        }
         bf16x8 ah[2], al[2], bh[2], bl[2];
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
@@ -177,13 +181,13 @@ __global__ void __launch_bounds__(THREADS) zgemm_bf16x3_kernel(ZGemmArgs g) {
           float a[8], b[8];
           if (g.same) {
             const int line = (BR ? r * Lh : 0) + (i >> 1 < Lh ? i >> 1 : Lh - 1);
-            zg_row8(buf, ls, line, i & 1, zin && i < g.na, zr, n, a);
+            zg_row8(buf, ls, line, i & 1, zin && i < g.na, zr, a);
 #pragma unroll
             for (int j = 0; j < 8; ++j) b[j] = a[j] * vg[j];
           } else {
             const int line = i < L ? i : L - 1;
-            zg_row8(buf, ls, line, 0, zin && i < g.na, zr, n, a);
-            zg_row8(buf, ls, line, 1, zin && i < g.nb, zr, n, b);
+            zg_row8(buf, ls, line, 0, zin && i < g.na, zr, a);
+            zg_row8(buf, ls, line, 1, zin && i < g.nb, zr, b);
 #pragma unroll
             for (int j = 0; j < 8; ++j) b[j] *= vg[j];
           }
@@ -223,7 +227,7 @@ static int zg_threads(int same) { return (same && ZG_SAME_PER_ROW) ? 384 : ZG_TH
 
 // twiddles, the tile's two G rows and the lines
 size_t zgemm_lds_bytes(int n, int nlines) {
-  return sizeof(float2) * (size_t)n + sizeof(float) * 2 * (size_t)n + sizeof(float2) * (size_t)nlines * (n + 1);
+  return sizeof(float2) * (size_t)n + sizeof(float) * 2 * (size_t)n + sizeof(float2) * (size_t)nlines * (n + 4);
 }
 
 // host check: every pass of the line FFT fits the staged registers and the

@@ -185,10 +185,12 @@ class GpuContext:
         (complex64)."""
         return torch.empty((n, n // 2, nslots, R + 1, 2), dtype=torch.complex64, device=self.device)
 
-    def shell_synth_half(self, delta, n, boxsize, edges, bin0, nshell, poly, work, out, slot0):
+    def shell_synth_half(self, delta, n, boxsize, edges, bin0, nshell, poly, work, out, slot0, reuse_cube=False):
         """x and y passes of the FP32 shell synthesis of bins bin0 .. bin0 +
         nshell - 1 into slots slot0 .. of ``out`` (from ``half_shells``).
-        Returns the kz rows written per shell (numpy int32)."""
+        ``reuse_cube``: same delta, poly and edges as the previous synthesis
+        call (the mode cube is kept).  Returns the kz rows written per shell
+        (numpy int32)."""
         edges = np.ascontiguousarray(edges, dtype=np.float64)
         R = self.synth_radius(boxsize, edges[-1])
         if delta is not None:
@@ -201,7 +203,8 @@ class GpuContext:
         check(self.lib.cnpc_shell_synth_half(self.h, n, float(boxsize), _ptr(delta),
                                              edges.ctypes.data_as(ctypes.c_void_p), len(edges) - 1, int(bin0),
                                              int(nshell), ctypes.byref(poly), _ptr(work), _ptr(out),
-                                             out.shape[2], int(slot0), rows.ctypes.data_as(ctypes.c_void_p)),
+                                             out.shape[2], int(slot0), rows.ctypes.data_as(ctypes.c_void_p),
+                                             int(bool(reuse_cube))),
               "cnpc_shell_synth_half")
         return rows
 

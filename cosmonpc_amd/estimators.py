@@ -157,8 +157,9 @@ class ShellBuilder:
         rows = []
         while done < nb:
             c = min(self.batch, nb - done)
+            # the mode cube of this delta and weight is built by the first batch
             rows.append(self.ctx.shell_synth_half(delta, self.n, self.L, self.edges, b0 + done, c, poly, self.work,
-                                                  out, slot0 + done))
+                                                  out, slot0 + done, reuse_cube=done > 0))
             done += c
         return np.concatenate(rows)
 

@@ -147,11 +147,13 @@ int cnpc_shell_synth(cnpc_ctx* ctx, int nmesh, double boxsize, int prec, const v
  * entries), the input of cnpc_contract_zgemm.  A shell's modes have
  * kz < kz_rows[s] (its own radius): only those rows of its slot are written.
  * kz_rows: host array of nshell ints, or NULL.  work: nshell half spectra.
- * n must be a multiple of 8.  Same shells as clustering_estimator.py:934-970
- * before their z transform. */
+ * reuse_cube != 0 skips rebuilding the weighted, binned mode cube: the caller
+ * guarantees the same delta_k contents, weight, edges and grid as the
+ * previous synthesis call on this context.  n must be a multiple of 8.
+ * Same shells as clustering_estimator.py:934-970 before their z transform. */
 int cnpc_shell_synth_half(cnpc_ctx* ctx, int nmesh, double boxsize, const void* delta_k, const double* edges,
                           int nbins, int bin0, int nshell, const cnpc_poly* weight, void* work, void* out,
-                          int nslots, int slot0, int* kz_rows);
+                          int nslots, int slot0, int* kz_rows, int reuse_cube);
 
 /* Pairwise plain contraction: out[p] = sum_x G A_{ia[p]} B_{ib[p]}, one full
  * streaming pass per pair (the np.sum(G_00 * F_i * F_j) of
