@@ -205,6 +205,17 @@ def _contract(ctx, mode, prec, G, R1, R2, pairs):
     return ctx.contract_gemm(mode, G, R1, R2, nblk=_gemm_blocks(ctx, mode, G, R1, R2)).cpu().numpy()
 
 
+def _zgemm_blocks(ctx, zargs):
+    """Workgroups of the fused contraction (1, 2 or 4 per CU), autotuned per
+    grid and row counts."""
+    from . import autotune
+
+    n, rows_a, rows_b, same = zargs[0], zargs[5], zargs[8], zargs[7] is None
+    ncu = torch.cuda.get_device_properties(ctx.device).multi_processor_count
+    return autotune.choose("zgemm_blocks", (n, rows_a, rows_b, same), [ncu, 2 * ncu, 4 * ncu],
+                           lambda c: ctx.contract_zgemm(*zargs, nblk=c))
+
+
 def _gemm_blocks(ctx, mode, G, R1, R2):
     """Split-K workgroups of the separate-path GEMM, autotuned per shape."""
     from . import autotune
@@ -318,9 +329,9 @@ def _config_matrix(ctx, builder, src1, src2, Gs, kinfo, ell_1, m1, ell_2, m2, sy
             C = 0j
             for G, gph in Gs:
                 if fused:
-                    Cm = ctx.contract_zgemm(n, L, kmax, G, R1[0], nc1 * len(ib), R1[1],
-                                            None if R2 is None else R2[0], nc2 * rows2,
-                                            None if R2 is None else R2[1]).cpu().numpy()
+                    zargs = (n, L, kmax, G, R1[0], nc1 * len(ib), R1[1], None if R2 is None else R2[0],
+                             nc2 * rows2, None if R2 is None else R2[1])
+                    Cm = ctx.contract_zgemm(*zargs, nblk=_zgemm_blocks(ctx, zargs)).cpu().numpy()
                 else:
                     Cm = _contract(ctx, mode, prec, G, R1, R2, pairs)
                 C = C + gph * Cm
