@@ -44,6 +44,7 @@ def parse():
     p.add_argument("--k-bins", type=int, default=30)
     p.add_argument("--no-fp64", action="store_true", help="skip the FP64 reference paths")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    p.add_argument("--no-graph", action="store_true", help="time the eager launches only (no hipGraph replay)")
     p.add_argument("--roofline-only", action="store_true")
     p.add_argument("--workload", choices=("metric", "mocks", "kernels"), default="metric",
                    help="mocks: 64 box mocks (pk + bk_sugi diagonal at 256^3), batched driver vs per-mock loop; "
@@ -70,7 +71,7 @@ def main():
     from cosmonpc_amd import angular as A
     from cosmonpc_amd import synth
     from cosmonpc_amd.device import get_context
-    from cosmonpc_amd.estimators import bk_box_g00, bk_box_signal_auto, contraction_grid
+    from cosmonpc_amd.estimators import bk_box_g00, bk_box_signal_auto, bk_box_signal_multi, contraction_grid
     from cosmonpc_amd.mesh import paint_field
 
     n, L = args.nmesh, 2000.0
@@ -106,32 +107,90 @@ def main():
     del f32
     log(f"fp32 paint+r2c+prepare {t_paint32 * 1e3:.1f} ms")
 
-    def signal_step(delta, prec, mode, zsynth="fused"):
+    def enqueue_step(delta, prec, mode, zsynth="fused", shared=False):
         """G_00 + both configurations' signal matrices, on the contraction
-        grid the estimator uses (reduced for the GEMM modes)."""
+        grid the estimator uses (reduced for the GEMM modes): the device work
+        of every configuration is enqueued before any result is read; the
+        returned function reads them.  ``shared``: one multi-configuration
+        call that synthesises the unweighted shells once for both
+        configurations (they are [0,0,0]'s legs and [2,0,2]'s second leg)."""
         m, nmax = contraction_grid(n, L, kmax, mode)
         dm = ctx.resample(delta, n, m, prec, nmax) if m != n else delta
         G = bk_box_g00(ctx, dm, m, prec)
-        out = [(n / m) ** 3 * bk_box_signal_auto(ctx, dm, G, m, L, prec, k_edge, a, mode, zsynth=zsynth)
-               for a in angus]
-        del G, dm
-        return out
+        if shared:
+            fin = bk_box_signal_multi(ctx, dm, G, m, L, prec, k_edge, angus, mode, zsynth=zsynth, defer=True)
+        else:
+            fins = [bk_box_signal_auto(ctx, dm, G, m, L, prec, k_edge, a, mode, zsynth=zsynth, defer=True)
+                    for a in angus]
+            fin = lambda: [f() for f in fins]  # noqa: E731
+        return fin, (n / m) ** 3
+
+    def finish_step(pending):
+        fin, factor = pending
+        return [factor * r for r in fin()]
+
+    def signal_step(delta, prec, mode, zsynth="fused", shared=False):
+        return finish_step(enqueue_step(delta, prec, mode, zsynth, shared))
 
     grid_m = contraction_grid(n, L, kmax, "bf16x3")[0]
     log(f"contraction grid {grid_m}^3 for the GEMM modes")
-
-    def step():
-        return signal_step(d32, 1, "bf16x3")
-
-    for _ in range(args.warmup):
-        res_bf16 = step()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        res_bf16 = step()
-    torch.cuda.synchronize()
-    t_step = (time.perf_counter() - t0) / args.steps
     bins_per_step = npairs * len(angus)
+
+    def timed_steps(fn):
+        for _ in range(args.warmup):
+            res = fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            res = fn()
+        torch.cuda.synchronize()
+        return res, (time.perf_counter() - t0) / args.steps
+
+    def rel_diff(a, b):
+        return max(float(np.max(np.abs(x - y)) / np.max(np.abs(y))) for x, y in zip(a, b))
+
+    # each configuration on its own (as one run_stats per configuration does)
+    res_indep, t_indep = timed_steps(lambda: signal_step(d32, 1, "bf16x3"))
+    log(f"bf16x3 signal stage, configurations apart {t_indep * 1e3:.1f} ms/step -> "
+        f"{bins_per_step / t_indep:.1f} bins/s")
+    # both configurations in one call, the shared leg synthesised once
+    res_eager, t_eager = timed_steps(lambda: signal_step(d32, 1, "bf16x3", shared=True))
+    share_diff = rel_diff(res_eager, res_indep)
+    log(f"bf16x3 signal stage, shared legs {t_eager * 1e3:.1f} ms/step -> {bins_per_step / t_eager:.1f} bins/s "
+        f"(max rel diff vs apart {share_diff:.1e})")
+    graph_info = {"configs_apart_ms_per_step": t_indep * 1e3, "configs_apart_bins_per_s": bins_per_step / t_indep,
+                  "shared_legs_eager_ms_per_step": t_eager * 1e3, "shared_vs_apart_max_rel_diff": share_diff}
+    if share_diff > 1e-12:
+        res_eager, t_eager = res_indep, t_indep
+
+    # the step recorded once as a hipGraph and replayed (the input spectrum
+    # d32 stays where it is; every replay recomputes G_00, the shells and the
+    # contractions from it)
+    res_bf16, t_step = res_eager, t_eager
+    if not args.no_graph:
+        from cosmonpc_amd.graphs import StepGraph
+
+        sg = StepGraph(ctx)
+        try:
+            pending = sg.record(lambda: enqueue_step(d32, 1, "bf16x3", shared=share_diff <= 1e-12), finish_step)
+        except Exception as e:  # recorded nothing: report the eager step
+            graph_info["graph_error"] = f"{type(e).__name__}: {e}"
+            log(f"graph recording failed: {graph_info['graph_error']}")
+            sg = None
+        if sg is not None:
+            def replay():
+                sg.replay()
+                return finish_step(pending)
+
+            res_graph, t_graph = timed_steps(replay)
+            diff = rel_diff(res_graph, res_eager)
+            graph_info.update({"graph_ms_per_step": t_graph * 1e3, "graph_vs_eager_max_rel_diff": diff})
+            log(f"bf16x3 signal stage, graph replay {t_graph * 1e3:.1f} ms/step -> "
+                f"{bins_per_step / t_graph:.1f} bins/s (max rel diff vs eager {diff:.1e})")
+            if diff <= 1e-12:
+                res_bf16, t_step = res_graph, t_graph
+            sg.close()
+            del pending
     value = bins_per_step / t_step
     log(f"bf16x3 signal stage {t_step * 1e3:.1f} ms/step -> {value:.1f} bins/s")
 
@@ -159,7 +218,7 @@ def main():
     del dm32
     log(f"kernels (grid {m}^3): {json.dumps(roof['stages'])}")
 
-    extra = {"separate_z_ms_per_step": t_sep * 1e3, "separate_z_bins_per_s": bins_per_step / t_sep,
+    extra = {**graph_info, "separate_z_ms_per_step": t_sep * 1e3, "separate_z_bins_per_s": bins_per_step / t_sep,
              "fp8x6_ms_per_step": t_fp8 * 1e3, "fp8x6_bins_per_s": bins_per_step / t_fp8}
     if not args.no_fp64:
         paint_field(ctx, pos_d, w_d, n, L, sampler, True, 0)

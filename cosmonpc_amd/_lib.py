@@ -43,6 +43,7 @@ SIGNATURES = {
     "cnpc_ctx_destroy": [_P],
     "cnpc_ctx_set_stream": [_P, _P],
     "cnpc_ctx_reserve": [_P, _L],
+    "cnpc_ctx_hold": [_P, _I],
     "cnpc_ctx_clear_plans": [_P],
     "cnpc_paint": [_P, _P, _P, _L, _I, _D, _I, _I, _P, _I, _P, _P],
     "cnpc_paint_batch": [_P, _P, _P, _P, _I, _L, _I, _D, _I, _I, _P, _I, _P, _P],

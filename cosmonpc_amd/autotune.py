@@ -10,8 +10,10 @@ is kept for the rest of the process; with ``CNPC_TUNE_CACHE`` set to a file
 path the choices also persist across processes (JSON).
 
 Every candidate computes the same result, so tuning never changes numbers.
-``CNPC_TUNE=0`` disables tuning (first candidate).
+``CNPC_TUNE=0`` disables tuning (first candidate), and so does ``frozen()``
+(used while a graph is recorded: timing would synchronise the stream).
 """
+import contextlib
 import json
 import os
 
@@ -19,6 +21,19 @@ import torch
 
 _CHOICES = {}
 _LOADED = False
+_FROZEN = [False]
+
+
+@contextlib.contextmanager
+def frozen():
+    """Within this block ``choose`` times nothing: cached choices or the
+    first candidate."""
+    prev = _FROZEN[0]
+    _FROZEN[0] = True
+    try:
+        yield
+    finally:
+        _FROZEN[0] = prev
 
 
 def _cache_path():
@@ -71,7 +86,7 @@ def choose(name, shape, candidates, run, units=None):
     key = f"{name}:{shape}"
     if key in _CHOICES:
         return _CHOICES[key]
-    if len(candidates) == 1 or os.environ.get("CNPC_TUNE", "1") == "0":
+    if len(candidates) == 1 or _FROZEN[0] or os.environ.get("CNPC_TUNE", "1") == "0":
         return candidates[0]
     best, best_t = candidates[0], float("inf")
     for c in candidates:

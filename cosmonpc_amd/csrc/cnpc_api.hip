@@ -58,6 +58,11 @@ struct cnpc_ctx {
   size_t synth_bytes = 0;
   void* pwork = nullptr;    // tiled paint: keys, tile counts, sorted particles
   size_t pwork_bytes = 0;
+  // recorded graphs hold the addresses of the work areas and plans: while
+  // holds > 0 a work area that grows keeps its old allocation here (freed
+  // when the last hold is released) and the plan cache is not cleared
+  int holds = 0;
+  std::vector<void*> retired;
 };
 
 #define CHECK_HIP(expr)                                                                        \
@@ -77,14 +82,40 @@ struct cnpc_ctx {
     }                          \
   } while (0)
 
+// A work area cannot grow while the context's stream is being captured into
+// a graph (the allocation and the synchronisation before the free are not
+// stream work); the same calls made once before the capture size it.
+static int check_not_capturing(cnpc_ctx* ctx, const char* what) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(ctx->stream, &cs) != hipSuccess) {
+    (void)hipGetLastError();  // a stream the query does not cover is not recording
+    cs = hipStreamCaptureStatusNone;
+  }
+  REQUIRE(cs == hipStreamCaptureStatusNone,
+          std::string(what) + " while the stream is being captured into a graph; make the same calls once before "
+          "the capture");
+  return 0;
+}
+
+// Frees a work area that is being replaced, or keeps it while recorded
+// graphs hold the context's addresses.
+static int release_area(cnpc_ctx* ctx, void*& p) {
+  if (!p) return 0;
+  if (ctx->holds > 0) {
+    ctx->retired.push_back(p);
+  } else {
+    CHECK_HIP(hipStreamSynchronize(ctx->stream));
+    CHECK_HIP(hipFree(p));
+  }
+  p = nullptr;
+  return 0;
+}
+
 static int ensure_scratch(cnpc_ctx* ctx, size_t bytes) {
   if (ctx->scratch_bytes >= bytes) return 0;
-  if (ctx->scratch) {
-    CHECK_HIP(hipStreamSynchronize(ctx->stream));
-    CHECK_HIP(hipFree(ctx->scratch));
-    ctx->scratch = nullptr;
-    ctx->scratch_bytes = 0;
-  }
+  if (int rc = check_not_capturing(ctx, "the context's scratch area must grow")) return rc;
+  if (int rc = release_area(ctx, ctx->scratch)) return rc;
+  ctx->scratch_bytes = 0;
   CHECK_HIP(hipMalloc(&ctx->scratch, bytes));
   ctx->scratch_bytes = bytes;
   return 0;
@@ -92,12 +123,9 @@ static int ensure_scratch(cnpc_ctx* ctx, size_t bytes) {
 
 static int ensure_synth(cnpc_ctx* ctx, size_t bytes) {
   if (ctx->synth_bytes >= bytes) return 0;
-  if (ctx->synth) {
-    CHECK_HIP(hipStreamSynchronize(ctx->stream));
-    CHECK_HIP(hipFree(ctx->synth));
-    ctx->synth = nullptr;
-    ctx->synth_bytes = 0;
-  }
+  if (int rc = check_not_capturing(ctx, "the context's synthesis area must grow")) return rc;
+  if (int rc = release_area(ctx, ctx->synth)) return rc;
+  ctx->synth_bytes = 0;
   CHECK_HIP(hipMalloc(&ctx->synth, bytes));
   ctx->synth_bytes = bytes;
   return 0;
@@ -105,12 +133,9 @@ static int ensure_synth(cnpc_ctx* ctx, size_t bytes) {
 
 static int ensure_pwork(cnpc_ctx* ctx, size_t bytes) {
   if (ctx->pwork_bytes >= bytes) return 0;
-  if (ctx->pwork) {
-    CHECK_HIP(hipStreamSynchronize(ctx->stream));
-    CHECK_HIP(hipFree(ctx->pwork));
-    ctx->pwork = nullptr;
-    ctx->pwork_bytes = 0;
-  }
+  if (int rc = check_not_capturing(ctx, "the context's paint area must grow")) return rc;
+  if (int rc = release_area(ctx, ctx->pwork)) return rc;
+  ctx->pwork_bytes = 0;
   CHECK_HIP(hipMalloc(&ctx->pwork, bytes));
   ctx->pwork_bytes = bytes;
   return 0;
@@ -125,6 +150,7 @@ static int get_plan(cnpc_ctx* ctx, int n, int batch, int prec, int dir, FftPlan*
     *out = &it->second;
     return 0;
   }
+  if (int rc = check_not_capturing(ctx, "a new FFT plan must be created")) return rc;
   static std::once_flag once;
   std::call_once(once, [] { rocfft_setup(); });
   FftPlan p;
@@ -185,6 +211,7 @@ int cnpc_ctx_create(cnpc_ctx** out, int device) {
 
 int cnpc_ctx_clear_plans(cnpc_ctx* ctx) {
   REQUIRE(ctx, "null context");
+  REQUIRE(ctx->holds == 0, "cnpc_ctx_clear_plans: recorded graphs still hold the context's plans");
   CHECK_HIP(hipStreamSynchronize(ctx->stream));
   for (auto& kv : ctx->plans) {
     if (kv.second.plan) rocfft_plan_destroy(kv.second.plan);
@@ -197,7 +224,9 @@ int cnpc_ctx_clear_plans(cnpc_ctx* ctx) {
 
 int cnpc_ctx_destroy(cnpc_ctx* ctx) {
   if (!ctx) return 0;
+  ctx->holds = 0;
   int rc = cnpc_ctx_clear_plans(ctx);
+  for (void* p : ctx->retired) (void)hipFree(p);
   if (ctx->scratch) (void)hipFree(ctx->scratch);
   if (ctx->pwork) (void)hipFree(ctx->pwork);
   if (ctx->synth) (void)hipFree(ctx->synth);
@@ -208,6 +237,18 @@ int cnpc_ctx_destroy(cnpc_ctx* ctx) {
 int cnpc_ctx_set_stream(cnpc_ctx* ctx, void* stream) {
   REQUIRE(ctx, "null context");
   ctx->stream = (hipStream_t)stream;
+  return 0;
+}
+
+int cnpc_ctx_hold(cnpc_ctx* ctx, int delta) {
+  REQUIRE(ctx && (delta == 1 || delta == -1), "cnpc_ctx_hold: delta must be +1 or -1");
+  REQUIRE(ctx->holds + delta >= 0, "cnpc_ctx_hold: no hold to release");
+  ctx->holds += delta;
+  if (ctx->holds == 0 && !ctx->retired.empty()) {
+    CHECK_HIP(hipStreamSynchronize(ctx->stream));
+    for (void* p : ctx->retired) CHECK_HIP(hipFree(p));
+    ctx->retired.clear();
+  }
   return 0;
 }
 
@@ -569,6 +610,7 @@ int cnpc_contract_pairs(cnpc_ctx* ctx, int prec, int64_t ncell, const void* G, c
           "contract_pairs: bad arguments");
   if (npairs == 0) return 0;
   for (int p = 0; p < npairs; ++p) REQUIRE(ia[p] >= 0 && ib[p] >= 0, "contract_pairs: negative shell index");
+  if (int rc = check_not_capturing(ctx, "contract_pairs reads its sums back to the host and cannot run")) return rc;
   const int nblocks = 2048;
   if (int rc = ensure_scratch(ctx, (size_t)(nblocks + npairs) * 8)) return rc;
   double* partial = (double*)ctx->scratch;
@@ -606,6 +648,7 @@ int cnpc_contract_gemm(cnpc_ctx* ctx, int mode, int64_t ncell, const void* G, co
 
 int cnpc_absmax_f32(cnpc_ctx* ctx, const float* x, int64_t n, double* out) {
   REQUIRE(ctx && x && out && n > 0, "absmax: bad arguments");
+  if (int rc = check_not_capturing(ctx, "absmax reads its result back to the host and cannot run")) return rc;
   const int nblocks = 1024;
   if (int rc = ensure_scratch(ctx, (size_t)nblocks * 8)) return rc;
   hipLaunchKernelGGL(absmax_kernel, dim3(nblocks), dim3(256), 0, ctx->stream, x, n, (double*)ctx->scratch);

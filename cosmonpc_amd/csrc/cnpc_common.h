@@ -198,6 +198,7 @@ struct ShellArgs {
 #define CNPC_MAX_RADICES 24
 
 #define CNPC_SYNTH_MAXS 128
+#define CNPC_MAX_DEVICES 64  // per-device launch state kept by the library
 
 struct SynthArgs {
   int n;        // grid

@@ -246,10 +246,20 @@ int zgemm_blocks_per_cu(int same) { return (same && ZG_SAME_PER_ROW) ? 2 : 1; }
 
 template <int T, bool BR>
 static hipError_t zg_launch(const ZGemmArgs& g, int nblk, size_t lds, hipStream_t st) {
-  // dynamic LDS beyond the default 64 KB must be allowed per kernel
-  hipError_t e = hipFuncSetAttribute((const void*)zgemm_bf16x3_kernel<T, BR>,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  // dynamic LDS beyond the default 64 KB must be allowed per kernel and
+  // device; the allowance only grows, so repeated launches (and launches
+  // recorded into a graph after an eager one) make no attribute call
+  static size_t allowed[CNPC_MAX_DEVICES] = {};
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
   if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= CNPC_MAX_DEVICES) return hipErrorInvalidDevice;
+  if (lds > allowed[dev]) {
+    e = hipFuncSetAttribute((const void*)zgemm_bf16x3_kernel<T, BR>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)lds);
+    if (e != hipSuccess) return e;
+    allowed[dev] = lds;
+  }
   hipLaunchKernelGGL((zgemm_bf16x3_kernel<T, BR>), dim3(nblk), dim3(T), lds, st, g);
   return hipGetLastError();
 }

@@ -303,6 +303,10 @@ class GpuContext:
     def sync(self):
         check(self.lib.cnpc_sync(self.h), "cnpc_sync")
 
+    def reserve(self, nbytes):
+        """Pre-size the context's reduction scratch (``cnpc_ctx_reserve``)."""
+        check(self.lib.cnpc_ctx_reserve(self.h, int(nbytes)), "cnpc_ctx_reserve")
+
     # ---------------------------------------------------------------- survey (full grids)
     def full(self, n, prec, batch=None):
         shape = (n, n, n) if batch is None else (batch, n, n, n)

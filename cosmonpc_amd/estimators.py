@@ -225,12 +225,22 @@ def _gemm_blocks(ctx, mode, G, R1, R2):
                            lambda c: ctx.contract_gemm(mode, G, R1, R2, nblk=c))
 
 
+_FIT_SEEN = {}  # last block size per (k_bins, bytes per bin, reserve)
+_RECORDING = [False]  # set while a graph records: block sizes repeat the eager run's
+
+
 def _fit_block(k_bins, bytes_per_bin, ctx, reserve=0.25):
     """Largest block of bins whose shells fit in the free HBM (keeping a
-    fraction in reserve for FFT work areas and spectra)."""
+    fraction in reserve for FFT work areas and spectra).  While a graph is
+    being recorded the size of the eager run before it is reused, so the
+    recorded launches are the ones that ran."""
+    key = (k_bins, bytes_per_bin, reserve)
+    if _RECORDING[0] and key in _FIT_SEEN:
+        return _FIT_SEEN[key]
     free, total = torch.cuda.mem_get_info(ctx.device)
     budget = free - reserve * total
-    return int(max(1, min(k_bins, budget // max(bytes_per_bin, 1))))
+    _FIT_SEEN[key] = int(max(1, min(k_bins, budget // max(bytes_per_bin, 1))))
+    return _FIT_SEEN[key]
 
 
 def _weight_components(ell, m, conj_w):
@@ -240,8 +250,13 @@ def _weight_components(ell, m, conj_w):
     return [(p, (-1) ** m * ph) for p, ph in A.shell_components(ell, -m)]
 
 
-def _config_matrix(ctx, builder, src1, src2, Gs, kinfo, ell_1, m1, ell_2, m2, sym_eq, sym_cj, mode, block_size,
-                   dvm, conj_w=False, zsynth="fused"):
+def _config_matrix(*args, **kwargs):
+    """``_config_plan`` run to completion: the signal matrix (numpy)."""
+    return _config_plan(*args, **kwargs)()
+
+
+def _config_plan(ctx, builder, src1, src2, Gs, kinfo, ell_1, m1, ell_2, m2, sym_eq, sym_cj, mode, block_size,
+                 dvm, conj_w=False, zsynth="fused", shell_cache=None):
     """3j-free signal matrix of one magnetic configuration,
     S_ij = sum_x G F_i F'_j, with the reference's upper-triangle evaluation and
     lower fill (transpose for F' = F, conjugate for F' = +-conj F;
@@ -255,7 +270,16 @@ def _config_matrix(ctx, builder, src1, src2, Gs, kinfo, ell_1, m1, ell_2, m2, sy
     With the split-bf16 GEMM and ``zsynth="fused"`` the shells stay half
     synthesised (x and y passes) and their z pass runs inside the contraction
     (``cnpc_contract_zgemm``) whenever a block of rows fits it; otherwise the
-    real shells are built and contracted separately."""
+    real shells are built and contracted separately.
+
+    The device work is enqueued here; the returned ``finish()`` reads the
+    results back and assembles the matrix, and may be called again after the
+    same work is replayed from a graph.  The fused contraction leaves its
+    results on the device until then; the other paths read them back as they
+    go.  ``shell_cache`` (a dict shared by several calls on the same spectra)
+    keeps the half-synthesised shells of the fused path, so a leg that
+    another configuration already synthesised (same spectrum, weight and
+    bins) is not synthesised again."""
     n, L, prec, k_edge, k_bins = kinfo
     N3 = n ** 3
     use_sym = sym_eq or sym_cj
@@ -282,10 +306,8 @@ def _config_matrix(ctx, builder, src1, src2, Gs, kinfo, ell_1, m1, ell_2, m2, sy
     if use_sym and _fit_block(k_bins, shell_bytes * nc1, ctx) == k_bins:
         bs_fit = k_bins  # one diagonal block: leg 2 reuses leg 1's shells
     if dvm == "full":
-        sub = np.zeros((k_bins, k_bins), dtype=np.complex128)
         bs = min(bs_fit, k_bins if block_size == "full" else int(block_size))
     else:
-        sub = np.zeros(k_bins, dtype=np.complex128)
         bs = bs_fit
     if fused:
         bs = min(bs, cap)
@@ -293,14 +315,20 @@ def _config_matrix(ctx, builder, src1, src2, Gs, kinfo, ell_1, m1, ell_2, m2, sy
     def shells(src, comps, bins):
         rows = len(comps) * len(bins)
         if fused:
+            key = (0 if src is None else src.data_ptr(), tuple(bytes(p) for p, _ in comps), tuple(bins), n, R)
+            if shell_cache is not None and key in shell_cache:
+                return shell_cache[key]
             H = ctx.half_shells(n, R, rows)
             kz = [builder.build_half(src, poly, bins, H, c * len(bins)) for c, (poly, _) in enumerate(comps)]
+            if shell_cache is not None:
+                shell_cache[key] = (H, np.concatenate(kz))
             return H, np.concatenate(kz)
         out = ctx.mesh(n, prec, rows).view(rows, N3)
         for c, (poly, _) in enumerate(comps):
             builder.build(src, poly, bins, out[c * len(bins):(c + 1) * len(bins)])
         return out
 
+    blocks = []  # (ib, bi, bj, rows2, want, [(result, phase)]) per block of rows and columns
     for bi in range(0, k_bins, bs):
         ib = list(range(bi, min(k_bins, bi + bs)))
         R1 = shells(src1, comps1, ib)
@@ -326,15 +354,25 @@ def _config_matrix(ctx, builder, src1, src2, Gs, kinfo, ell_1, m1, ell_2, m2, sy
             rows2 = len(ib) if R2 is None else len(jb)
             pairs = [(c1 * len(ib) + (i - bi), c2 * rows2 + (j - bj)) for (i, j) in want
                      for c1 in range(nc1) for c2 in range(nc2)]
-            C = 0j
+            parts = []
             for G, gph in Gs:
                 if fused:
                     zargs = (n, L, kmax, G, R1[0], nc1 * len(ib), R1[1], None if R2 is None else R2[0],
                              nc2 * rows2, None if R2 is None else R2[1])
-                    Cm = ctx.contract_zgemm(*zargs, nblk=_zgemm_blocks(ctx, zargs)).cpu().numpy()
+                    # device result, read back by finish()
+                    parts.append((ctx.contract_zgemm(*zargs, nblk=_zgemm_blocks(ctx, zargs)), gph))
                 else:
-                    Cm = _contract(ctx, mode, prec, G, R1, R2, pairs)
-                C = C + gph * Cm
+                    parts.append((_contract(ctx, mode, prec, G, R1, R2, pairs), gph))
+            blocks.append((ib, bi, bj, rows2, want, parts))
+            del R2
+        del R1
+
+    def finish():
+        sub = np.zeros((k_bins, k_bins) if dvm == "full" else k_bins, dtype=np.complex128)
+        for ib, bi, bj, rows2, want, parts in blocks:
+            C = 0j
+            for Cm, gph in parts:
+                C = C + gph * (Cm.cpu().numpy() if torch.is_tensor(Cm) else Cm)
             for (i, j) in want:
                 s = 0j
                 for c1 in range(nc1):
@@ -344,35 +382,42 @@ def _config_matrix(ctx, builder, src1, src2, Gs, kinfo, ell_1, m1, ell_2, m2, sy
                     sub[i, j] = s
                 else:
                     sub[i] = s
-            del R2
-        del R1
-    if dvm == "full" and use_sym:
-        iu = np.triu_indices(k_bins, 1)
-        sub[iu[1], iu[0]] = sub[iu] if sym_eq else np.conj(sub[iu])
-    return sub
+        if dvm == "full" and use_sym:
+            iu = np.triu_indices(k_bins, 1)
+            sub[iu[1], iu[0]] = sub[iu] if sym_eq else np.conj(sub[iu])
+        return sub
+
+    return finish
 
 
 def _bk_signal(ctx, src1, src2, G, kinfo, mcfg, tjv, ell_1, ell_2, sym_allowed, mode, block_size, dvm,
-               conj_w=False, shell_method="synth", zsynth="fused"):
+               conj_w=False, shell_method="synth", zsynth="fused", defer=False, shell_cache=None):
     """Sum over the box magnetic configs of 3j * sum_x G F_i F'_j with the
     reference's symmetry fill and 2 Re accumulation
     (``clustering_estimator.py:867-1061``; with ``conj_w`` and null sources,
-    the S3 loop of ``:1219-1376``)."""
+    the S3 loop of ``:1219-1376``).  The device work of every magnetic
+    config is enqueued before any result is read back; ``defer`` returns the
+    function that reads them instead of the sum."""
     n, L, prec, k_edge, k_bins = kinfo
     builder = ShellBuilder(ctx, n, L, prec, k_edge, method=shell_method)
     same_ell = ell_1 == ell_2
-    if dvm == "full":
-        total = np.zeros((k_bins, k_bins), dtype=np.complex128)
-    else:
-        total = np.zeros(k_bins, dtype=np.complex128)
     Gs = G if isinstance(G, list) else [(G, 1.0)]
+    plans = []
     for ss, (m1, m2, M) in enumerate(mcfg):
         sym_eq = sym_allowed and same_ell and m1 == m2
         sym_cj = sym_allowed and same_ell and m1 == -m2
-        sub = tjv[ss] * _config_matrix(ctx, builder, src1, src2, Gs, kinfo, ell_1, m1, ell_2, m2, sym_eq, sym_cj,
-                                       mode, block_size, dvm, conj_w, zsynth)
-        total += sub if (m1, m2, M) == (0, 0, 0) else 2 * np.real(sub)
-    return total
+        plans.append((tjv[ss], (m1, m2, M), _config_plan(ctx, builder, src1, src2, Gs, kinfo, ell_1, m1, ell_2, m2,
+                                                         sym_eq, sym_cj, mode, block_size, dvm, conj_w, zsynth,
+                                                         shell_cache)))
+
+    def finish():
+        total = np.zeros((k_bins, k_bins) if dvm == "full" else k_bins, dtype=np.complex128)
+        for t, cfg, plan in plans:
+            sub = t * plan()
+            total += sub if cfg == (0, 0, 0) else 2 * np.real(sub)
+        return total
+
+    return finish if defer else finish()
 
 
 def _nice_size(m):
@@ -431,15 +476,37 @@ def bk_box_g00(ctx, delta, n, prec):
 
 
 def bk_box_signal_auto(ctx, delta, G, n, boxsize, prec, k_edge, angu_config, contraction, dvm="full",
-                       block_size="full", zsynth="fused"):
+                       block_size="full", zsynth="fused", defer=False, shell_cache=None):
     """Signal stage of an auto-correlation box bispectrum: from the
     compensated delta_k and G_00 resident in HBM to ``total_res``
-    (``clustering_estimator.py:867-1061``) for one angular configuration."""
+    (``clustering_estimator.py:867-1061``) for one angular configuration.
+    ``defer``: enqueue only and return the function that reads the result."""
     ell_1, ell_2, L = angu_config
     mcfg, tjv = A.get_magnetic_configs_box(ell_1, ell_2, L)
     k_bins = len(k_edge) - 1
     return _bk_signal(ctx, delta, delta, G, (n, boxsize, prec, np.asarray(k_edge), k_bins), mcfg, tjv, ell_1,
-                      ell_2, True, _gemm_mode(contraction, prec), block_size, dvm, zsynth=zsynth)
+                      ell_2, True, _gemm_mode(contraction, prec), block_size, dvm, zsynth=zsynth, defer=defer,
+                      shell_cache=shell_cache)
+
+
+def bk_box_signal_multi(ctx, delta, G, n, boxsize, prec, k_edge, angu_configs, contraction, dvm="full",
+                        block_size="full", zsynth="fused", defer=False):
+    """``bk_box_signal_auto`` for several angular configurations of the same
+    field (the reference runs one ``run_stats`` per configuration).  With the
+    fused contraction, shells that several configurations use are
+    synthesised once: the unweighted shells of [0,0,0] are the second leg of
+    every [l,0,l].  Returns the list of ``total_res`` (or, with ``defer``,
+    the function that reads it); each equals the single-configuration
+    result."""
+    cache = {}
+    fins = [bk_box_signal_auto(ctx, delta, G, n, boxsize, prec, k_edge, a, contraction, dvm, block_size, zsynth,
+                               defer=True, shell_cache=cache) for a in angu_configs]
+    cache.clear()  # freed buffers are reused in stream order, after the work that reads them
+
+    def finish():
+        return [f() for f in fins]
+
+    return finish if defer else finish()
 
 
 def calculate_bk_sugi_box(rfield_a, rfield_b, rfield_c, correlation_mode, stat_attrs, comm=None, **kwargs):

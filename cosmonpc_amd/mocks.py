@@ -95,8 +95,14 @@ def run_mocks(configs, catalogs, batch=16, skip_existing=True, arrays=None):
             pos_l.append(np.ascontiguousarray(pos, dtype=np.float64))
             w_l.append(np.ascontiguousarray(w, dtype=np.float64))
         off = np.concatenate([[0], np.cumsum([len(w) for w in w_l])]).astype(np.int64)
-        pos_d = torch.from_numpy(np.concatenate(pos_l)).to(ctx.device)
-        w_d = torch.from_numpy(np.concatenate(w_l)).to(ctx.device)
+        # each catalog goes straight to its slice of the device arrays (no
+        # host-side concatenation of the chunk)
+        pos_d = torch.empty((int(off[-1]), 3), dtype=torch.float64, device=ctx.device)
+        w_d = torch.empty(int(off[-1]), dtype=torch.float64, device=ctx.device)
+        for b in range(len(chunk)):
+            if off[b + 1] > off[b]:
+                pos_d[off[b]:off[b + 1]].copy_(torch.from_numpy(pos_l[b]))
+                w_d[off[b]:off[b + 1]].copy_(torch.from_numpy(w_l[b]))
         nb = len(chunk)
         m0 = ctx.mesh(n, prec, nb)
         m1 = ctx.mesh(n, prec, nb) if interlaced else None

@@ -22,6 +22,11 @@
  *  - Calls do not allocate device memory except the first use of an FFT
  *    shape (plan + work area, cached in the context) and growth of the
  *    context's reduction scratch (cnpc_ctx_reserve pre-sizes it).
+ *  - Graph recording: the enqueue-only calls (no host read-back) may be
+ *    captured into a hipGraph on the context's stream once the same calls
+ *    have run eagerly; a call that would create a plan or grow a work area,
+ *    or that reads back to the host (cnpc_contract_pairs, cnpc_absmax_f32),
+ *    fails while the stream is capturing.
  */
 #ifndef CNPC_H
 #define CNPC_H
@@ -59,6 +64,13 @@ int cnpc_ctx_destroy(cnpc_ctx* ctx);
 int cnpc_ctx_set_stream(cnpc_ctx* ctx, void* hip_stream);
 int cnpc_ctx_reserve(cnpc_ctx* ctx, int64_t scratch_bytes);
 int cnpc_ctx_clear_plans(cnpc_ctx* ctx);
+/* delta = +1 / -1: take / release a hold on the context's work areas and
+ * plans for a recorded graph that uses their addresses.  While held, a work
+ * area that must grow keeps its old allocation alive (freed when the last
+ * hold is released) and cnpc_ctx_clear_plans fails.  No counterpart in the
+ * reference (pmesh has no graphs); replays must stay ordered with the
+ * context's other work (same stream). */
+int cnpc_ctx_hold(cnpc_ctx* ctx, int delta);
 
 /* Paint weighted particles onto mesh0 (and the +0.5-cell shifted copy onto
  * mesh1 when interlaced); every cell of the meshes is overwritten.  window:
