@@ -160,7 +160,9 @@ def main():
         f"(max rel diff vs apart {share_diff:.1e})")
     graph_info = {"configs_apart_ms_per_step": t_indep * 1e3, "configs_apart_bins_per_s": bins_per_step / t_indep,
                   "shared_legs_eager_ms_per_step": t_eager * 1e3, "shared_vs_apart_max_rel_diff": share_diff}
-    if share_diff > 1e-12:
+    # both are split-bf16 results (~1e-6 of max|B| from FP64); sharing changes
+    # only the FP32 summation order of the contraction
+    if share_diff > 1e-5:
         res_eager, t_eager = res_indep, t_indep
 
     # the step recorded once as a hipGraph and replayed (the input spectrum
@@ -172,7 +174,7 @@ def main():
 
         sg = StepGraph(ctx)
         try:
-            pending = sg.record(lambda: enqueue_step(d32, 1, "bf16x3", shared=share_diff <= 1e-12), finish_step)
+            pending = sg.record(lambda: enqueue_step(d32, 1, "bf16x3", shared=share_diff <= 1e-5), finish_step)
         except Exception as e:  # recorded nothing: report the eager step
             graph_info["graph_error"] = f"{type(e).__name__}: {e}"
             log(f"graph recording failed: {graph_info['graph_error']}")
@@ -275,8 +277,8 @@ def main():
 
 
 def kernel_roofline(ctx, delta, n, L, k_edge, torch, np, A):
-    """Times each stage of one [0,0,0] signal pass with HIP events on the
-    context stream and returns the dominant kernel's roofline."""
+    """Times each stage of the signal pass with HIP events on the context
+    stream and returns the dominant kernel's roofline."""
     from cosmonpc_amd.estimators import bk_box_g00
 
     K = len(k_edge) - 1
@@ -310,6 +312,17 @@ def kernel_roofline(ctx, delta, n, L, k_edge, torch, np, A):
         kz[b0:b0 + c] = ctx.shell_synth_half(delta, n, L, k_edge, b0, c, poly, work, H, b0)
     ctx.contract_zgemm(n, L, kmax, G, H, K, kz)
     t_zgemm = timed(lambda: ctx.contract_zgemm(n, L, kmax, G, H, K, kz), 3)
+    # the launch of the measured step: [2,0,2] (A = Y_20-weighted shells,
+    # B = the unweighted ones) with [0,0,0] = B^T G B from the same launch
+    H2 = ctx.half_shells(n, R, K)
+    kz2 = np.zeros(K, dtype=np.int32)
+    poly2 = A.shell_components(2, 0)[0][0]
+    for b0 in range(0, K, batch):
+        c = min(batch, K - b0)
+        kz2[b0:b0 + c] = ctx.shell_synth_half(delta, n, L, k_edge, b0, c, poly2, work, H2, b0)
+    ctx.contract_zgemm(n, L, kmax, G, H2, K, kz2, H, K, kz, with_bb=True)
+    t_bb = timed(lambda: ctx.contract_zgemm(n, L, kmax, G, H2, K, kz2, H, K, kz, with_bb=True), 3)
+    del H2
     # the separate path: real shells (cube, x, y and z FFT kernels) + GEMM
     t_synth = []
     for b0 in range(0, K, batch):
@@ -336,6 +349,7 @@ def kernel_roofline(ctx, delta, n, L, k_edge, torch, np, A):
     # written / read kz rows only (each shell holds kz < its own radius)
     b_half = int(np.mean([kz[b0:b0 + batch].sum() for b0 in range(0, K - batch + 1, batch)])) * n * n * 8
     b_zgemm = int(kz.sum()) * n * n * 8 + N3 * 4
+    b_bb = int(kz.sum() + kz2.sum()) * n * n * 8 + N3 * 4
     b_c2r = batch * (half * 8 + N3 * 4)
     b_filter = half * 8 + batch * half * 8
     b_gemm = (K + 1) * N3 * 4
@@ -343,6 +357,7 @@ def kernel_roofline(ctx, delta, n, L, k_edge, torch, np, A):
         "shell_synth_half_batch4": {"ms_per_launch": per_launch(t_half), "launches": len(t_half),
                                     "total_ms": sum(x[0] for x in t_half), "bytes": b_half},
         "zgemm_bf16x3": {"ms_per_launch": t_zgemm, "launches": 1, "total_ms": t_zgemm, "bytes": b_zgemm},
+        "zgemm_bf16x3_bb": {"ms_per_launch": t_bb, "launches": 1, "total_ms": t_bb, "bytes": b_bb},
         "shell_synth_batch4": {"ms_per_launch": per_launch(t_synth), "launches": len(t_synth),
                                "total_ms": sum(x[0] for x in t_synth), "bytes": b_synth},
         "gemm_x3_bf16": {"ms_per_launch": t_gemm, "launches": 1, "total_ms": t_gemm, "bytes": b_gemm},
@@ -353,7 +368,9 @@ def kernel_roofline(ctx, delta, n, L, k_edge, torch, np, A):
     }
     for v in stages.values():
         v["GBps"] = v["bytes"] / v["ms_per_launch"] / 1e6
-    dom = max(("shell_synth_half_batch4", "zgemm_bf16x3"), key=lambda k: stages[k]["total_ms"])
+    # the measured step runs 15 synthesis launches (both weights) and one
+    # two-output contraction
+    dom = max(("shell_synth_half_batch4", "zgemm_bf16x3_bb"), key=lambda k: stages[k]["total_ms"])
     ach = stages[dom]["GBps"]
     roof = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
             "traffic": measured_traffic(dom), "kernel": dom, "algorithmic_bytes_per_launch": stages[dom]["bytes"]}
@@ -499,7 +516,16 @@ def bench_kernels(args):
                                       int(kz[8:12].sum()) * m * m * 8)
     out["zgemm_bf16x3"] = (timed(lambda: ctx.contract_zgemm(m, L, kmax, G, H, K, kz)),
                            int(kz.sum()) * m * m * 8 + m ** 3 * 4)
-    del H
+    # [2,0,2] beside [0,0,0]: A = Y_20-weighted shells, B = the unweighted
+    # ones, and B^T G B ([0,0,0]) from the same launch
+    H2 = ctx.half_shells(m, R, K)
+    kz2 = np.zeros(K, dtype=np.int32)
+    poly2 = A.shell_components(2, 0)[0][0]
+    for b0 in range(0, K, 4):
+        kz2[b0:b0 + 4] = ctx.shell_synth_half(dm, m, L, k_edge, b0, min(4, K - b0), poly2, work, H2, b0)
+    out["zgemm_bf16x3_bb"] = (timed(lambda: ctx.contract_zgemm(m, L, kmax, G, H2, K, kz2, H, K, kz, with_bb=True)),
+                              int(kz.sum() + kz2.sum()) * m * m * 8 + m ** 3 * 4)
+    del H, H2
     src = torch.empty(1 << 28, dtype=torch.float32, device=dev)
     dst = torch.empty_like(src)
     src.fill_(1.0)

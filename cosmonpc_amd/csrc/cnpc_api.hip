@@ -507,10 +507,12 @@ int cnpc_shell_synth_half(cnpc_ctx* ctx, int n, double boxsize, const void* D, c
 
 int cnpc_contract_zgemm(cnpc_ctx* ctx, int n, double boxsize, double kmax, const float* G, const void* HA, int na,
                         int sa_slots, const int* kz_a, const void* HB, int nb, int sb_slots, const int* kz_b,
-                        int nblk, double* C) {
+                        int nblk, double* C, double* C_bb) {
   REQUIRE(ctx, "null context");
   REQUIRE(n >= 8 && n % 8 == 0 && boxsize > 0 && kmax > 0, "contract_zgemm: nmesh must be a multiple of 8");
   REQUIRE(G && HA && C, "contract_zgemm: null buffers");
+  REQUIRE(!C_bb || (HB && HB != HA), "contract_zgemm: C_bb needs a B different from A");
+  REQUIRE(!C_bb || (C_bb != C), "contract_zgemm: C_bb must not alias C");
   REQUIRE(na >= 1 && na <= 32 && sa_slots >= na, "contract_zgemm: na must be in [1, 32] and fit the slots of HA");
   const int same = HB == nullptr || HB == HA;
   if (same) {
@@ -539,7 +541,7 @@ int cnpc_contract_zgemm(cnpc_ctx* ctx, int n, double boxsize, double kmax, const
     nblk *= zgemm_blocks_per_cu(same);
   }
   if (nblk > 4096) nblk = 4096;
-  const size_t pbytes = (size_t)nblk * 1024 * 8;
+  const size_t pbytes = (size_t)nblk * 1024 * 8 * (C_bb ? 2 : 1);  // FP64 partials (two sets with C_bb)
   if (int rc = ensure_scratch(ctx, pbytes + (size_t)n * 8)) return rc;
   char* base = (char*)ctx->scratch;
   CHECK_HIP(launch_twiddles(n, base + pbytes, ctx->stream));
@@ -551,7 +553,7 @@ int cnpc_contract_zgemm(cnpc_ctx* ctx, int n, double boxsize, double kmax, const
             "contract_zgemm: kz rows per slot must be in [0, R + 1]");
   }
   CHECK_HIP(launch_zgemm(a, R + 1, HA, na, sa_slots, qa, same ? HA : HB, nb, sb_slots, qb, same, G, base + pbytes,
-                         nblk, (double*)base, C, ctx->stream));
+                         nblk, (double*)base, C, C_bb, ctx->stream));
   return 0;
 }
 

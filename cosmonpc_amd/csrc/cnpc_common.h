@@ -305,7 +305,7 @@ int zgemm_supported(const SynthArgs& a, int na, int nb, int same, size_t lds_lim
 int zgemm_blocks_per_cu(int same);
 hipError_t launch_zgemm(const SynthArgs& a, int Q, const void* HA, int na, int sa_slots, const int* qa,
                         const void* HB, int nb, int sb_slots, const int* qb, int same, const float* G, const void* tw,
-                        int nblk, double* partial, double* C, hipStream_t st);
+                        int nblk, double* partial, double* C, double* Cbb, hipStream_t st);
 hipError_t launch_gemm_finalize(const double* partial, int nblk, int a0, int b0, int na, int nb, double inv_scale,
                                 double* C, int ldc, hipStream_t st);
 hipError_t launch_synth(int prec, const SynthArgs& a, const BinEdges& E, const PolySpec& W, const void* D, void* V,

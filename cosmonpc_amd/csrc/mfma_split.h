@@ -11,14 +11,18 @@ typedef double f64x4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 
-__device__ __forceinline__ void block_reduce_store(const float (&acc)[2][2][4], double* out) {
+// Sums the accumulators of waves w_lo .. w_hi-1 (all by default) into one
+// FP64 32 x 32 tile, wave by wave in a fixed order, and stores it to out.
+// Every thread of the workgroup must call it (it synchronises).
+__device__ __forceinline__ void block_reduce_store(const float (&acc)[2][2][4], double* out, int w_lo = 0,
+                                                   int w_hi = 1 << 20) {
   // acc[bi][bj][j] holds C[16*bi + 4*(lane/16) + j][16*bj + lane%16]
   __shared__ double red[32 * 32];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   for (int i = threadIdx.x; i < 1024; i += blockDim.x) red[i] = 0.0;
   __syncthreads();
-  const int nw = blockDim.x >> 6;
-  for (int ww = 0; ww < nw; ++ww) {
+  const int nw = (int)(blockDim.x >> 6) < w_hi ? (int)(blockDim.x >> 6) : w_hi;
+  for (int ww = w_lo; ww < nw; ++ww) {
     if (w == ww) {
       for (int bi = 0; bi < 2; ++bi)
         for (int bj = 0; bj < 2; ++bj)
@@ -30,6 +34,7 @@ __device__ __forceinline__ void block_reduce_store(const float (&acc)[2][2][4], 
     __syncthreads();
   }
   for (int i = threadIdx.x; i < 1024; i += blockDim.x) out[i] = red[i];
+  __syncthreads();  // red may be reused by a second call
 }
 
 template <bool F16>

@@ -23,6 +23,10 @@
 //      consecutive values of their A and B rows, form B * G, split hi + lo in
 //      bf16 and issue the three 16x16x32 MFMAs of each of the 2 x 2 output
 //      blocks (FP32 accumulate).
+// With B != A the same lines also give S'_bc = sum_x B_b G B_c (BB): the
+// first half of the waves accumulates A^T G B, the second half B^T G B over
+// the same chunks, so a configuration whose legs are both B (e.g. [0,0,0]
+// beside [2,0,2]) costs no FFT of its own.
 // Slot s holds kz rows below q[s] only (its own shell radius); the rest of
 // its line is zero.
 // The per-wave accumulators are summed in a fixed order into one FP64
@@ -59,6 +63,7 @@ struct ZGemmArgs {
   const float* G;           // [x][y][z]
   const float2* tw;         // e^{2 pi i j / n}
   double* partial;          // [gridDim.x][32 * 32]
+  double* partial_bb;       // [gridDim.x][32 * 32] of B^T G B (BB launches)
 };
 
 // 8 consecutive z values of one part of one planar line (two 16-byte LDS
@@ -76,7 +81,7 @@ __device__ __forceinline__ void zg_row8(const float2* buf, int ls, int line, int
 // shells 2l, 2l+1), 2 * nchr chunks; B == A without BR: one round per row
 // (line l holds shells 2l, 2l+1 of that row).  B != A: one round per row
 // (line l holds A_l and B_l), nchr chunks.
-template <int THREADS, bool BR>
+template <int THREADS, bool BR, bool BB>
 __global__ void __launch_bounds__(THREADS) zgemm_bf16x3_kernel(ZGemmArgs g) {
   // lines of n + 4 complex: 16-byte aligned planes for the planar last pass
   const int n = g.n, Q = g.Q, R = Q - 1, ls = n + 4, half = n >> 1, L = g.nlines;
@@ -91,6 +96,9 @@ __global__ void __launch_bounds__(THREADS) zgemm_bf16x3_kernel(ZGemmArgs g) {
     qb[threadIdx.x] = g.qb[threadIdx.x];
   }
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  // BB: waves [0, nw/2) take A^T G B, waves [nw/2, nw) B^T G B (wave-uniform)
+  const bool wbb = BB && w >= (nw >> 1);
+  const int wq = BB ? (wbb ? w - (nw >> 1) : w) : w, nwq = BB ? (nw >> 1) : nw;
   const int rr = lane & 15, kg = lane >> 4;
   const int nchr = (n + 31) >> 5;      // chunks per row
   const int Lh = (g.na + 1) >> 1;      // lines per row when B == A
@@ -162,7 +170,7 @@ __global__ void __launch_bounds__(THREADS) zgemm_bf16x3_kernel(ZGemmArgs g) {
       __syncthreads();
       stockham_lds_planar<ZG_MAXB>(buf, W, n, ls, L, g.fft);
       const int nchunk = (g.same && BR) ? 2 * nchr : nchr;
-      for (int c = w; c < nchunk; c += nw) {
+      for (int c = wq; c < nchunk; c += nwq) {
         const int r = (g.same && BR) ? (c >= nchr ? 1 : 0) : r0;
         const int z0 = 32 * (c - ((g.same && BR && r) ? nchr : 0)) + 8 * kg;
         const bool zin = z0 < n;      // n % 8 == 0: a group of 8 is all in or all out
@@ -182,6 +190,11 @@ __global__ void __launch_bounds__(THREADS) zgemm_bf16x3_kernel(ZGemmArgs g) {
           if (g.same) {
             const int line = (BR ? r * Lh : 0) + (i >> 1 < Lh ? i >> 1 : Lh - 1);
             zg_row8(buf, ls, line, i & 1, zin && i < g.na, zr, a);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) b[j] = a[j] * vg[j];
+          } else if (wbb) {
+            const int line = i < L ? i : L - 1;
+            zg_row8(buf, ls, line, 1, zin && i < g.nb, zr, a);
 #pragma unroll
             for (int j = 0; j < 8; ++j) b[j] = a[j] * vg[j];
           } else {
@@ -209,7 +222,12 @@ __global__ void __launch_bounds__(THREADS) zgemm_bf16x3_kernel(ZGemmArgs g) {
       __syncthreads();  // the next round overwrites buf (and Gs)
     }
   }
-  block_reduce_store(acc, g.partial + (int64_t)blockIdx.x * 1024);
+  if (BB) {
+    block_reduce_store(acc, g.partial + (int64_t)blockIdx.x * 1024, 0, nw >> 1);
+    block_reduce_store(acc, g.partial_bb + (int64_t)blockIdx.x * 1024, nw >> 1, nw);
+  } else {
+    block_reduce_store(acc, g.partial + (int64_t)blockIdx.x * 1024);
+  }
 }
 
 // Launch shapes.  B == A: 2 * ceil(na/2) lines per round (both rows) with
@@ -244,7 +262,7 @@ int zgemm_supported(const SynthArgs& a, int na, int nb, int same, size_t lds_lim
 // workgroups per CU the launch shape allows
 int zgemm_blocks_per_cu(int same) { return (same && ZG_SAME_PER_ROW) ? 2 : 1; }
 
-template <int T, bool BR>
+template <int T, bool BR, bool BB = false>
 static hipError_t zg_launch(const ZGemmArgs& g, int nblk, size_t lds, hipStream_t st) {
   // dynamic LDS beyond the default 64 KB must be allowed per kernel and
   // device; the allowance only grows, so repeated launches (and launches
@@ -255,18 +273,19 @@ static hipError_t zg_launch(const ZGemmArgs& g, int nblk, size_t lds, hipStream_
   if (e != hipSuccess) return e;
   if (dev < 0 || dev >= CNPC_MAX_DEVICES) return hipErrorInvalidDevice;
   if (lds > allowed[dev]) {
-    e = hipFuncSetAttribute((const void*)zgemm_bf16x3_kernel<T, BR>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)lds);
+    e = hipFuncSetAttribute((const void*)zgemm_bf16x3_kernel<T, BR, BB>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
     allowed[dev] = lds;
   }
-  hipLaunchKernelGGL((zgemm_bf16x3_kernel<T, BR>), dim3(nblk), dim3(T), lds, st, g);
+  hipLaunchKernelGGL((zgemm_bf16x3_kernel<T, BR, BB>), dim3(nblk), dim3(T), lds, st, g);
   return hipGetLastError();
 }
 
 hipError_t launch_zgemm(const SynthArgs& a, int Q, const void* HA, int na, int sa_slots, const int* qa,
                         const void* HB, int nb, int sb_slots, const int* qb, int same, const float* G, const void* tw,
-                        int nblk, double* partial, double* C, hipStream_t st) {
+                        int nblk, double* partial, double* C, double* Cbb, hipStream_t st) {
+  if (Cbb && same) return hipErrorInvalidValue;
   ZGemmArgs g;
   g.n = a.n; g.Q = Q; g.na = na; g.nb = same ? na : nb;
   g.sa_slots = sa_slots; g.sb_slots = same ? sa_slots : sb_slots;
@@ -283,16 +302,21 @@ hipError_t launch_zgemm(const SynthArgs& a, int Q, const void* HA, int na, int s
   g.G = G;
   g.tw = (const float2*)tw;
   g.partial = partial;
+  g.partial_bb = partial + (int64_t)nblk * 1024;  // used by BB launches only
   const size_t lds = zgemm_lds_bytes(a.n, g.nlines);
   hipError_t e;
   if (same && ZG_SAME_PER_ROW)
     e = zg_launch<384, false>(g, nblk, lds, st);
   else if (same)
     e = zg_launch<ZG_THREADS, true>(g, nblk, lds, st);
+  else if (Cbb)
+    e = zg_launch<ZG_THREADS, false, true>(g, nblk, lds, st);
   else
     e = zg_launch<ZG_THREADS, false>(g, nblk, lds, st);
   if (e != hipSuccess) return e;
-  return launch_gemm_finalize(partial, nblk, 0, 0, na, g.nb, 1.0, C, g.nb, st);
+  e = launch_gemm_finalize(partial, nblk, 0, 0, na, g.nb, 1.0, C, g.nb, st);
+  if (e != hipSuccess || !Cbb) return e;
+  return launch_gemm_finalize(g.partial_bb, nblk, 0, 0, g.nb, g.nb, 1.0, Cbb, g.nb, st);
 }
 
 }  // namespace cnpc

@@ -205,15 +205,58 @@ def _contract(ctx, mode, prec, G, R1, R2, pairs):
     return ctx.contract_gemm(mode, G, R1, R2, nblk=_gemm_blocks(ctx, mode, G, R1, R2)).cpu().numpy()
 
 
-def _zgemm_blocks(ctx, zargs):
+def _zgemm_blocks(ctx, zargs, with_bb=False):
     """Workgroups of the fused contraction (1, 2 or 4 per CU), autotuned per
     grid and row counts."""
     from . import autotune
 
     n, rows_a, rows_b, same = zargs[0], zargs[5], zargs[8], zargs[7] is None
     ncu = torch.cuda.get_device_properties(ctx.device).multi_processor_count
-    return autotune.choose("zgemm_blocks", (n, rows_a, rows_b, same), [ncu, 2 * ncu, 4 * ncu],
-                           lambda c: ctx.contract_zgemm(*zargs, nblk=c))
+    shape = (n, rows_a, rows_b, same) + (("bb",) if with_bb else ())
+    return autotune.choose("zgemm_blocks", shape, [ncu, 2 * ncu, 4 * ncu],
+                           lambda c: ctx.contract_zgemm(*zargs, nblk=c, with_bb=with_bb))
+
+
+class _ZRequest:
+    """A fused contraction asked for by a configuration plan; ``value`` is
+    its device result once the batch has run."""
+
+    def __init__(self, zargs):
+        self.zargs = zargs
+        self.value = None
+
+
+class ZBatch:
+    """Fused contractions of several configurations launched together.  A
+    B == A request on shells X and a B != A request (Y, X) with the same G
+    and the same X buffer become one launch: the B != A kernel returns
+    sum_x X G X from the z transforms it already does (``with_bb``), so the
+    [0,0,0] signal matrix costs no z pass beside [l,0,l]'s."""
+
+    def __init__(self, ctx):
+        self.ctx = ctx
+        self.requests = []
+
+    def request(self, zargs):
+        r = _ZRequest(zargs)
+        self.requests.append(r)
+        return r
+
+    def run(self):
+        same = [r for r in self.requests if r.zargs[7] is None]
+        for r in self.requests:
+            if r.zargs[7] is None or r.value is not None:
+                continue
+            z = r.zargs
+            mate = next((s for s in same if s.value is None and s.zargs[3] is z[3] and s.zargs[4] is z[7]
+                         and s.zargs[5] == z[8] and np.array_equal(s.zargs[6][:z[8]], z[9][:z[8]])), None)
+            if mate is not None:
+                r.value, mate.value = self.ctx.contract_zgemm(*z, nblk=_zgemm_blocks(self.ctx, z, True),
+                                                             with_bb=True)
+        for r in self.requests:
+            if r.value is None:
+                r.value = self.ctx.contract_zgemm(*r.zargs, nblk=_zgemm_blocks(self.ctx, r.zargs))
+        self.requests = []
 
 
 def _gemm_blocks(ctx, mode, G, R1, R2):
@@ -256,7 +299,7 @@ def _config_matrix(*args, **kwargs):
 
 
 def _config_plan(ctx, builder, src1, src2, Gs, kinfo, ell_1, m1, ell_2, m2, sym_eq, sym_cj, mode, block_size,
-                 dvm, conj_w=False, zsynth="fused", shell_cache=None):
+                 dvm, conj_w=False, zsynth="fused", shell_cache=None, zbatch=None):
     """3j-free signal matrix of one magnetic configuration,
     S_ij = sum_x G F_i F'_j, with the reference's upper-triangle evaluation and
     lower fill (transpose for F' = F, conjugate for F' = +-conj F;
@@ -279,7 +322,9 @@ def _config_plan(ctx, builder, src1, src2, Gs, kinfo, ell_1, m1, ell_2, m2, sym_
     go.  ``shell_cache`` (a dict shared by several calls on the same spectra)
     keeps the half-synthesised shells of the fused path, so a leg that
     another configuration already synthesised (same spectrum, weight and
-    bins) is not synthesised again."""
+    bins) is not synthesised again; with ``zbatch`` (a ``ZBatch``) the fused
+    contractions are only requested here and launched by its ``run()``,
+    before ``finish``."""
     n, L, prec, k_edge, k_bins = kinfo
     N3 = n ** 3
     use_sym = sym_eq or sym_cj
@@ -360,7 +405,10 @@ def _config_plan(ctx, builder, src1, src2, Gs, kinfo, ell_1, m1, ell_2, m2, sym_
                     zargs = (n, L, kmax, G, R1[0], nc1 * len(ib), R1[1], None if R2 is None else R2[0],
                              nc2 * rows2, None if R2 is None else R2[1])
                     # device result, read back by finish()
-                    parts.append((ctx.contract_zgemm(*zargs, nblk=_zgemm_blocks(ctx, zargs)), gph))
+                    if zbatch is not None:
+                        parts.append((zbatch.request(zargs), gph))
+                    else:
+                        parts.append((ctx.contract_zgemm(*zargs, nblk=_zgemm_blocks(ctx, zargs)), gph))
                 else:
                     parts.append((_contract(ctx, mode, prec, G, R1, R2, pairs), gph))
             blocks.append((ib, bi, bj, rows2, want, parts))
@@ -372,6 +420,8 @@ def _config_plan(ctx, builder, src1, src2, Gs, kinfo, ell_1, m1, ell_2, m2, sym_
         for ib, bi, bj, rows2, want, parts in blocks:
             C = 0j
             for Cm, gph in parts:
+                if isinstance(Cm, _ZRequest):
+                    Cm = Cm.value
                 C = C + gph * (Cm.cpu().numpy() if torch.is_tensor(Cm) else Cm)
             for (i, j) in want:
                 s = 0j
@@ -391,7 +441,7 @@ def _config_plan(ctx, builder, src1, src2, Gs, kinfo, ell_1, m1, ell_2, m2, sym_
 
 
 def _bk_signal(ctx, src1, src2, G, kinfo, mcfg, tjv, ell_1, ell_2, sym_allowed, mode, block_size, dvm,
-               conj_w=False, shell_method="synth", zsynth="fused", defer=False, shell_cache=None):
+               conj_w=False, shell_method="synth", zsynth="fused", defer=False, shell_cache=None, zbatch=None):
     """Sum over the box magnetic configs of 3j * sum_x G F_i F'_j with the
     reference's symmetry fill and 2 Re accumulation
     (``clustering_estimator.py:867-1061``; with ``conj_w`` and null sources,
@@ -408,7 +458,7 @@ def _bk_signal(ctx, src1, src2, G, kinfo, mcfg, tjv, ell_1, ell_2, sym_allowed, 
         sym_cj = sym_allowed and same_ell and m1 == -m2
         plans.append((tjv[ss], (m1, m2, M), _config_plan(ctx, builder, src1, src2, Gs, kinfo, ell_1, m1, ell_2, m2,
                                                          sym_eq, sym_cj, mode, block_size, dvm, conj_w, zsynth,
-                                                         shell_cache)))
+                                                         shell_cache, zbatch)))
 
     def finish():
         total = np.zeros((k_bins, k_bins) if dvm == "full" else k_bins, dtype=np.complex128)
@@ -476,7 +526,7 @@ def bk_box_g00(ctx, delta, n, prec):
 
 
 def bk_box_signal_auto(ctx, delta, G, n, boxsize, prec, k_edge, angu_config, contraction, dvm="full",
-                       block_size="full", zsynth="fused", defer=False, shell_cache=None):
+                       block_size="full", zsynth="fused", defer=False, shell_cache=None, zbatch=None):
     """Signal stage of an auto-correlation box bispectrum: from the
     compensated delta_k and G_00 resident in HBM to ``total_res``
     (``clustering_estimator.py:867-1061``) for one angular configuration.
@@ -486,7 +536,7 @@ def bk_box_signal_auto(ctx, delta, G, n, boxsize, prec, k_edge, angu_config, con
     k_bins = len(k_edge) - 1
     return _bk_signal(ctx, delta, delta, G, (n, boxsize, prec, np.asarray(k_edge), k_bins), mcfg, tjv, ell_1,
                       ell_2, True, _gemm_mode(contraction, prec), block_size, dvm, zsynth=zsynth, defer=defer,
-                      shell_cache=shell_cache)
+                      shell_cache=shell_cache, zbatch=zbatch)
 
 
 def bk_box_signal_multi(ctx, delta, G, n, boxsize, prec, k_edge, angu_configs, contraction, dvm="full",
@@ -494,13 +544,17 @@ def bk_box_signal_multi(ctx, delta, G, n, boxsize, prec, k_edge, angu_configs, c
     """``bk_box_signal_auto`` for several angular configurations of the same
     field (the reference runs one ``run_stats`` per configuration).  With the
     fused contraction, shells that several configurations use are
-    synthesised once: the unweighted shells of [0,0,0] are the second leg of
-    every [l,0,l].  Returns the list of ``total_res`` (or, with ``defer``,
-    the function that reads it); each equals the single-configuration
-    result."""
+    synthesised once (the unweighted shells of [0,0,0] are the second leg of
+    every [l,0,l]) and the contractions are launched together (``ZBatch``:
+    [0,0,0]'s comes out of [l,0,l]'s launch).  Returns the list of
+    ``total_res`` (or, with ``defer``, the function that reads it); each
+    equals the single-configuration result up to the FP32 accumulation order
+    of the contraction."""
     cache = {}
+    zbatch = ZBatch(ctx)
     fins = [bk_box_signal_auto(ctx, delta, G, n, boxsize, prec, k_edge, a, contraction, dvm, block_size, zsynth,
-                               defer=True, shell_cache=cache) for a in angu_configs]
+                               defer=True, shell_cache=cache, zbatch=zbatch) for a in angu_configs]
+    zbatch.run()
     cache.clear()  # freed buffers are reused in stream order, after the work that reads them
 
     def finish():

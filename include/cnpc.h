@@ -192,10 +192,14 @@ int cnpc_contract_gemm(cnpc_ctx* ctx, int mode, int64_t ncell, const void* G, co
  * kz_rows that call reported for each slot (host arrays; NULL = all R+1
  * rows).  HB NULL (or HA) means B == A.  na, nb <= 32.  Equals
  * cnpc_shell_synth followed by cnpc_contract_gemm mode 0 without the real
- * shells in HBM; the block loop of clustering_estimator.py:911-981. */
+ * shells in HBM; the block loop of clustering_estimator.py:911-981.
+ * C_bb (NULL = not computed; needs B != A): also C_bb[b][c] = sum_x B_b G
+ * B_c (nb x nb) from the same z transforms, the signal matrix of a second
+ * configuration whose legs are both B (the [0,0,0] configuration beside
+ * [l,0,l], which the reference computes in a separate run). */
 int cnpc_contract_zgemm(cnpc_ctx* ctx, int nmesh, double boxsize, double kmax, const float* G, const void* HA, int na,
                         int sa_slots, const int* kz_a, const void* HB, int nb, int sb_slots, const int* kz_b,
-                        int nblk, double* C);
+                        int nblk, double* C, double* C_bb);
 
 /* *ok = 1 when cnpc_contract_zgemm accepts this grid and these row counts
  * on the current device (LDS line buffers and FFT staging), else 0. */

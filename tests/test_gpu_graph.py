@@ -49,16 +49,20 @@ def _eager(ctx, X, shared=True):
 
 
 def test_shared_legs_equal_configurations_apart(gpu_ctx):
-    """The multi-configuration call synthesises [0,0,0]'s shells once and
-    reuses them as [2,0,2]'s second leg: the results are the per-configuration
-    ones, bit for bit (same kernels on the same values)."""
+    """The multi-configuration call synthesises [0,0,0]'s shells once, reuses
+    them as [2,0,2]'s second leg and takes [0,0,0]'s matrix from [2,0,2]'s
+    contraction (B^T G B of the same z transforms): the results are the
+    per-configuration ones up to the FP32 summation order of the split-bf16
+    contraction (1e-5 of max |S|; the split error itself is ~1e-6)."""
     ctx = gpu_ctx
     X = _spectrum(ctx, 7)
     apart = _eager(ctx, X, shared=False)
     shared = _eager(ctx, X, shared=True)
     for a, b in zip(apart, shared):
         assert np.max(np.abs(a)) > 0
-        np.testing.assert_array_equal(a, b)
+        err = np.max(np.abs(a - b)) / np.max(np.abs(a))
+        _log(f"shared vs apart rel max diff {err:.2e}")
+        assert err < 1e-5
 
 
 def test_graph_replays_match_eager(gpu_ctx):

@@ -343,6 +343,16 @@ def test_zgemm_fused_matches_fp64_contraction(gpu_ctx, n, L, kmax):
             ref = ctx.contract_gemm("gemm_fp64", G64, A64, B64).cpu().numpy()
             sep = ctx.contract_gemm("bf16x3", G32, A32, B32).cpu().numpy()
             got = ctx.contract_zgemm(n, L, kmax, G32, HA, na, kza, HB, nb, kzb).cpu().numpy()
+            # the same launch with B^T G B from the same z transforms
+            got2, gbb = ctx.contract_zgemm(n, L, kmax, G32, HA, na, kza, HB, nb, kzb, with_bb=True)
+            ref_bb = ctx.contract_gemm("gemm_fp64", G64, B64).cpu().numpy()
+            sep_bb = ctx.contract_gemm("bf16x3", G32, B32).cpu().numpy()
+            for tag, g2, r2, s2 in (("with_bb C", got2.cpu().numpy(), ref, sep),
+                                    ("with_bb C_bb", gbb.cpu().numpy(), ref_bb, sep_bb)):
+                sc = np.max(np.abs(r2))
+                e2, e2s = np.max(np.abs(g2 - r2)) / sc, np.max(np.abs(s2 - r2)) / sc
+                _log(f"zgemm n={n} {what} {tag}: rel err fused {e2:.2e}, separate {e2s:.2e}")
+                assert e2 <= max(2 * e2s, 1e-6), (what, tag, e2, e2s)
         scale = np.max(np.abs(ref))
         err, err_sep = np.max(np.abs(got - ref)) / scale, np.max(np.abs(sep - ref)) / scale
         _log(f"zgemm n={n} {what}: rel err fused {err:.2e}, separate {err_sep:.2e}")
