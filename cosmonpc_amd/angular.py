@@ -248,10 +248,13 @@ _Q_CACHE = {}
 
 def get_q_ells_eff(i, j, k_eff, ell_1, ell_2, L, k3_bins, k3_eff):
     """q_{l1 l2 L}(k1, k2, k3) at effective k (``math_evaluator.py:592-616``).
-    Memoised on its arguments: k_eff depends only on the grid and the bins,
-    so every mock of a campaign asks for the same values."""
-    key = (i, j, ell_1, ell_2, L, k3_bins, np.asarray(k_eff, dtype=np.float64).tobytes(),
-           np.asarray(k3_eff, dtype=np.float64).tobytes())
+    Memoised: k_eff depends only on the grid and the bins, so every mock of
+    a campaign asks for the same values.  The device binning sums |k| with
+    FP64 atomics, so k_eff of two calls can differ in the last bits; the key
+    holds the float32 images of k_eff and k3_eff (a q computed at the first
+    call's k_eff differs from the exact one by ~1e-16 relative)."""
+    key = (i, j, ell_1, ell_2, L, k3_bins, np.asarray(k_eff, dtype=np.float32).tobytes(),
+           np.asarray(k3_eff, dtype=np.float32).tobytes())
     hit = _Q_CACHE.get(key)
     if hit is None:
         if len(_Q_CACHE) > 65536:

@@ -153,6 +153,14 @@ def test_magnetic_configs_and_q_ells():
     q = A.get_q_ells_eff(0, 3, k_eff, 0, 0, 0, 7, k3_eff)
     inside = (k3_eff >= abs(k_eff[0] - k_eff[3])) & (k3_eff <= k_eff[0] + k_eff[3])
     np.testing.assert_allclose(q, inside.astype(float), atol=1e-15)
+    # k_eff from the device binning differs between calls in the last bits
+    # (atomic sums): the memo still hits and stays at the exact values
+    A._Q_CACHE.clear()
+    A.get_q_ells_eff(1, 2, k_eff, 2, 2, 2, 7, k3_eff)
+    keff2, k3eff2 = np.nextafter(k_eff, 1.0), np.nextafter(k3_eff, 0.0)
+    hit = A.get_q_ells_eff(1, 2, keff2, 2, 2, 2, 7, k3eff2)
+    assert len(A._Q_CACHE) == 1
+    np.testing.assert_allclose(hit, OA.get_q_ells_eff(1, 2, keff2, 2, 2, 2, 7, k3eff2), rtol=1e-12, atol=1e-14)
 
 
 def test_compensations_closed_form():
