@@ -267,7 +267,11 @@ def main():
         "data": "synthetic (seeded lognormal + linear RSD catalog)",
         "config": {"workload": f"bk_sugi box full {n}^3 pcs interlaced, {K} k-bins 0.01-0.31, "
                                f"angu [0,0,0]+[2,0,2], {len(pos)} particles, box 2000",
-                   "bins_per_step": bins_per_step, "paint_fp32_ms": t_paint32 * 1e3},
+                   "bins_per_step": bins_per_step, "paint_fp32_ms": t_paint32 * 1e3,
+                   "step": ("both configurations in one pass (shared shell legs, two-output contraction)"
+                            if graph_info.get("shared_vs_apart_max_rel_diff", 1.0) <= 1e-5
+                            else "configurations one at a time")
+                           + (", hipGraph replay" if res_bf16 is not res_eager else ", eager launches")},
         "roofline": roof["roofline"],
         "cpu_baseline": cpu,
         "extra": dict(extra, autotune=autotune_choices()),

@@ -29,7 +29,7 @@ LAST_TIMINGS = {}
 
 
 def _sync_time(ctx):
-    torch.cuda.synchronize(ctx.device)
+    ctx.sync()  # the context's stream only (a copy stream may be busy)
     return time.perf_counter()
 
 

@@ -13,11 +13,14 @@ of a chunk of catalogs instead:
   and result dict is the one ``run_stats`` would write for that catalog.
 
 Catalogs whose outputs all exist already are skipped, so an interrupted
-campaign resumes where it stopped.
+campaign resumes where it stopped.  The next chunk's catalogs are read and
+copied to the device by a prefetch thread on a copy stream while the
+current chunk is measured.
 """
 import logging
 import os
 import time
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 import torch
@@ -36,6 +39,33 @@ LAST_TIMINGS = {}
 
 def _task_kwargs(cfg):
     return {k: v for k, v in cfg.items() if k not in ("statistic", "correlation_mode", "geometry", "catalogs")}
+
+
+def _stage(chunk, base, arrays, device, stream):
+    """Reads the catalogs of ``chunk`` and copies each into its slice of one
+    device array pair on ``stream`` (prefetch thread).  Returns the host
+    arrays, the offsets, the device arrays and the copy's completion event."""
+    pos_l, w_l = [], []
+    for path in chunk:
+        if arrays is not None and path in arrays:
+            pos, w = arrays[path]
+        else:
+            pos, w = read_box_catalog(path, base["column_names"], apply_rsd=base.get("apply_rsd", False),
+                                      para_cosmo=base.get("cosmology"), boxsize=base["boxsize"],
+                                      redshift_box=base.get("redshift_box"), los=base.get("rsd"))
+        pos_l.append(np.ascontiguousarray(pos, dtype=np.float64))
+        w_l.append(np.ascontiguousarray(w, dtype=np.float64))
+    off = np.concatenate([[0], np.cumsum([len(w) for w in w_l])]).astype(np.int64)
+    with torch.cuda.device(device), torch.cuda.stream(stream):
+        pos_d = torch.empty((int(off[-1]), 3), dtype=torch.float64, device=device)
+        w_d = torch.empty(int(off[-1]), dtype=torch.float64, device=device)
+        for b in range(len(chunk)):
+            if off[b + 1] > off[b]:
+                pos_d[off[b]:off[b + 1]].copy_(torch.from_numpy(pos_l[b]))
+                w_d[off[b]:off[b + 1]].copy_(torch.from_numpy(w_l[b]))
+        done = torch.cuda.Event()
+        done.record(stream)
+    return pos_l, w_l, off, pos_d, w_d, done
 
 
 def run_mocks(configs, catalogs, batch=16, skip_existing=True, arrays=None):
@@ -81,58 +111,63 @@ def run_mocks(configs, catalogs, batch=16, skip_existing=True, arrays=None):
     sampler, interlaced = base["sampler"], bool(base["interlaced"])
     results = {}
     t_paint = t_est = 0.0
-    for c0 in range(0, len(todo), batch):
-        chunk = todo[c0:c0 + batch]
-        t0 = time.perf_counter()
-        pos_l, w_l = [], []
-        for path in chunk:
-            if arrays is not None and path in arrays:
-                pos, w = arrays[path]
-            else:
-                pos, w = read_box_catalog(path, base["column_names"], apply_rsd=base.get("apply_rsd", False),
-                                          para_cosmo=base.get("cosmology"), boxsize=base["boxsize"],
-                                          redshift_box=base.get("redshift_box"), los=base.get("rsd"))
-            pos_l.append(np.ascontiguousarray(pos, dtype=np.float64))
-            w_l.append(np.ascontiguousarray(w, dtype=np.float64))
-        off = np.concatenate([[0], np.cumsum([len(w) for w in w_l])]).astype(np.int64)
-        # each catalog goes straight to its slice of the device arrays (no
-        # host-side concatenation of the chunk)
-        pos_d = torch.empty((int(off[-1]), 3), dtype=torch.float64, device=ctx.device)
-        w_d = torch.empty(int(off[-1]), dtype=torch.float64, device=ctx.device)
-        for b in range(len(chunk)):
-            if off[b + 1] > off[b]:
-                pos_d[off[b]:off[b + 1]].copy_(torch.from_numpy(pos_l[b]))
-                w_d[off[b]:off[b + 1]].copy_(torch.from_numpy(w_l[b]))
-        nb = len(chunk)
-        m0 = ctx.mesh(n, prec, nb)
-        m1 = ctx.mesh(n, prec, nb) if interlaced else None
-        ctx.paint_batch(pos_d, w_d, off, n, L, sampler, interlaced, prec, m0, m1)
-        del pos_d, w_d
-        s1 = ctx.spectrum(n, prec, nb)
-        ctx.r2c(m0, s1, n, prec, nb)
-        del m0
-        s2 = None
-        if interlaced:
-            s2 = ctx.spectrum(n, prec, nb)
-            ctx.r2c(m1, s2, n, prec, nb)
-            del m1
-        torch.cuda.synchronize(ctx.device)
-        t1 = time.perf_counter()
-        for b, path in enumerate(chunk):
-            field = DeviceField(s1[b], s2[b] if interlaced else None, n, L, prec, sampler, interlaced)
-            attrs = {"NZ_a": np.sum(w_l[b]) / V, "N_gal_a": len(w_l[b]), "interlaced": interlaced,
-                     "sampler": sampler}
-            res = []
-            for c in configs:
-                cfg = dict(c)
-                cfg["catalogs"] = {"data_a": path}
-                cfg["fields"] = (attrs, field)
-                cfg["gc_collect"] = False
-                res.append(run_task(**cfg))
-            results[path] = res
-        torch.cuda.synchronize(ctx.device)
-        t_paint += t1 - t0
-        t_est += time.perf_counter() - t1
-        del s1, s2
+    chunks = [todo[c0:c0 + batch] for c0 in range(0, len(todo), batch)]
+    main = torch.cuda.current_stream(ctx.device)  # the context's stream
+    copy_stream = torch.cuda.Stream(ctx.device)
+    pool = ThreadPoolExecutor(max_workers=1)
+    try:
+        nxt = pool.submit(_stage, chunks[0], base, arrays, ctx.device, copy_stream)
+        for ci, chunk in enumerate(chunks):
+            t0 = time.perf_counter()
+            pos_l, w_l, off, pos_d, w_d, done = nxt.result()
+            if ci + 1 < len(chunks):  # read and copy the next chunk meanwhile
+                nxt = pool.submit(_stage, chunks[ci + 1], base, arrays, ctx.device, copy_stream)
+            main.wait_event(done)
+            # allocated on the copy stream, used on the context stream: the
+            # allocator must not reuse them before the paint has read them
+            pos_d.record_stream(main)
+            w_d.record_stream(main)
+            t_p, t_e = _measure_chunk(ctx, configs, chunk, pos_l, w_l, off, pos_d, w_d, n, L, V, prec, sampler,
+                                      interlaced, results, t0)
+            del pos_d, w_d
+            t_paint += t_p
+            t_est += t_e
+    finally:
+        pool.shutdown(wait=True)
     LAST_TIMINGS.update({"paint_fft_s": t_paint, "estimators_s": t_est})
     return results
+
+
+def _measure_chunk(ctx, configs, chunk, pos_l, w_l, off, pos_d, w_d, n, L, V, prec, sampler, interlaced,
+                   results, t0):
+    """Paint + batched r2c of one chunk, then every config on every catalog
+    of it; returns (paint + FFT seconds, estimator seconds).  Waits on the
+    context stream only (the copy stream may be filling the next chunk)."""
+    main = torch.cuda.current_stream(ctx.device)
+    nb = len(chunk)
+    m0 = ctx.mesh(n, prec, nb)
+    m1 = ctx.mesh(n, prec, nb) if interlaced else None
+    ctx.paint_batch(pos_d, w_d, off, n, L, sampler, interlaced, prec, m0, m1)
+    s1 = ctx.spectrum(n, prec, nb)
+    ctx.r2c(m0, s1, n, prec, nb)
+    del m0
+    s2 = None
+    if interlaced:
+        s2 = ctx.spectrum(n, prec, nb)
+        ctx.r2c(m1, s2, n, prec, nb)
+        del m1
+    main.synchronize()
+    t1 = time.perf_counter()
+    for b, path in enumerate(chunk):
+        field = DeviceField(s1[b], s2[b] if interlaced else None, n, L, prec, sampler, interlaced)
+        attrs = {"NZ_a": np.sum(w_l[b]) / V, "N_gal_a": len(w_l[b]), "interlaced": interlaced, "sampler": sampler}
+        res = []
+        for c in configs:
+            cfg = dict(c)
+            cfg["catalogs"] = {"data_a": path}
+            cfg["fields"] = (attrs, field)
+            cfg["gc_collect"] = False
+            res.append(run_task(**cfg))
+        results[path] = res
+    main.synchronize()
+    return t1 - t0, time.perf_counter() - t1

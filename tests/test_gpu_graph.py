@@ -65,6 +65,26 @@ def test_shared_legs_equal_configurations_apart(gpu_ctx):
         assert err < 1e-5
 
 
+def test_multi_configuration_pass_without_shared_legs(gpu_ctx):
+    """Configurations with complex shell components and several magnetic
+    configurations ([1,1,0], [2,2,2], [2,0,2] beside [0,0,0]) in one
+    multi-configuration call equal the calls one at a time (1e-5 of max |S|:
+    only the FP32 summation order of the contraction may differ)."""
+    from cosmonpc_amd.estimators import bk_box_g00, bk_box_signal_auto, bk_box_signal_multi
+
+    ctx = gpu_ctx
+    X = _spectrum(ctx, 9)
+    angus = [(1, 1, 0), (0, 0, 0), (2, 2, 2), (2, 0, 2)]
+    src = X.clone()
+    G = bk_box_g00(ctx, src, N, 1)
+    multi = bk_box_signal_multi(ctx, src, G, N, L, 1, EDGES, angus, "bf16x3")
+    for a, got in zip(angus, multi):
+        want = bk_box_signal_auto(ctx, src, G, N, L, 1, EDGES, a, "bf16x3")
+        err = np.max(np.abs(got - want)) / np.max(np.abs(want))
+        _log(f"multi {a}: rel max diff {err:.2e}")
+        assert np.max(np.abs(want)) > 0 and err < 1e-5
+
+
 def test_graph_replays_match_eager(gpu_ctx):
     import torch
 
