@@ -263,6 +263,23 @@ def get_q_ells_eff(i, j, k_eff, ell_1, ell_2, L, k3_bins, k3_eff):
     return hit.copy()
 
 
+def get_q_ells(i, j, k_center, k_min, k_max, k_bins, ell_1, ell_2, L, k3_bins):
+    """q_{l1 l2 L} at the bin centres k_center[i], k_center[j] and the k3 bin
+    centres of linspace(2 k_min, 2 k_max, 2 k_bins + 1)[:-1]
+    (``math_evaluator.py:560-589`` with ``:389-415``; the S3 model of
+    ``shotnoise_ana_all_eff=False``).  Memoised."""
+    e = np.linspace(2 * k_min, 2 * k_max, k_bins * 2 + 1)[:-1]
+    k3_center = 0.5 * (e[1:] + e[:-1])
+    key = ("centres", i, j, ell_1, ell_2, L, k3_bins, float(k_min), float(k_max), int(k_bins),
+           np.asarray(k_center, dtype=np.float64).tobytes())
+    hit = _Q_CACHE.get(key)
+    if hit is None:
+        if len(_Q_CACHE) > 65536:
+            _Q_CACHE.clear()
+        hit = _Q_CACHE[key] = _q_ells_eff(i, j, k_center, ell_1, ell_2, L, k3_bins, k3_center)
+    return hit.copy()
+
+
 def _q_ells_eff(i, j, k_eff, ell_1, ell_2, L, k3_bins, k3_eff):
     q = np.zeros(k3_bins)
     lmin = min(ell_1, ell_2)

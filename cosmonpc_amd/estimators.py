@@ -679,14 +679,17 @@ def calculate_bk_sugi_box(rfield_a, rfield_b, rfield_c, correlation_mode, stat_a
         def s3_ana():
             if tracer_type == "abb":
                 return 0.0
-            if not all_eff:
-                raise NotImplementedError("shotnoise_ana_all_eff=False is not supported")
+
+            def q_of(i, j):  # effective k (default) or bin centres (:1102-1126)
+                if all_eff:
+                    return A.get_q_ells_eff(i, j, k_eff, ell_1, ell_2, L, k3_bins, k3_eff)
+                return A.get_q_ells(i, j, k_center, k_min, k_max, k_bins, ell_1, ell_2, L, k3_bins)
             with np.errstate(invalid="ignore", divide="ignore"):
                 weighted_k_num = k_num_k3 / k3_center
             if dvm == "diagonal":
                 s3 = np.zeros(k_bins, dtype="complex128")
                 for i in range(k_bins):
-                    q = A.get_q_ells_eff(i, i, k_eff, ell_1, ell_2, L, k3_bins, k3_eff)
+                    q = q_of(i, i)
                     numer = np.sum(q * total_sum / k3_center)
                     denom = np.sum(weighted_k_num[: 2 * i + 1])
                     if denom > 0:
@@ -695,7 +698,7 @@ def calculate_bk_sugi_box(rfield_a, rfield_b, rfield_c, correlation_mode, stat_a
                 s3 = np.zeros((k_bins, k_bins), dtype="complex128")
                 for i in range(k_bins):
                     for j in range(k_bins):
-                        q = A.get_q_ells_eff(i, j, k_eff, ell_1, ell_2, L, k3_bins, k3_eff)
+                        q = q_of(i, j)
                         valid = A.get_valid_k3_bins(k_center[i], k_center[j], k_min, k_max, k_bins)
                         numer = np.sum(q * total_sum / k3_center)
                         denom = np.sum(weighted_k_num * valid)

@@ -235,6 +235,36 @@ def get_associated_legendre_coefficients_eff(ell, m, k1, k2, k3_eff, mode="13"):
     return res
 
 
+def get_associated_legendre_coefficients(ell, m, k1, k2, k_min, k_max, kbin, mode="13"):
+    """``math_evaluator.py:389-415``: as the effective-k form, at the k3 bin
+    centres of linspace(2 k_min, 2 k_max, 2 kbin + 1)[:-1]."""
+    e = np.linspace(2 * k_min, 2 * k_max, kbin * 2 + 1)[:-1]
+    c = 0.5 * (e[1:] + e[:-1])
+    res = np.zeros(len(c))
+    for i, k3 in enumerate(c):
+        if abs(k1 - k2) <= k3 <= (k1 + k2):
+            if mode == "13":
+                mu = (k3 ** 2 + k1 ** 2 - k2 ** 2) / (2 * k1 * k3)
+            else:
+                mu = (k3 ** 2 + k2 ** 2 - k1 ** 2) / (2 * k2 * k3)
+            res[i] = assoc_legendre(ell, m, mu)
+    res *= (-1) ** (ell + m)
+    return res
+
+
+def get_q_ells(i, j, k_center, k_min, k_max, k_bins, ell_1, ell_2, L, k3_bins):
+    """``get_q_ells`` (``math_evaluator.py:560-589``): q at the bin centres
+    (``shotnoise_ana_all_eff=False``)."""
+    q = np.zeros(k3_bins).astype("f8")
+    lmin = min(ell_1, ell_2)
+    for xx in range(-lmin, lmin + 1):
+        tj = wigner_3j(ell_1, ell_2, L, xx, -xx, 0)
+        a13 = get_associated_legendre_coefficients(ell_1, xx, k_center[i], k_center[j], k_min, k_max, k_bins, "13")
+        a23 = get_associated_legendre_coefficients(ell_2, -xx, k_center[i], k_center[j], k_min, k_max, k_bins, "23")
+        q += (-1) ** xx * tj * a13 * a23
+    return q
+
+
 def get_valid_k3_bins(k1, k2, k_min, k_max, kbin):
     """``math_evaluator.py:448-452``."""
     e = np.linspace(2 * k_min, 2 * k_max, kbin * 2 + 1)[:-1]

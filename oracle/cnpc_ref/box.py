@@ -212,7 +212,7 @@ def calculate_bk_sugi_box(rfield_a, rfield_b, rfield_c, correlation_mode, stat_a
                 if all_eff:
                     q = A.get_q_ells_eff(i, i, k_eff, ell_1, ell_2, L, k3_bins, k3_eff)
                 else:
-                    raise NotImplementedError("shotnoise_ana_all_eff=False")
+                    q = A.get_q_ells(i, i, k_center, k_min, k_max, k_bins, ell_1, ell_2, L, k3_bins)
                 numer = np.sum(q * total_sum / k3_center)
                 denom = np.sum(weighted_k_num[: 2 * i + 1])
                 if denom > 0:
@@ -224,7 +224,7 @@ def calculate_bk_sugi_box(rfield_a, rfield_b, rfield_c, correlation_mode, stat_a
                     if all_eff:
                         q = A.get_q_ells_eff(i, j, k_eff, ell_1, ell_2, L, k3_bins, k3_eff)
                     else:
-                        raise NotImplementedError("shotnoise_ana_all_eff=False")
+                        q = A.get_q_ells(i, j, k_center, k_min, k_max, k_bins, ell_1, ell_2, L, k3_bins)
                     valid = A.get_valid_k3_bins(k_center[i], k_center[j], k_min, k_max, k_bins)
                     numer = np.sum(q * total_sum / k3_center)
                     denom = np.sum(weighted_k_num * valid)

@@ -235,6 +235,34 @@ def test_bk_box_reduced_grid_matches_oracle(gpu_ctx, angu):
     assert rel_to_max(got["B_sugi"], ref["B_sugi"]) < 1e-10
 
 
+@pytest.mark.parametrize("dvm", ["full", "diagonal"])
+@pytest.mark.parametrize("angu", [(0, 0, 0), (2, 0, 2)])
+def test_bk_box_s3_ana_bin_centres(gpu_ctx, angu, dvm):
+    """``shotnoise_ana_all_eff=False``: the analytic S3 with q at the bin
+    centres (clustering_estimator.py:1102-1126, :1137-1160) against the
+    oracle.  For l > 0 it differs from the effective-k default (q_000 is the
+    triangle indicator, the same at centres and effective k here)."""
+    from cosmonpc_amd.estimators import calculate_bk_sugi_box
+    from cosmonpc_amd.mesh import paint_field
+
+    _, _, B = _oracle()
+    n, L = 32, 300.0
+    pos = catalog(30000, L, 13, ngrid=n)
+    w = np.ones(len(pos))
+    attrs, fa, _, _ = B.get_mesh_box({"a": (pos, w)}, [n] * 3, [L] * 3, "tsc", True)
+    sa = dict(attrs)
+    sa.update(bk_attrs(n, L, 0.03, 0.15, 6, angu, dvm, "tsc", True, contraction="gemm_fp64",
+                       shotnoise_ana_all_eff=False))
+    ref = B.calculate_bk_sugi_box(fa, None, None, "auto", sa)
+    f = paint_field(gpu_ctx, pos, w, n, L, "tsc", True, 0)
+    got = calculate_bk_sugi_box(f, None, None, "auto", sa)
+    assert rel_to_max(got["SN_terms"]["SN3"], ref["SN_terms"]["SN3"]) < 1e-10
+    assert rel_to_max(got["B_sugi"], ref["B_sugi"]) < 1e-10
+    if angu != (0, 0, 0):
+        eff = calculate_bk_sugi_box(f, None, None, "auto", dict(sa, shotnoise_ana_all_eff=True))
+        assert rel_to_max(eff["SN_terms"]["SN3"], got["SN_terms"]["SN3"]) > 1e-6
+
+
 def test_bk_box_reduced_grid_s3_fft(gpu_ctx):
     from cosmonpc_amd.estimators import calculate_bk_sugi_box
     from cosmonpc_amd.mesh import paint_field

@@ -153,6 +153,19 @@ def test_magnetic_configs_and_q_ells():
     q = A.get_q_ells_eff(0, 3, k_eff, 0, 0, 0, 7, k3_eff)
     inside = (k3_eff >= abs(k_eff[0] - k_eff[3])) & (k3_eff <= k_eff[0] + k_eff[3])
     np.testing.assert_allclose(q, inside.astype(float), atol=1e-15)
+    # bin-centre form (shotnoise_ana_all_eff=False): q_000 is the triangle
+    # indicator at the k3 bin centres; the product equals the oracle
+    k_min, k_max, kb = 0.01, 0.09, 4
+    kc = 0.5 * (np.linspace(k_min, k_max, kb + 1)[1:] + np.linspace(k_min, k_max, kb + 1)[:-1])
+    e3 = np.linspace(2 * k_min, 2 * k_max, 2 * kb + 1)[:-1]
+    c3 = 0.5 * (e3[1:] + e3[:-1])
+    q = OA.get_q_ells(0, 3, kc, k_min, k_max, kb, 0, 0, 0, len(c3))
+    np.testing.assert_allclose(q, ((c3 >= abs(kc[0] - kc[3])) & (c3 <= kc[0] + kc[3])).astype(float), atol=1e-15)
+    for cfg in ([0, 0, 0], [2, 0, 2], [1, 1, 0], [2, 2, 2]):
+        for i in range(kb):
+            for j in range(kb):
+                np.testing.assert_allclose(A.get_q_ells(i, j, kc, k_min, k_max, kb, *cfg, len(c3)),
+                                           OA.get_q_ells(i, j, kc, k_min, k_max, kb, *cfg, len(c3)), atol=1e-13)
     # k_eff from the device binning differs between calls in the last bits
     # (atomic sums): the memo still hits and stays at the exact values
     A._Q_CACHE.clear()
