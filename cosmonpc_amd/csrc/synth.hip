@@ -27,6 +27,12 @@
 
 namespace cnpc {
 
+// threads per workgroup of the x, y and z passes
+#ifndef CNPC_SYNTH_THREADS
+#define CNPC_SYNTH_THREADS 256
+#endif
+constexpr int SYNTH_THREADS = CNPC_SYNTH_THREADS;
+
 // ------------------------------------------------------------------ cube
 template <typename T>
 __global__ void __launch_bounds__(256) cube_kernel(SynthArgs a, BinEdges E, PolySpec W,
@@ -68,7 +74,7 @@ __global__ void __launch_bounds__(256) cube_kernel(SynthArgs a, BinEdges E, Poly
 // ------------------------------------------------------------------ x pass
 // one workgroup per (block of NL ky lines, kz); line l = ky index ky0 + l
 template <typename T>
-__global__ void __launch_bounds__(256) xfft_kernel(SynthArgs a, int NL, const typename cplx<T>::type* __restrict__ V,
+__global__ void __launch_bounds__(SYNTH_THREADS) xfft_kernel(SynthArgs a, int NL, const typename cplx<T>::type* __restrict__ V,
                                                    const signed char* __restrict__ bins,
                                                    const typename cplx<T>::type* __restrict__ twg,
                                                    typename cplx<T>::type* __restrict__ U) {
@@ -137,7 +143,7 @@ __global__ void __launch_bounds__(256) xfft_kernel(SynthArgs a, int NL, const ty
 // layout H[x][y/2][slot0 + s][kz][y%2] of the fused z-synthesis contraction
 // (each (x, y pair) tile one contiguous run over all slots).
 template <typename T>
-__global__ void __launch_bounds__(256) yfft_kernel(SynthArgs a, int NL, const typename cplx<T>::type* __restrict__ U,
+__global__ void __launch_bounds__(SYNTH_THREADS) yfft_kernel(SynthArgs a, int NL, const typename cplx<T>::type* __restrict__ U,
                                                    const typename cplx<T>::type* __restrict__ twg,
                                                    typename cplx<T>::type* __restrict__ Tm, int nslots, int slot0) {
   typedef typename cplx<T>::type C;
@@ -201,7 +207,7 @@ __global__ void __launch_bounds__(256) yfft_kernel(SynthArgs a, int NL, const ty
 // one workgroup per (plane s*n + x, block of 2*NL rows); line l carries rows
 // y0+2l (real part) and y0+2l+1 (imaginary part)
 template <typename T>
-__global__ void __launch_bounds__(256) zfft_kernel(SynthArgs a, int NL, const typename cplx<T>::type* __restrict__ Tm,
+__global__ void __launch_bounds__(SYNTH_THREADS) zfft_kernel(SynthArgs a, int NL, const typename cplx<T>::type* __restrict__ Tm,
                                                    const typename cplx<T>::type* __restrict__ twg,
                                                    T* __restrict__ out) {
   typedef typename cplx<T>::type C;
@@ -299,11 +305,12 @@ int synth_lines_per_block(int prec, const SynthArgs& a) {
   int nl = (int)((budget - (size_t)n * cs) / ((size_t)(n + 1) * cs));
   const int maxb = prec == 0 ? MaxB<double>::v : MaxB<float>::v;
   for (int s = 0; s < a.nrad; ++s) {
-    // nl * n / R butterflies <= maxb * 256 (one per thread for the composite radices)
-    const int lim = (wide_radix(a.rad[s]) ? 1 : maxb) * 256 * a.rad[s] / n;
+    // nl * n / R butterflies <= maxb * threads (one per thread for the composite radices)
+    const int lim = (wide_radix(a.rad[s]) ? 1 : maxb) * SYNTH_THREADS * a.rad[s] / n;
     if (nl > lim) nl = lim;
   }
-  return nl > 16 ? 16 : nl;
+  const int cap = 16 * SYNTH_THREADS / 256;
+  return nl > cap ? cap : nl;
 }
 
 template <typename T>
@@ -322,11 +329,12 @@ static hipError_t launch_synth_t(const SynthArgs& a, const BinEdges& E, const Po
   const int NL = synth_lines_per_block(prec, a);
   const size_t lds = sizeof(C) * ((size_t)n + (size_t)NL * (n + 1));
   const int64_t planes = (int64_t)a.nshell * n;
-  hipLaunchKernelGGL(xfft_kernel<T>, dim3((P + NL - 1) / NL, Q), blk, lds, st, a, NL, (const C*)V, bins, (const C*)tw,
-                     U);
-  hipLaunchKernelGGL(yfft_kernel<T>, dim3((unsigned)planes, (Q + NL - 1) / NL), blk, lds, st, a, NL, (const C*)U,
+  const dim3 fblk(SYNTH_THREADS);
+  hipLaunchKernelGGL(xfft_kernel<T>, dim3((P + NL - 1) / NL, Q), fblk, lds, st, a, NL, (const C*)V, bins,
+                     (const C*)tw, U);
+  hipLaunchKernelGGL(yfft_kernel<T>, dim3((unsigned)planes, (Q + NL - 1) / NL), fblk, lds, st, a, NL, (const C*)U,
                      (const C*)tw, Tm, 0, 0);
-  hipLaunchKernelGGL(zfft_kernel<T>, dim3((unsigned)planes, (n + 2 * NL - 1) / (2 * NL)), blk, lds, st, a, NL,
+  hipLaunchKernelGGL(zfft_kernel<T>, dim3((unsigned)planes, (n + 2 * NL - 1) / (2 * NL)), fblk, lds, st, a, NL,
                      (const C*)Tm, (const C*)tw, (T*)out);
   return hipGetLastError();
 }
@@ -359,9 +367,10 @@ hipError_t launch_synth_half(const SynthArgs& a, const BinEdges& E, const PolySp
   const int NL = synth_lines_per_block(1, a);
   const size_t lds = sizeof(C) * ((size_t)n + (size_t)NL * (n + 1));
   const int64_t planes = (int64_t)a.nshell * n;
-  hipLaunchKernelGGL(xfft_kernel<float>, dim3((P + NL - 1) / NL, Q), blk, lds, st, a, NL, (const C*)V, bins,
+  const dim3 fblk(SYNTH_THREADS);
+  hipLaunchKernelGGL(xfft_kernel<float>, dim3((P + NL - 1) / NL, Q), fblk, lds, st, a, NL, (const C*)V, bins,
                      (const C*)tw, (C*)work);
-  hipLaunchKernelGGL(yfft_kernel<float>, dim3((unsigned)planes, (Q + NL - 1) / NL), blk, lds, st, a, NL,
+  hipLaunchKernelGGL(yfft_kernel<float>, dim3((unsigned)planes, (Q + NL - 1) / NL), fblk, lds, st, a, NL,
                      (const C*)work, (const C*)tw, (C*)out, nslots, slot0);
   return hipGetLastError();
 }

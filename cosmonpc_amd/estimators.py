@@ -248,8 +248,10 @@ class ZBatch:
             if r.zargs[7] is None or r.value is not None:
                 continue
             z = r.zargs
-            mate = next((s for s in same if s.value is None and s.zargs[3] is z[3] and s.zargs[4] is z[7]
-                         and s.zargs[5] == z[8] and np.array_equal(s.zargs[6][:z[8]], z[9][:z[8]])), None)
+            # same grid, box and k_max, the same G, X the very buffer and rows
+            mate = next((s for s in same if s.value is None and s.zargs[:3] == z[:3] and s.zargs[3] is z[3]
+                         and s.zargs[4] is z[7] and s.zargs[5] == z[8]
+                         and np.array_equal(s.zargs[6][:z[8]], z[9][:z[8]])), None)
             if mate is not None:
                 r.value, mate.value = self.ctx.contract_zgemm(*z, nblk=_zgemm_blocks(self.ctx, z, True),
                                                              with_bb=True)
