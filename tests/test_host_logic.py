@@ -242,3 +242,33 @@ def test_autotune_cache_and_disable(tmp_path, monkeypatch):
     assert autotune.choose("gemm_blocks", (1, 2), [2048, 1024], ran.append) == 4096
     assert autotune.choose("synth_half_batch", (400, 99, 30), [4, 8], ran.append) == 4
     assert ran == [] and autotune.lookup("synth_half_batch", (400, 99, 30)) is None
+
+
+def test_zbatch_pairs_same_and_pair_requests(monkeypatch):
+    """A B == A request on shells X and a B != A request (Y, X) with the same
+    G, X buffer, rows and grid become one two-output launch; a request with
+    another G, or rows that differ, is launched on its own.  The frozen
+    autotuner times nothing."""
+    from cosmonpc_amd import autotune, estimators
+
+    calls = []
+
+    class FakeCtx:
+        def contract_zgemm(self, *z, nblk=0, with_bb=False):
+            calls.append((z[4], z[7], with_bb))
+            return ("C", "Cbb") if with_bb else "C"
+
+    monkeypatch.setattr(estimators, "_zgemm_blocks", lambda ctx, z, bb=False: 0)
+    G, G2, X, Y, Z = object(), object(), object(), object(), object()
+    kz = np.arange(4, dtype=np.int32)
+    b = estimators.ZBatch(FakeCtx())
+    same = b.request((40, 500.0, 0.1, G, X, 4, kz, None, 4, None))
+    pair = b.request((40, 500.0, 0.1, G, Y, 4, kz, X, 4, kz))
+    other_g = b.request((40, 500.0, 0.1, G2, Z, 4, kz, X, 4, kz))
+    lone = b.request((40, 500.0, 0.1, G, Z, 3, kz[:3], None, 3, None))
+    with autotune.frozen():
+        b.run()
+    assert (Y, X, True) in calls and len([c for c in calls if c[2]]) == 1
+    assert pair.value == "C" and same.value == "Cbb"
+    assert other_g.value == "C" and lone.value == "C"
+    assert len(calls) == 3 and b.requests == []
