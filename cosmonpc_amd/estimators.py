@@ -528,7 +528,8 @@ def bk_box_g00(ctx, delta, n, prec):
 
 
 def bk_box_signal_auto(ctx, delta, G, n, boxsize, prec, k_edge, angu_config, contraction, dvm="full",
-                       block_size="full", zsynth="fused", defer=False, shell_cache=None, zbatch=None):
+                       block_size="full", zsynth="fused", defer=False, shell_cache=None, zbatch=None,
+                       shell_method="synth"):
     """Signal stage of an auto-correlation box bispectrum: from the
     compensated delta_k and G_00 resident in HBM to ``total_res``
     (``clustering_estimator.py:867-1061``) for one angular configuration.
@@ -537,12 +538,12 @@ def bk_box_signal_auto(ctx, delta, G, n, boxsize, prec, k_edge, angu_config, con
     mcfg, tjv = A.get_magnetic_configs_box(ell_1, ell_2, L)
     k_bins = len(k_edge) - 1
     return _bk_signal(ctx, delta, delta, G, (n, boxsize, prec, np.asarray(k_edge), k_bins), mcfg, tjv, ell_1,
-                      ell_2, True, _gemm_mode(contraction, prec), block_size, dvm, zsynth=zsynth, defer=defer,
-                      shell_cache=shell_cache, zbatch=zbatch)
+                      ell_2, True, _gemm_mode(contraction, prec), block_size, dvm, shell_method=shell_method,
+                      zsynth=zsynth, defer=defer, shell_cache=shell_cache, zbatch=zbatch)
 
 
 def bk_box_signal_multi(ctx, delta, G, n, boxsize, prec, k_edge, angu_configs, contraction, dvm="full",
-                        block_size="full", zsynth="fused", defer=False):
+                        block_size="full", zsynth="fused", defer=False, shell_method="synth"):
     """``bk_box_signal_auto`` for several angular configurations of the same
     field (the reference runs one ``run_stats`` per configuration).  With the
     fused contraction, shells that several configurations use are
@@ -555,7 +556,8 @@ def bk_box_signal_multi(ctx, delta, G, n, boxsize, prec, k_edge, angu_configs, c
     cache = {}
     zbatch = ZBatch(ctx)
     fins = [bk_box_signal_auto(ctx, delta, G, n, boxsize, prec, k_edge, a, contraction, dvm, block_size, zsynth,
-                               defer=True, shell_cache=cache, zbatch=zbatch) for a in angu_configs]
+                               defer=True, shell_cache=cache, zbatch=zbatch, shell_method=shell_method)
+            for a in angu_configs]
     zbatch.run()
     cache.clear()  # freed buffers are reused in stream order, after the work that reads them
 
@@ -563,6 +565,55 @@ def bk_box_signal_multi(ctx, delta, G, n, boxsize, prec, k_edge, angu_configs, c
         return [f() for f in fins]
 
     return finish if defer else finish()
+
+
+SIGNAL_KEYS = ("k_min", "k_max", "k_bins", "data_vector_mode", "block_size", "contraction", "shell_method",
+               "zsynth", "shell_grid", "precision")
+
+
+def signal_group_key(cfg):
+    """Settings that must agree for box auto-correlation bk_sugi
+    configurations to share one multi-configuration signal pass (None for
+    configurations that cannot: cross-correlation, survey, not bk_sugi)."""
+    if cfg.get("statistic") != "bk_sugi" or cfg.get("correlation_mode") != "auto" or \
+            cfg.get("geometry") != "box-like":
+        return None
+    dvm = cfg.get("data_vector_mode", "diagonal")
+    prec = cfg.get("precision", "fp64")
+    resolved = dict(cfg, data_vector_mode=dvm, precision=prec,
+                    block_size=cfg.get("block_size", 1 if dvm == "diagonal" else "full"),
+                    contraction=cfg.get("contraction", "gemm_fp64" if prec == "fp64" else "bf16x3"),
+                    shell_method=cfg.get("shell_method", "synth"), zsynth=cfg.get("zsynth", "fused"),
+                    shell_grid=cfg.get("shell_grid", "reduced"))
+    return tuple(repr(resolved[k]) for k in SIGNAL_KEYS)
+
+
+def bk_box_signals(field, stat_attrs_list):
+    """``total_res`` of several box auto-correlation bk_sugi configurations
+    of one painted field (``signal_group_key`` equal), from one
+    ``bk_box_signal_multi`` pass: the delta_k, G_00 and reduced-grid spectra
+    are made once, shells shared by several configurations are synthesised
+    once and [0,0,0] comes out of [l,0,l]'s contraction.  Each entry is what
+    ``calculate_bk_sugi_box`` computes for that configuration
+    (``clustering_estimator.py:808-1061``), up to the FP32 summation order of
+    the split-bf16 contraction; pass it back as ``signal=``."""
+    ctx = get_context()
+    sa = stat_attrs_list[0]
+    dvm = sa.get("data_vector_mode", "diagonal")
+    block_size = sa.get("block_size", 1 if dvm == "diagonal" else "full")
+    n, Lbox, prec = field.n, field.boxsize, field.prec
+    k_min, k_max, k_bins = sa["k_min"], sa["k_max"], sa["k_bins"]
+    k_edge = np.linspace(k_min, k_max, k_bins + 1)
+    contraction = _gemm_mode(sa.get("contraction", "gemm_fp64" if prec == 0 else "bf16x3"), prec)
+    vol_per_cell = np.prod(np.array(sa["boxsize"])) / np.prod(np.array(sa["nmesh"]))
+    ca = field.delta_k(ctx, 1, scale=1.0 / vol_per_cell, sub0=sa["NZ_a"])
+    m, nmax = contraction_grid(n, Lbox, k_max, contraction, sa.get("shell_grid", "reduced"))
+    ca_m = reduce_spectra(ctx, [ca], n, prec, m, nmax)[0] if m != n else ca
+    G = bk_box_g00(ctx, ca_m, m, prec)
+    res = bk_box_signal_multi(ctx, ca_m, G, m, Lbox, prec, k_edge, [tuple(s["angu_config"]) for s in stat_attrs_list],
+                              contraction, dvm, block_size, sa.get("zsynth", "fused"),
+                              shell_method=sa.get("shell_method", "synth"))
+    return [(n / m) ** 3 * r for r in res]
 
 
 def calculate_bk_sugi_box(rfield_a, rfield_b, rfield_c, correlation_mode, stat_attrs, comm=None, **kwargs):
@@ -617,23 +668,29 @@ def calculate_bk_sugi_box(rfield_a, rfield_b, rfield_c, correlation_mode, stat_a
     with np.errstate(invalid="ignore", divide="ignore"):
         k_eff = ksum / k_num
     t1 = _sync_time(ctx)
-    # leg 2 shells come from tracer a for aaa/aab, else b (:899-902)
-    src2 = ca if tracer_type in ("aaa", "aab") else cb
-    same_tracer = correlation_mode == "auto" or tracer_type == "aab"
+    signal = kwargs.get("signal")
     # the contraction runs on the reduced grid m (exact up to rounding)
     m, nmax = contraction_grid(n, Lbox, k_max, contraction, stat_attrs.get("shell_grid", "reduced"))
-    if m != n:
-        ca_m, src2_m, gsrc_m = reduce_spectra(ctx, [ca, src2, gsrc], n, prec, m, nmax)
-    else:
-        ca_m, src2_m, gsrc_m = ca, src2, gsrc
     grid_factor = (n / m) ** 3
-    G = bk_box_g00(ctx, gsrc_m, m, prec)
     kinfo = (m, Lbox, prec, k_edge, k_bins)
-    total_res = grid_factor * _bk_signal(ctx, ca_m, src2_m, G, kinfo, mcfg, tjv, ell_1, ell_2, same_tracer,
-                                         contraction, block_size, dvm, shell_method=shell_method, zsynth=zsynth)
+    if signal is not None:
+        # total_res of this configuration from a multi-configuration pass
+        # over the same field (bk_box_signals)
+        total_res = np.asarray(signal)
+    else:
+        # leg 2 shells come from tracer a for aaa/aab, else b (:899-902)
+        src2 = ca if tracer_type in ("aaa", "aab") else cb
+        same_tracer = correlation_mode == "auto" or tracer_type == "aab"
+        if m != n:
+            ca_m, src2_m, gsrc_m = reduce_spectra(ctx, [ca, src2, gsrc], n, prec, m, nmax)
+        else:
+            ca_m, src2_m, gsrc_m = ca, src2, gsrc
+        G = bk_box_g00(ctx, gsrc_m, m, prec)
+        total_res = grid_factor * _bk_signal(ctx, ca_m, src2_m, G, kinfo, mcfg, tjv, ell_1, ell_2, same_tracer,
+                                             contraction, block_size, dvm, shell_method=shell_method, zsynth=zsynth)
+        del G, ca_m, src2_m, gsrc_m
     t2 = _sync_time(ctx)
     LAST_TIMINGS.update({"prepare_s": t1 - t0, "signal_s": t2 - t1, "contraction": contraction, "grid": m})
-    del G, ca_m, src2_m, gsrc_m
 
     # ---------------------------------------------------------- shot noise
     k_center = 0.5 * (k_edge[1:] + k_edge[:-1])

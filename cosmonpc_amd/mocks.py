@@ -10,7 +10,12 @@ of a chunk of catalogs instead:
 * one batched rocFFT r2c per mesh copy transforms the whole chunk;
 * each catalog's spectra then go through the same estimators and writers as
   ``run_stats`` (``run_task`` with the painted field), so every output file
-  and result dict is the one ``run_stats`` would write for that catalog.
+  and result dict is the one ``run_stats`` would write for that catalog;
+* bk_sugi configurations that share their binning and contraction settings
+  (``estimators.signal_group_key``) get their signal matrices from one
+  multi-configuration pass per catalog (``estimators.bk_box_signals``:
+  shared shell legs, [0,0,0] out of [l,0,l]'s contraction), equal to the
+  one-at-a-time results up to FP32 summation order.
 
 Catalogs whose outputs all exist already are skipped, so an interrupted
 campaign resumes where it stopped.  The next chunk's catalogs are read and
@@ -27,6 +32,7 @@ import torch
 
 from .catalog import read_box_catalog
 from .device import PREC, get_context
+from .estimators import bk_box_signals, signal_group_key
 from .mesh import DeviceField
 from .param_helper import catalog_check, validate_config
 from .task_executor import check_task, result_path, run_task
@@ -158,15 +164,27 @@ def _measure_chunk(ctx, configs, chunk, pos_l, w_l, off, pos_d, w_d, n, L, V, pr
         del m1
     main.synchronize()
     t1 = time.perf_counter()
+    groups = {}
+    for i, c in enumerate(configs):
+        key = signal_group_key(c)
+        if key is not None:
+            groups.setdefault(key, []).append(i)
+    shared = [g for g in groups.values() if len(g) > 1]
     for b, path in enumerate(chunk):
         field = DeviceField(s1[b], s2[b] if interlaced else None, n, L, prec, sampler, interlaced)
         attrs = {"NZ_a": np.sum(w_l[b]) / V, "N_gal_a": len(w_l[b]), "interlaced": interlaced, "sampler": sampler}
+        signals = {}
+        for g in shared:
+            for i, sig in zip(g, bk_box_signals(field, [dict(configs[i], **attrs) for i in g])):
+                signals[i] = sig
         res = []
-        for c in configs:
+        for i, c in enumerate(configs):
             cfg = dict(c)
             cfg["catalogs"] = {"data_a": path}
             cfg["fields"] = (attrs, field)
             cfg["gc_collect"] = False
+            if i in signals:
+                cfg["signal"] = signals[i]
             res.append(run_task(**cfg))
         results[path] = res
     main.synchronize()

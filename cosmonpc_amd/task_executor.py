@@ -76,13 +76,16 @@ def run_task(statistic, correlation_mode, geometry, catalogs, **kwargs):
     """``task_executor.py:11-328``.  ``kwargs`` may carry ``arrays`` (catalog
     arrays instead of files), the GPU options ``precision`` /
     ``contraction``, ``fields`` = (mesh attrs, field_a) of an already
-    painted box catalog (auto-correlation; used by the batched mock driver)
-    and ``gc_collect`` (default True: the reference's gc pass after the pk
+    painted box catalog (auto-correlation; used by the batched mock driver),
+    ``signal`` = the bk box ``total_res`` already computed by a
+    multi-configuration pass (``estimators.bk_box_signals``) and
+    ``gc_collect`` (default True: the reference's gc pass after the pk
     estimator, ``task_executor.py:115-116``)."""
     logging.basicConfig(level=logging.INFO, format="%(asctime)s - %(module)s - %(levelname)s - %(message)s")
     logging.info(f"Running {correlation_mode} {statistic} task with catalogs:{catalogs},\n")
     arrays = kwargs.pop("arrays", None)
     painted = kwargs.pop("fields", None)
+    signal = kwargs.pop("signal", None)  # bk box total_res from a multi-configuration pass
     collect = kwargs.pop("gc_collect", True)  # the batched mock driver skips the per-task gc pass
     if painted is not None and (geometry != "box-like" or correlation_mode != "auto"):
         raise ValueError("pre-painted fields are supported for box-like auto-correlation only")
@@ -163,7 +166,7 @@ def run_task(statistic, correlation_mode, geometry, catalogs, **kwargs):
         t1 = time.time()
         logging.info(f"Time to create overdensity field(s): {t1 - t0:.2f} seconds")
         if geometry == "box-like":
-            bk_res = calculate_bk_sugi_box(fa, fb, fc, correlation_mode, stat_attrs, **kwargs)
+            bk_res = calculate_bk_sugi_box(fa, fb, fc, correlation_mode, stat_attrs, signal=signal, **kwargs)
         else:
             bk_res = calculate_bk_sugi_survey(fa, fb, fc, correlation_mode, stat_attrs, **kwargs)
         del fa, fb, fc

@@ -94,3 +94,31 @@ def test_run_mocks_empty_catalog(gpu_ctx, tmp_path):
     alone = run_mocks([pk2], [paths[3]], arrays=arrays)
     assert rel_to_max(got[paths[3]][0]["P0"], alone[paths[3]][0]["P0"]) < 1e-10
     assert got[paths[2]][0]["N_gal_a"] == 0
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("fp32", 1e-5)])
+def test_run_mocks_shared_bk_signal(gpu_ctx, tmp_path, precision, tol):
+    """bk_sugi [0,0,0] and [2,0,2] of one catalog in one run_mocks call share
+    one multi-configuration signal pass (estimators.bk_box_signals); each
+    result equals run_stats of that configuration alone: FP64 to 1e-10 (the
+    same contractions), split-bf16 to 1e-5 of max|B| (FP32 summation order;
+    its error against FP64 is ~1e-6)."""
+    from cosmonpc_amd import run_stats
+    from cosmonpc_amd.estimators import signal_group_key
+    from cosmonpc_amd.mocks import run_mocks
+
+    arrays = _arrays(tmp_path)
+    path = [p for p in arrays if len(arrays[p][1]) > 0][0]
+    _, bk = _cfgs(tmp_path / "batched")
+    cfgs = [dict(bk, angu_config=a, precision=precision) for a in ([0, 0, 0], [2, 0, 2])]
+    assert signal_group_key(cfgs[0]) == signal_group_key(cfgs[1]) is not None
+    print(f"shared bk signal {precision}", flush=True)
+    got = run_mocks(cfgs, [path], arrays=arrays)[path]
+    for cfg, res in zip(cfgs, got):
+        ref = run_stats(dict(cfg, output_dir=str(tmp_path / "single"), catalogs={"data_a": path},
+                             arrays={"data_a": arrays[path]}))
+        for key in ("Bk_raw", "B_sugi"):
+            err = rel_to_max(res[key], ref[key])
+            print(f"{cfg['angu_config']} {key} rel {err:.2e}", flush=True)
+            assert err < tol, (cfg["angu_config"], key, err)
+        np.testing.assert_array_equal(res["nmodes"], ref["nmodes"])
