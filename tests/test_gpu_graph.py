@@ -135,3 +135,65 @@ def test_graph_refuses_host_readback(gpu_ctx):
         sg.record(lambda: ctx.absmax(x), warmup=False)
     sg.close()
     assert ctx.absmax(x) == 2.5
+
+
+def test_multi_single_configuration_is_the_plain_call(gpu_ctx):
+    """One configuration through the multi-configuration call is the plain
+    call, bit for bit (nothing to share, the same launches)."""
+    from cosmonpc_amd.estimators import bk_box_g00, bk_box_signal_auto, bk_box_signal_multi
+
+    ctx = gpu_ctx
+    src = _spectrum(ctx, 11)
+    G = bk_box_g00(ctx, src, N, 1)
+    for a in ANGUS:
+        got = bk_box_signal_multi(ctx, src, G, N, L, 1, EDGES, [a], "bf16x3")[0]
+        want = bk_box_signal_auto(ctx, src, G, N, L, 1, EDGES, a, "bf16x3")
+        np.testing.assert_array_equal(got, want)
+
+
+@pytest.mark.parametrize("na,nb", [(32, 1), (1, 32), (17, 9)])
+def test_two_output_contraction_row_extremes(gpu_ctx, na, nb):
+    """C and C_bb of the two-output launch against the FP64 GEMM of the FP64
+    real shells, at the row limits (32 rows, a single row, odd counts)."""
+    from cosmonpc_amd import angular as A
+
+    ctx = gpu_ctx
+    n, Lb, kmax = 48, 600.0, 0.12
+    edges = np.linspace(0.01, kmax, 33)
+    R = ctx.synth_radius(Lb, kmax)
+    import torch
+
+    D32 = _spectrum(ctx, 13)
+    # the FP64 shells come from the same (FP32-valued) spectrum
+    D64 = torch.view_as_complex(torch.view_as_real(D32).double().contiguous())
+    rng = np.random.default_rng(5)
+    G64 = torch.from_numpy(1.0 + 0.5 * rng.standard_normal(n ** 3)).cuda()
+    G32 = G64.float().contiguous()
+    p0, p2 = A.shell_components(0, 0)[0][0], A.shell_components(2, 0)[0][0]
+
+    def real64(poly, rows):
+        out = ctx.mesh(n, 0, rows)
+        for b0 in range(0, rows, 8):
+            c = min(8, rows - b0)
+            ctx.shell_synth(D64, n, Lb, 0, edges, b0, c, poly, ctx.spectrum(n, 0, c), out[b0:b0 + c])
+        return out.view(rows, n ** 3)
+
+    def half32(poly, rows):
+        H = ctx.half_shells(n, R, rows)
+        work = ctx.spectrum(n, 1, 4)
+        kz = np.zeros(rows, dtype=np.int32)
+        for b0 in range(0, rows, 4):
+            c = min(4, rows - b0)
+            kz[b0:b0 + c] = ctx.shell_synth_half(D32, n, Lb, edges, b0, c, poly, work, H, b0)
+        return H, kz
+
+    HA, kza = half32(p2, na)
+    HB, kzb = half32(p0, nb)
+    C, Cbb = ctx.contract_zgemm(n, Lb, kmax, G32, HA, na, kza, HB, nb, kzb, with_bb=True)
+    A64, B64 = real64(p2, na), real64(p0, nb)
+    for got, ref, what in ((C, ctx.contract_gemm("gemm_fp64", G64, A64, B64), "C"),
+                           (Cbb, ctx.contract_gemm("gemm_fp64", G64, B64), "C_bb")):
+        got, ref = got.cpu().numpy(), ref.cpu().numpy()
+        err = np.max(np.abs(got - ref)) / np.max(np.abs(ref))
+        _log(f"na={na} nb={nb} {what}: rel err {err:.2e}")
+        assert got.shape == ref.shape and err < 1e-5, (what, err)
