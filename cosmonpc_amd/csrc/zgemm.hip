@@ -28,7 +28,9 @@
 // the same chunks, so a configuration whose legs are both B (e.g. [0,0,0]
 // beside [2,0,2]) costs no FFT of its own.
 // Slot s holds kz rows below q[s] only (its own shell radius); the rest of
-// its line is zero.
+// its line is zero.  B != A reads its operands without masks: lines L .. 31
+// are zero rows, and the G rows are zero beyond z = n, so the padded part of
+// the last chunk (whatever the lines hold there, all finite) adds nothing.
 // The per-wave accumulators are summed in a fixed order into one FP64
 // 32 x 32 partial per workgroup, and the partials in a fixed order by the
 // finalize kernel (deterministic).
@@ -49,6 +51,12 @@ namespace cnpc {
 constexpr int ZG_THREADS = ZG_THREADS_DEF;
 constexpr int ZG_MAXB = ZG_MAXB_DEF;  // butterflies per thread and FFT pass
 
+// G row stride in LDS: the full width of a row's 32-wide chunks
+__host__ __device__ inline int zg_gstride(int n) { return ((n + 31) >> 5) << 5; }
+// floats of zeros after the lines: the unmasked reads of the last line's
+// last chunk reach at most 32 - 8 floats past its second plane
+constexpr int ZG_TAIL_PAD = 32;
+
 struct ZGemmArgs {
   int n, Q;                 // grid, non-zero kz per line (R + 1)
   int na, nb;               // rows of A and B (<= 32)
@@ -68,6 +76,13 @@ struct ZGemmArgs {
 
 // 8 consecutive z values of one part of one planar line (two 16-byte LDS
 // loads), 0 where !ok
+__device__ __forceinline__ void zg_row8u(const float2* buf, int ls, int line, int part, int z0, float (&v)[8]) {
+  const float4* p = (const float4*)((const float*)buf + 2 * line * ls + part * ls + z0);
+  const float4 a = p[0], b = p[1];
+  v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+  v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+}
+
 __device__ __forceinline__ void zg_row8(const float2* buf, int ls, int line, int part, bool ok, int z0,
                                         float (&v)[8]) {
   const float4* p = (const float4*)((const float*)buf + 2 * line * ls + part * ls + z0);
@@ -85,11 +100,24 @@ template <int THREADS, bool BR, bool BB>
 __global__ void __launch_bounds__(THREADS) zgemm_bf16x3_kernel(ZGemmArgs g) {
   // lines of n + 4 complex: 16-byte aligned planes for the planar last pass
   const int n = g.n, Q = g.Q, R = Q - 1, ls = n + 4, half = n >> 1, L = g.nlines;
+  const int gls = zg_gstride(n);        // G row stride: the chunks' full width
   extern __shared__ unsigned char smem[];
   float2* W = (float2*)smem;
-  float* Gs = (float*)(W + n);          // G rows y, y + 1 of the tile
-  float2* buf = (float2*)(Gs + 2 * n);
+  float* Gs = (float*)(W + n);          // G rows y, y + 1 of the tile (zero beyond n)
+  float2* buf = (float2*)(Gs + 2 * gls);
   build_pass_twiddles(W, g.tw, n, g.fft);
+  if (!BR) {
+    // zero once: the G row tails, lines L .. 31, the float range [2n, 2 ls)
+    // of every line (never written by the FFT) and the tail pad after buf
+    for (int t = threadIdx.x; t < 2 * (gls - n); t += blockDim.x)
+      Gs[(t / (gls - n)) * gls + n + t % (gls - n)] = 0.f;
+    float* fb = (float*)buf;
+    for (int t = threadIdx.x; t < (32 - L) * 2 * ls + ZG_TAIL_PAD; t += blockDim.x) fb[2 * L * ls + t] = 0.f;
+    for (int t = threadIdx.x; t < L * 2 * (ls - n); t += blockDim.x) {
+      const int l = t / (2 * (ls - n)), j = t - l * 2 * (ls - n);
+      fb[2 * l * ls + 2 * n + j] = 0.f;
+    }
+  }
   __shared__ int qa[32], qb[32];  // kz rows per slot (read by lane-varying index)
   if (threadIdx.x < 32) {
     qa[threadIdx.x] = g.qa[threadIdx.x];
@@ -112,7 +140,7 @@ __global__ void __launch_bounds__(THREADS) zgemm_bf16x3_kernel(ZGemmArgs g) {
 #pragma unroll
       for (int q = 0; q < 4; ++q) acc[i][j][q] = 0.f;
   f16x8 fdh, fdl;  // unused fp16 outputs of split8
-  const int rounds = (g.same && BR) ? 1 : 2;
+  const int rounds = BR ? 1 : 2;  // BR: B == A, both rows in one round
   for (int64_t tile = blockIdx.x; tile < g.ntiles; tile += gridDim.x) {
     const int x = (int)(tile / half), yp = (int)(tile - (int64_t)x * half);
     const float4* TA = g.HA + tile * g.sa_slots * Q;
@@ -125,9 +153,9 @@ __global__ void __launch_bounds__(THREADS) zgemm_bf16x3_kernel(ZGemmArgs g) {
       }
       if (r0 == 0) {
         const float* Grow = g.G + ((int64_t)x * n + 2 * yp) * n;
-        for (int t = threadIdx.x; t < 2 * n; t += blockDim.x) Gs[t] = Grow[t];
+        for (int t = threadIdx.x; t < 2 * n; t += blockDim.x) Gs[(t >= n ? gls - n : 0) + t] = Grow[t];
       }
-      if (g.same) {
+      if (BR) {
         // (l, kz): shells 2l, 2l+1; both rows -> lines l and Lh + l (BR),
         // else row r0 -> line l
         for (int t = threadIdx.x; t < Lh * Q; t += blockDim.x) {
@@ -169,43 +197,56 @@ __global__ void __launch_bounds__(THREADS) zgemm_bf16x3_kernel(ZGemmArgs g) {
       }
       __syncthreads();
       stockham_lds_planar<ZG_MAXB>(buf, W, n, ls, L, g.fft);
-      const int nchunk = (g.same && BR) ? 2 * nchr : nchr;
+      const int nchunk = BR ? 2 * nchr : nchr;
       for (int c = wq; c < nchunk; c += nwq) {
-        const int r = (g.same && BR) ? (c >= nchr ? 1 : 0) : r0;
-        const int z0 = 32 * (c - ((g.same && BR && r) ? nchr : 0)) + 8 * kg;
-        const bool zin = z0 < n;      // n % 8 == 0: a group of 8 is all in or all out
-        const int zr = zin ? z0 : 0;  // LDS reads stay inside the line
-        float vg[8];
-        {
-          const float4* gp = (const float4*)(Gs + r * n + zr);
-          const float4 g0 = gp[0], g1 = gp[1];
-          vg[0] = zin ? g0.x : 0.f; vg[1] = zin ? g0.y : 0.f; vg[2] = zin ? g0.z : 0.f; vg[3] = zin ? g0.w : 0.f;
-          vg[4] = zin ? g1.x : 0.f; vg[5] = zin ? g1.y : 0.f; vg[6] = zin ? g1.z : 0.f; vg[7] = zin ? g1.w : 0.f;
-        }
+        const int r = BR ? (c >= nchr ? 1 : 0) : r0;
+        const int z0 = 32 * (c - ((BR && r) ? nchr : 0)) + 8 * kg;
         bf16x8 ah[2], al[2], bh[2], bl[2];
+        if (BR) {  // B == A: rows beyond na and z beyond n masked
+          const bool zin = z0 < n;      // n % 8 == 0: a group of 8 is all in or all out
+          const int zr = zin ? z0 : 0;  // LDS reads stay inside the line
+          float vg[8];
+          {
+            const float4* gp = (const float4*)(Gs + r * gls + zr);
+            const float4 g0 = gp[0], g1 = gp[1];
+            vg[0] = zin ? g0.x : 0.f; vg[1] = zin ? g0.y : 0.f; vg[2] = zin ? g0.z : 0.f; vg[3] = zin ? g0.w : 0.f;
+            vg[4] = zin ? g1.x : 0.f; vg[5] = zin ? g1.y : 0.f; vg[6] = zin ? g1.z : 0.f; vg[7] = zin ? g1.w : 0.f;
+          }
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
-          const int i = 16 * h + rr;
-          float a[8], b[8];
-          if (g.same) {
-            const int line = (BR ? r * Lh : 0) + (i >> 1 < Lh ? i >> 1 : Lh - 1);
+          for (int h = 0; h < 2; ++h) {
+            const int i = 16 * h + rr;
+            float a[8], b[8];
+            const int line = r * Lh + (i >> 1 < Lh ? i >> 1 : Lh - 1);
             zg_row8(buf, ls, line, i & 1, zin && i < g.na, zr, a);
 #pragma unroll
             for (int j = 0; j < 8; ++j) b[j] = a[j] * vg[j];
-          } else if (wbb) {
-            const int line = i < L ? i : L - 1;
-            zg_row8(buf, ls, line, 1, zin && i < g.nb, zr, a);
-#pragma unroll
-            for (int j = 0; j < 8; ++j) b[j] = a[j] * vg[j];
-          } else {
-            const int line = i < L ? i : L - 1;
-            zg_row8(buf, ls, line, 0, zin && i < g.na, zr, a);
-            zg_row8(buf, ls, line, 1, zin && i < g.nb, zr, b);
-#pragma unroll
-            for (int j = 0; j < 8; ++j) b[j] *= vg[j];
+            split8<false>(a, ah[h], al[h], fdh, fdl);
+            split8<false>(b, bh[h], bl[h], fdh, fdl);
           }
-          split8<false>(a, ah[h], al[h], fdh, fdl);
-          split8<false>(b, bh[h], bl[h], fdh, fdl);
+        } else {  // B != A: line i is row i, no masks (zero rows, zero G tail)
+          float vg[8];
+          {
+            const float4* gp = (const float4*)(Gs + r * gls + z0);
+            const float4 g0 = gp[0], g1 = gp[1];
+            vg[0] = g0.x; vg[1] = g0.y; vg[2] = g0.z; vg[3] = g0.w;
+            vg[4] = g1.x; vg[5] = g1.y; vg[6] = g1.z; vg[7] = g1.w;
+          }
+#pragma unroll
+          for (int h = 0; h < 2; ++h) {
+            const int i = 16 * h + rr;
+            float a[8], b[8];
+            zg_row8u(buf, ls, i, wbb ? 1 : 0, z0, a);
+            if (wbb) {
+#pragma unroll
+              for (int j = 0; j < 8; ++j) b[j] = a[j] * vg[j];
+            } else {
+              zg_row8u(buf, ls, i, 1, z0, b);
+#pragma unroll
+              for (int j = 0; j < 8; ++j) b[j] *= vg[j];
+            }
+            split8<false>(a, ah[h], al[h], fdh, fdl);
+            split8<false>(b, bh[h], bl[h], fdh, fdl);
+          }
         }
 #pragma unroll
         for (int i = 0; i < 2; ++i)
@@ -230,22 +271,18 @@ __global__ void __launch_bounds__(THREADS) zgemm_bf16x3_kernel(ZGemmArgs g) {
   }
 }
 
-// Launch shapes.  B == A: 2 * ceil(na/2) lines per round (both rows) with
-// 768 threads, one workgroup per CU, unless ZG_SAME_PER_ROW picks per-row
-// rounds of ceil(na/2) lines with 384 threads, two workgroups per CU.
-// B != A: max(na, nb) lines per round, 768 threads.
-#ifndef ZG_SAME_PER_ROW
-#define ZG_SAME_PER_ROW 0
-#endif
-static int zg_lines(int na, int nb, int same) {
-  if (same) return ZG_SAME_PER_ROW ? (na + 1) / 2 : 2 * ((na + 1) / 2);
-  return na > nb ? na : nb;
-}
-static int zg_threads(int same) { return (same && ZG_SAME_PER_ROW) ? 384 : ZG_THREADS; }
+// Launch shapes.  B == A (kernel BR = true): 2 * ceil(na/2) lines per round
+// (both rows).  B != A: max(na, nb) lines per round, 32 lines of LDS (the
+// rows beyond stay zero).  768 threads, one workgroup per CU.
+static int zg_lines(int na, int nb, int same) { return same ? 2 * ((na + 1) / 2) : (na > nb ? na : nb); }
+static int zg_lines_alloc(int L, int same) { return same ? L : 32; }
+static int zg_threads(int) { return ZG_THREADS; }
 
-// twiddles, the tile's two G rows and the lines
+// twiddles, the tile's two G rows (stride zg_gstride) and the lines (plus
+// the tail pad that the unmasked reads of the last chunk may touch)
 size_t zgemm_lds_bytes(int n, int nlines) {
-  return sizeof(float2) * (size_t)n + sizeof(float) * 2 * (size_t)n + sizeof(float2) * (size_t)nlines * (n + 4);
+  return sizeof(float2) * (size_t)n + sizeof(float) * 2 * (size_t)zg_gstride(n) +
+         sizeof(float2) * (size_t)nlines * (n + 4) + sizeof(float) * ZG_TAIL_PAD;
 }
 
 // host check: every pass of the line FFT fits the staged registers and the
@@ -256,11 +293,11 @@ int zgemm_supported(const SynthArgs& a, int na, int nb, int same, size_t lds_lim
   const int L = zg_lines(na, nb, same);
   for (int s = 0; s < a.nrad; ++s)
     if ((int64_t)L * (n / a.rad[s]) > (int64_t)(wide_radix(a.rad[s]) ? 1 : ZG_MAXB) * zg_threads(same)) return 0;
-  return zgemm_lds_bytes(n, L) + 32 * 32 * sizeof(double) <= lds_limit ? 1 : 0;
+  return zgemm_lds_bytes(n, zg_lines_alloc(L, same)) + 32 * 32 * sizeof(double) <= lds_limit ? 1 : 0;
 }
 
 // workgroups per CU the launch shape allows
-int zgemm_blocks_per_cu(int same) { return (same && ZG_SAME_PER_ROW) ? 2 : 1; }
+int zgemm_blocks_per_cu(int) { return 1; }
 
 template <int T, bool BR, bool BB = false>
 static hipError_t zg_launch(const ZGemmArgs& g, int nblk, size_t lds, hipStream_t st) {
@@ -303,11 +340,9 @@ hipError_t launch_zgemm(const SynthArgs& a, int Q, const void* HA, int na, int s
   g.tw = (const float2*)tw;
   g.partial = partial;
   g.partial_bb = partial + (int64_t)nblk * 1024;  // used by BB launches only
-  const size_t lds = zgemm_lds_bytes(a.n, g.nlines);
+  const size_t lds = zgemm_lds_bytes(a.n, zg_lines_alloc(g.nlines, same));
   hipError_t e;
-  if (same && ZG_SAME_PER_ROW)
-    e = zg_launch<384, false>(g, nblk, lds, st);
-  else if (same)
+  if (same)
     e = zg_launch<ZG_THREADS, true>(g, nblk, lds, st);
   else if (Cbb)
     e = zg_launch<ZG_THREADS, false, true>(g, nblk, lds, st);
