@@ -28,9 +28,10 @@
 // the same chunks, so a configuration whose legs are both B (e.g. [0,0,0]
 // beside [2,0,2]) costs no FFT of its own.
 // Slot s holds kz rows below q[s] only (its own shell radius); the rest of
-// its line is zero.  B != A reads its operands without masks: lines L .. 31
-// are zero rows, and the G rows are zero beyond z = n, so the padded part of
-// the last chunk (whatever the lines hold there, all finite) adds nothing.
+// its line is zero.  The MFMA phase reads its operands without masks: rows
+// without a shell read zero lines (lines L .. 31 for B != A, line L for
+// B == A), and the G rows are zero beyond z = n, so the padded part of the
+// last chunk (whatever the lines hold there, all finite) adds nothing.
 // The per-wave accumulators are summed in a fixed order into one FP64
 // 32 x 32 partial per workgroup, and the partials in a fixed order by the
 // finalize kernel (deterministic).
@@ -75,7 +76,7 @@ struct ZGemmArgs {
 };
 
 // 8 consecutive z values of one part of one planar line (two 16-byte LDS
-// loads), 0 where !ok
+// loads)
 __device__ __forceinline__ void zg_row8u(const float2* buf, int ls, int line, int part, int z0, float (&v)[8]) {
   const float4* p = (const float4*)((const float*)buf + 2 * line * ls + part * ls + z0);
   const float4 a = p[0], b = p[1];
@@ -83,19 +84,10 @@ __device__ __forceinline__ void zg_row8u(const float2* buf, int ls, int line, in
   v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
 }
 
-__device__ __forceinline__ void zg_row8(const float2* buf, int ls, int line, int part, bool ok, int z0,
-                                        float (&v)[8]) {
-  const float4* p = (const float4*)((const float*)buf + 2 * line * ls + part * ls + z0);
-  const float4 a = p[0], b = p[1];
-  v[0] = ok ? a.x : 0.f; v[1] = ok ? a.y : 0.f; v[2] = ok ? a.z : 0.f; v[3] = ok ? a.w : 0.f;
-  v[4] = ok ? b.x : 0.f; v[5] = ok ? b.y : 0.f; v[6] = ok ? b.z : 0.f; v[7] = ok ? b.w : 0.f;
-}
-
 // One round = the lines of one FFT and the MFMA chunks they feed.  B == A
-// with BR: one round per tile, both rows (line r * Lh + l holds row r of
-// shells 2l, 2l+1), 2 * nchr chunks; B == A without BR: one round per row
-// (line l holds shells 2l, 2l+1 of that row).  B != A: one round per row
-// (line l holds A_l and B_l), nchr chunks.
+// (BR): one round per tile, both rows (line r * Lh + l holds row r of
+// shells 2l, 2l+1), 2 * nchr chunks.  B != A: one round per row (line l
+// holds A_l and B_l), nchr chunks; BB also accumulates B^T G B.
 template <int THREADS, bool BR, bool BB>
 __global__ void __launch_bounds__(THREADS) zgemm_bf16x3_kernel(ZGemmArgs g) {
   // lines of n + 4 complex: 16-byte aligned planes for the planar last pass
@@ -106,13 +98,15 @@ __global__ void __launch_bounds__(THREADS) zgemm_bf16x3_kernel(ZGemmArgs g) {
   float* Gs = (float*)(W + n);          // G rows y, y + 1 of the tile (zero beyond n)
   float2* buf = (float2*)(Gs + 2 * gls);
   build_pass_twiddles(W, g.tw, n, g.fft);
-  if (!BR) {
-    // zero once: the G row tails, lines L .. 31, the float range [2n, 2 ls)
-    // of every line (never written by the FFT) and the tail pad after buf
+  {
+    // zero once: the G row tails, the lines beyond L (zero rows), the float
+    // range [2n, 2 ls) of every line (never written by the FFT) and the
+    // tail pad after the lines
+    const int lalloc = BR ? L + 1 : 32;
     for (int t = threadIdx.x; t < 2 * (gls - n); t += blockDim.x)
       Gs[(t / (gls - n)) * gls + n + t % (gls - n)] = 0.f;
     float* fb = (float*)buf;
-    for (int t = threadIdx.x; t < (32 - L) * 2 * ls + ZG_TAIL_PAD; t += blockDim.x) fb[2 * L * ls + t] = 0.f;
+    for (int t = threadIdx.x; t < (lalloc - L) * 2 * ls + ZG_TAIL_PAD; t += blockDim.x) fb[2 * L * ls + t] = 0.f;
     for (int t = threadIdx.x; t < L * 2 * (ls - n); t += blockDim.x) {
       const int l = t / (2 * (ls - n)), j = t - l * 2 * (ls - n);
       fb[2 * l * ls + 2 * n + j] = 0.f;
@@ -202,22 +196,20 @@ __global__ void __launch_bounds__(THREADS) zgemm_bf16x3_kernel(ZGemmArgs g) {
         const int r = BR ? (c >= nchr ? 1 : 0) : r0;
         const int z0 = 32 * (c - ((BR && r) ? nchr : 0)) + 8 * kg;
         bf16x8 ah[2], al[2], bh[2], bl[2];
-        if (BR) {  // B == A: rows beyond na and z beyond n masked
-          const bool zin = z0 < n;      // n % 8 == 0: a group of 8 is all in or all out
-          const int zr = zin ? z0 : 0;  // LDS reads stay inside the line
+        if (BR) {  // B == A: row i is part i & 1 of line r Lh + i / 2; rows past the lines read the zero line L
           float vg[8];
           {
-            const float4* gp = (const float4*)(Gs + r * gls + zr);
+            const float4* gp = (const float4*)(Gs + r * gls + z0);
             const float4 g0 = gp[0], g1 = gp[1];
-            vg[0] = zin ? g0.x : 0.f; vg[1] = zin ? g0.y : 0.f; vg[2] = zin ? g0.z : 0.f; vg[3] = zin ? g0.w : 0.f;
-            vg[4] = zin ? g1.x : 0.f; vg[5] = zin ? g1.y : 0.f; vg[6] = zin ? g1.z : 0.f; vg[7] = zin ? g1.w : 0.f;
+            vg[0] = g0.x; vg[1] = g0.y; vg[2] = g0.z; vg[3] = g0.w;
+            vg[4] = g1.x; vg[5] = g1.y; vg[6] = g1.z; vg[7] = g1.w;
           }
 #pragma unroll
           for (int h = 0; h < 2; ++h) {
             const int i = 16 * h + rr;
             float a[8], b[8];
-            const int line = r * Lh + (i >> 1 < Lh ? i >> 1 : Lh - 1);
-            zg_row8(buf, ls, line, i & 1, zin && i < g.na, zr, a);
+            const int line = (i >> 1) < Lh ? r * Lh + (i >> 1) : L;
+            zg_row8u(buf, ls, line, i & 1, z0, a);
 #pragma unroll
             for (int j = 0; j < 8; ++j) b[j] = a[j] * vg[j];
             split8<false>(a, ah[h], al[h], fdh, fdl);
@@ -272,10 +264,11 @@ __global__ void __launch_bounds__(THREADS) zgemm_bf16x3_kernel(ZGemmArgs g) {
 }
 
 // Launch shapes.  B == A (kernel BR = true): 2 * ceil(na/2) lines per round
-// (both rows).  B != A: max(na, nb) lines per round, 32 lines of LDS (the
-// rows beyond stay zero).  768 threads, one workgroup per CU.
+// (both rows) and one zero line.  B != A: max(na, nb) lines per round, 32
+// lines of LDS (the rows beyond stay zero).  768 threads, one workgroup per
+// CU.
 static int zg_lines(int na, int nb, int same) { return same ? 2 * ((na + 1) / 2) : (na > nb ? na : nb); }
-static int zg_lines_alloc(int L, int same) { return same ? L : 32; }
+static int zg_lines_alloc(int L, int same) { return same ? L + 1 : 32; }
 static int zg_threads(int) { return ZG_THREADS; }
 
 // twiddles, the tile's two G rows (stride zg_gstride) and the lines (plus
