@@ -250,17 +250,26 @@ def get_q_ells_eff(i, j, k_eff, ell_1, ell_2, L, k3_bins, k3_eff):
     """q_{l1 l2 L}(k1, k2, k3) at effective k (``math_evaluator.py:592-616``).
     Memoised: k_eff depends only on the grid and the bins, so every mock of
     a campaign asks for the same values.  The device binning sums |k| with
-    FP64 atomics, so k_eff of two calls can differ in the last bits; the key
-    holds the float32 images of k_eff and k3_eff (a q computed at the first
-    call's k_eff differs from the exact one by ~1e-16 relative)."""
-    key = (i, j, ell_1, ell_2, L, k3_bins, np.asarray(k_eff, dtype=np.float32).tobytes(),
-           np.asarray(k3_eff, dtype=np.float32).tobytes())
+    FP64 atomics, so k_eff of two calls can differ in the last bits.  The
+    key holds the float32 images of the values q depends on (k_eff[i],
+    k_eff[j], k3_eff); a hit is used only when its FP64 values agree with
+    the new ones to 1e-13 relative (atomic-order noise), otherwise q is
+    recomputed and replaces the entry."""
+    k12 = np.array([k_eff[i], k_eff[j]], dtype=np.float64)
+    k3 = np.asarray(k3_eff, dtype=np.float64)
+    key = (i, j, ell_1, ell_2, L, k3_bins, k12.astype(np.float32).tobytes(), k3.astype(np.float32).tobytes())
     hit = _Q_CACHE.get(key)
-    if hit is None:
-        if len(_Q_CACHE) > 65536:
-            _Q_CACHE.clear()
-        hit = _Q_CACHE[key] = _q_ells_eff(i, j, k_eff, ell_1, ell_2, L, k3_bins, k3_eff)
-    return hit.copy()
+    if hit is not None and _close(hit[0], k12) and _close(hit[1], k3):
+        return hit[2].copy()
+    if len(_Q_CACHE) > 65536:
+        _Q_CACHE.clear()
+    q = _q_ells_eff(i, j, k_eff, ell_1, ell_2, L, k3_bins, k3_eff)
+    _Q_CACHE[key] = (k12, k3.copy(), q)
+    return q.copy()
+
+
+def _close(a, b, rtol=1e-13):
+    return a.shape == b.shape and bool(np.all(np.abs(a - b) <= rtol * np.abs(b)))
 
 
 def get_q_ells(i, j, k_center, k_min, k_max, k_bins, ell_1, ell_2, L, k3_bins):

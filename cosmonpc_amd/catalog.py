@@ -198,7 +198,7 @@ def read_survey_catalog(ctx, catalog, column_names, z_range, comp_weight_plan, p
     dev = ctx.device
 
     def col(a):
-        return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to(dev)
+        return torch.from_numpy(np.require(a, dtype=np.float64, requirements=["C", "W"])).to(dev)
 
     pos = ctx.radecz_to_xyz(col(arr["RA"]), col(arr["DEC"]), col(arr["Z"]), float(para_cosmo["Omega0"]))
     cat = SurveyCatalog(pos, col(weight), col(arr["WEIGHT_FKP"]), col(nz))

@@ -117,6 +117,7 @@ __global__ void __launch_bounds__(THREADS) zgemm_bf16x3_kernel(ZGemmArgs g) {
     qa[threadIdx.x] = g.qa[threadIdx.x];
     qb[threadIdx.x] = g.qb[threadIdx.x];
   }
+  __syncthreads();  // every wave reads qa / qb in its first fill loop
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
   // BB: waves [0, nw/2) take A^T G B, waves [nw/2, nw) B^T G B (wave-uniform)
   const bool wbb = BB && w >= (nw >> 1);

@@ -41,11 +41,18 @@ class DeviceField:
         return out
 
 
+def _host_f64(a):
+    """A host array as a CPU FP64 tensor (a copy only when the array is not
+    writable C-ordered FP64: torch.from_numpy of read-only memory is
+    undefined)."""
+    return torch.from_numpy(np.require(a, dtype=np.float64, requirements=["C", "W"]))
+
+
 def paint_field(ctx, pos, w, n, boxsize, sampler, interlaced, prec, boxcenter=None):
     """Paint host or device particles and return a DeviceField."""
     dev = ctx.device
-    pos_d = torch.as_tensor(np.ascontiguousarray(pos, dtype=np.float64)).to(dev) if not torch.is_tensor(pos) else pos
-    w_d = torch.as_tensor(np.ascontiguousarray(w, dtype=np.float64)).to(dev) if not torch.is_tensor(w) else w
+    pos_d = pos if torch.is_tensor(pos) else _host_f64(pos).to(dev)
+    w_d = w if torch.is_tensor(w) else _host_f64(w).to(dev)
     m0 = ctx.mesh(n, prec)
     m1 = ctx.mesh(n, prec) if interlaced else None
     ctx.paint(pos_d, w_d, n, boxsize, sampler, interlaced, prec, m0, m1, boxcenter)

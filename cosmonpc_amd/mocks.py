@@ -59,8 +59,10 @@ def _stage(chunk, base, arrays, device, stream):
             pos, w = read_box_catalog(path, base["column_names"], apply_rsd=base.get("apply_rsd", False),
                                       para_cosmo=base.get("cosmology"), boxsize=base["boxsize"],
                                       redshift_box=base.get("redshift_box"), los=base.get("rsd"))
-        pos_l.append(np.ascontiguousarray(pos, dtype=np.float64))
-        w_l.append(np.ascontiguousarray(w, dtype=np.float64))
+        # writable C-ordered FP64 (a copy only for read-only or other
+        # inputs): torch.from_numpy of a read-only array is undefined
+        pos_l.append(np.require(pos, dtype=np.float64, requirements=["C", "W"]))
+        w_l.append(np.require(w, dtype=np.float64, requirements=["C", "W"]))
     off = np.concatenate([[0], np.cumsum([len(w) for w in w_l])]).astype(np.int64)
     with torch.cuda.device(device), torch.cuda.stream(stream):
         pos_d = torch.empty((int(off[-1]), 3), dtype=torch.float64, device=device)

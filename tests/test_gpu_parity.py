@@ -348,6 +348,10 @@ def test_zgemm_fused_matches_fp64_contraction(gpu_ctx, n, L, kmax):
 
     def half32(src, poly, rows, nslots):
         H = ctx.half_shells(n, R, nslots)
+        # rows and slots the synthesis leaves unwritten hold NaN: a read of
+        # any of them (e.g. before the kz row counts are visible to every
+        # wave) poisons the result
+        H.fill_(complex(float("nan"), float("nan")))
         work = ctx.spectrum(n, 1, 4)
         kz = np.zeros(rows, dtype=np.int32)
         for b0 in range(0, rows, 4):

@@ -272,3 +272,28 @@ def test_zbatch_pairs_same_and_pair_requests(monkeypatch):
     assert pair.value == "C" and same.value == "Cbb"
     assert other_g.value == "C" and lone.value == "C"
     assert len(calls) == 3 and b.requests == []
+
+
+def test_q_ells_memo_rejects_nearby_keys():
+    """The memo of q_l(k1, k2, k3) serves a hit only for k values equal to
+    atomic-order noise (1e-13 relative); k values that share float32 images
+    but differ beyond that get their own q."""
+    from cosmonpc_amd import angular as A
+    from oracle.cnpc_ref import angular as R
+
+    A._Q_CACHE.clear()
+    k_eff = np.array([0.05, 0.07, 0.09, 0.11])
+    e = np.linspace(2 * 0.04, 2 * 0.12, 2 * 4 + 1)[:-1]
+    k3 = 0.5 * (e[1:] + e[:-1])
+    q0 = A.get_q_ells_eff(1, 2, k_eff, 2, 0, 2, len(k3), k3)
+    np.testing.assert_allclose(q0, R.get_q_ells_eff(1, 2, k_eff, 2, 0, 2, len(k3), k3), rtol=1e-13, atol=1e-15)
+    # last-bit noise: served from the memo
+    noisy = k_eff * (1 + 2e-16)
+    assert np.array_equal(A.get_q_ells_eff(1, 2, noisy, 2, 0, 2, len(k3), k3), q0)
+    assert len(A._Q_CACHE) == 1
+    # the same float32 images, 1e-9 apart: recomputed at the new values
+    moved = k_eff * (1 + 1e-9)
+    assert np.array_equal(moved.astype(np.float32), k_eff.astype(np.float32))
+    q1 = A.get_q_ells_eff(1, 2, moved, 2, 0, 2, len(k3), k3)
+    np.testing.assert_allclose(q1, R.get_q_ells_eff(1, 2, moved, 2, 0, 2, len(k3), k3), rtol=1e-13, atol=1e-15)
+    assert not np.array_equal(q1, q0)
