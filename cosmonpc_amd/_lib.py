@@ -57,6 +57,7 @@ SIGNATURES = {
     "cnpc_shell_synth": [_P, _I, _D, _I, _P, _P, _I, _I, _I, _P, _P, _P],
     "cnpc_shell_synth_half": [_P, _I, _D, _P, _P, _I, _I, _I, _P, _P, _P, _I, _I, _P, _I],
     "cnpc_contract_zgemm": [_P, _I, _D, _D, _P, _P, _I, _I, _P, _P, _I, _I, _P, _I, _P, _P],
+    "cnpc_contract_zgemm_noalloc": [_P, _I, _D, _D, _P, _P, _I, _I, _P, _P, _I, _I, _P, _I, _P, _P],
     "cnpc_contract_zgemm_supported": [_P, _I, _I, _I, _I, ctypes.POINTER(_I)],
     "cnpc_contract_pairs": [_P, _I, _L, _P, _P, _P, _I, _P, _P, _P],
     "cnpc_contract_gemm": [_P, _I, _L, _P, _P, _I, _P, _I, _I, _F, _F, _I, _P],

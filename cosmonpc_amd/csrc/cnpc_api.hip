@@ -19,6 +19,15 @@ namespace cnpc {
 static thread_local std::string g_err;
 void set_error(const std::string& msg) { g_err = msg; }
 
+// Static scratch of the allocation-free entry point
+// (cnpc_contract_zgemm_noalloc): the FP64 partial tiles and the line
+// twiddles.  Each device holds its own copy until the process ends; host
+// code uses its device address from hipGetSymbolAddress (g_scratch_ptr),
+// never the array's name.
+static __device__ __align__(256) unsigned char scratch[CNPC_NOALLOC_SCRATCH_BYTES];
+static void* g_scratch_ptr[CNPC_MAX_DEVICES] = {};
+static std::mutex g_scratch_mu;
+
 __global__ void zero_kernel(uint4* p, int64_t n16, unsigned char* tail, int ntail) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (int64_t)gridDim.x * blockDim.x)
     p[i] = make_uint4(0, 0, 0, 0);
@@ -203,6 +212,16 @@ int cnpc_ctx_create(cnpc_ctx** out, int device) {
   CHECK_HIP(hipGetDeviceCount(&ndev));
   REQUIRE(device >= 0 && device < ndev, "cnpc_ctx_create: invalid device " + std::to_string(device));
   CHECK_HIP(hipSetDevice(device));
+  REQUIRE(device < CNPC_MAX_DEVICES, "cnpc_ctx_create: device index beyond the library's limit");
+  {  // once per device, with the device current: the static scratch's address
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+      void* p = nullptr;
+      CHECK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+      REQUIRE(p != nullptr, "cnpc_ctx_create: no device address for the static scratch");
+      g_scratch_ptr[device] = p;
+    }
+  }
   cnpc_ctx* c = new cnpc_ctx();
   c->device = device;
   *out = c;
@@ -505,9 +524,12 @@ int cnpc_shell_synth_half(cnpc_ctx* ctx, int n, double boxsize, const void* D, c
   return 0;
 }
 
-int cnpc_contract_zgemm(cnpc_ctx* ctx, int n, double boxsize, double kmax, const float* G, const void* HA, int na,
-                        int sa_slots, const int* kz_a, const void* HB, int nb, int sb_slots, const int* kz_b,
-                        int nblk, double* C, double* C_bb) {
+// fixed == nullptr: partials and twiddles in the context's scratch area
+// (grown on demand); else in the CNPC_NOALLOC_SCRATCH_BYTES at fixed (no
+// allocation: nblk is capped to what fits, sizes beyond it are refused).
+static int contract_zgemm_impl(cnpc_ctx* ctx, int n, double boxsize, double kmax, const float* G, const void* HA,
+                               int na, int sa_slots, const int* kz_a, const void* HB, int nb, int sb_slots,
+                               const int* kz_b, int nblk, double* C, double* C_bb, char* fixed) {
   REQUIRE(ctx, "null context");
   REQUIRE(n >= 8 && n % 8 == 0 && boxsize > 0 && kmax > 0, "contract_zgemm: nmesh must be a multiple of 8");
   REQUIRE(G && HA && C, "contract_zgemm: null buffers");
@@ -536,14 +558,30 @@ int cnpc_contract_zgemm(cnpc_ctx* ctx, int n, double boxsize, double kmax, const
   CHECK_HIP(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
   REQUIRE(zgemm_supported(a, na, nb, same, (size_t)lds_max),
           "contract_zgemm: grid too large for the LDS line buffers");
-  if (nblk <= 0) {  // as many workgroups as stay resident
+  const size_t per_blk = (size_t)1024 * 8 * (C_bb ? 2 : 1);  // FP64 partials (two sets with C_bb)
+  const size_t twb = ((size_t)n * 8 + 255) / 256 * 256;       // line twiddles
+  const bool auto_blk = nblk <= 0;
+  if (auto_blk) {  // as many workgroups as stay resident
     CHECK_HIP(hipDeviceGetAttribute(&nblk, hipDeviceAttributeMultiprocessorCount, dev));
     nblk *= zgemm_blocks_per_cu(same);
   }
   if (nblk > 4096) nblk = 4096;
-  const size_t pbytes = (size_t)nblk * 1024 * 8 * (C_bb ? 2 : 1);  // FP64 partials (two sets with C_bb)
-  if (int rc = ensure_scratch(ctx, pbytes + (size_t)n * 8)) return rc;
-  char* base = (char*)ctx->scratch;
+  char* base = nullptr;
+  if (fixed) {
+    REQUIRE(twb + per_blk <= CNPC_NOALLOC_SCRATCH_BYTES, "contract_zgemm_noalloc: nmesh " + std::to_string(n) +
+                                                             " needs more than the static scratch");
+    const int fit = (int)((CNPC_NOALLOC_SCRATCH_BYTES - twb) / per_blk);
+    if (auto_blk && nblk > fit) nblk = fit;
+    REQUIRE(nblk <= fit, "contract_zgemm_noalloc: " + std::to_string(nblk) + " workgroups need " +
+                             std::to_string(twb + nblk * per_blk) + " bytes of scratch, more than the static " +
+                             std::to_string(CNPC_NOALLOC_SCRATCH_BYTES) + " (at most " + std::to_string(fit) +
+                             " workgroups" + (C_bb ? " with C_bb)" : ")"));
+    base = fixed;
+  } else {
+    if (int rc = ensure_scratch(ctx, nblk * per_blk + twb)) return rc;
+    base = (char*)ctx->scratch;
+  }
+  const size_t pbytes = nblk * per_blk;
   CHECK_HIP(launch_twiddles(n, base + pbytes, ctx->stream));
   int qa[32], qb[32];
   for (int s = 0; s < 32; ++s) {
@@ -555,6 +593,23 @@ int cnpc_contract_zgemm(cnpc_ctx* ctx, int n, double boxsize, double kmax, const
   CHECK_HIP(launch_zgemm(a, R + 1, HA, na, sa_slots, qa, same ? HA : HB, nb, sb_slots, qb, same, G, base + pbytes,
                          nblk, (double*)base, C, C_bb, ctx->stream));
   return 0;
+}
+
+int cnpc_contract_zgemm(cnpc_ctx* ctx, int n, double boxsize, double kmax, const float* G, const void* HA, int na,
+                        int sa_slots, const int* kz_a, const void* HB, int nb, int sb_slots, const int* kz_b,
+                        int nblk, double* C, double* C_bb) {
+  return contract_zgemm_impl(ctx, n, boxsize, kmax, G, HA, na, sa_slots, kz_a, HB, nb, sb_slots, kz_b, nblk, C, C_bb,
+                             nullptr);
+}
+
+int cnpc_contract_zgemm_noalloc(cnpc_ctx* ctx, int n, double boxsize, double kmax, const float* G, const void* HA,
+                                int na, int sa_slots, const int* kz_a, const void* HB, int nb, int sb_slots,
+                                const int* kz_b, int nblk, double* C, double* C_bb) {
+  REQUIRE(ctx, "null context");
+  REQUIRE(ctx->device >= 0 && ctx->device < CNPC_MAX_DEVICES && g_scratch_ptr[ctx->device],
+          "contract_zgemm_noalloc: the static scratch of this device was not set up (cnpc_ctx_create)");
+  return contract_zgemm_impl(ctx, n, boxsize, kmax, G, HA, na, sa_slots, kz_a, HB, nb, sb_slots, kz_b, nblk, C, C_bb,
+                             (char*)g_scratch_ptr[ctx->device]);
 }
 
 int cnpc_contract_zgemm_supported(cnpc_ctx* ctx, int n, int na, int nb, int same, int* ok) {

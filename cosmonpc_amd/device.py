@@ -215,13 +215,16 @@ class GpuContext:
         return bool(ok.value)
 
     def contract_zgemm(self, n, boxsize, kmax, G, HA, na, kz_a, HB=None, nb=None, kz_b=None, nblk=0,
-                       with_bb=False):
+                       with_bb=False, noalloc=False, out=None):
         """C[a, b] = sum_x A_a G B_b over the shells in slots 0 .. na-1 of HA
         and 0 .. nb-1 of HB (None: B = A), z pass fused (device float64
         tensor).  kz_a / kz_b: kz rows per slot as ``shell_synth_half``
         returned them (the rows above are not written by the synthesis).
         ``with_bb`` (B != A): returns (C, C_bb) with C_bb[b, c] = sum_x B_b G
-        B_c from the same z transforms."""
+        B_c from the same z transforms.
+        ``noalloc``: through cnpc_contract_zgemm_noalloc (partials in the
+        library's static scratch; nothing allocated by the call).  ``out``:
+        caller-allocated C (and C_bb) tensors to write into."""
         if kz_a is None or (HB is not None and kz_b is None):
             raise ValueError("contract_zgemm needs the kz rows of every slot")
         if with_bb and (HB is None or HB is HA):
@@ -246,14 +249,21 @@ class GpuContext:
             return kz
 
         ka, kb = rows(kz_a, na), rows(kz_b, nb_) if HB is not None else None
-        C = torch.empty((na, nb_), dtype=torch.float64, device=self.device)
-        Cbb = torch.empty((nb_, nb_), dtype=torch.float64, device=self.device) if with_bb else None
-        check(self.lib.cnpc_contract_zgemm(self.h, n, float(boxsize), float(kmax), _ptr(G), _ptr(HA), int(na),
-                                           HA.shape[2], None if ka is None else ka.ctypes.data_as(ctypes.c_void_p),
-                                           _ptr(HB), int(nb_), HA.shape[2] if HB is None else HB.shape[2],
-                                           None if kb is None else kb.ctypes.data_as(ctypes.c_void_p), int(nblk),
-                                           _ptr(C), _ptr(Cbb)),
-              "cnpc_contract_zgemm")
+        if out is None:
+            C = torch.empty((na, nb_), dtype=torch.float64, device=self.device)
+            Cbb = torch.empty((nb_, nb_), dtype=torch.float64, device=self.device) if with_bb else None
+        else:
+            C, Cbb = (out if with_bb else (out, None))
+            if (C.dtype != torch.float64 or tuple(C.shape) != (na, nb_) or not C.is_contiguous()
+                    or (with_bb and (Cbb.dtype != torch.float64 or tuple(Cbb.shape) != (nb_, nb_)
+                                     or not Cbb.is_contiguous()))):
+                raise ValueError("out must be contiguous float64 (na, nb) [and (nb, nb)] tensors")
+        fn = self.lib.cnpc_contract_zgemm_noalloc if noalloc else self.lib.cnpc_contract_zgemm
+        check(fn(self.h, n, float(boxsize), float(kmax), _ptr(G), _ptr(HA), int(na), HA.shape[2],
+                 None if ka is None else ka.ctypes.data_as(ctypes.c_void_p), _ptr(HB), int(nb_),
+                 HA.shape[2] if HB is None else HB.shape[2],
+                 None if kb is None else kb.ctypes.data_as(ctypes.c_void_p), int(nblk), _ptr(C), _ptr(Cbb)),
+              "cnpc_contract_zgemm_noalloc" if noalloc else "cnpc_contract_zgemm")
         return (C, Cbb) if with_bb else C
 
     def shell_filter(self, delta, n, boxsize, prec, edges, bin0, nshell, poly, out):

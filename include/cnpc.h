@@ -30,6 +30,7 @@
  */
 #ifndef CNPC_H
 #define CNPC_H
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -200,6 +201,25 @@ int cnpc_contract_gemm(cnpc_ctx* ctx, int mode, int64_t ncell, const void* G, co
 int cnpc_contract_zgemm(cnpc_ctx* ctx, int nmesh, double boxsize, double kmax, const float* G, const void* HA, int na,
                         int sa_slots, const int* kz_a, const void* HB, int nb, int sb_slots, const int* kz_b,
                         int nblk, double* C, double* C_bb);
+
+/* Size of the static per-device scratch of cnpc_contract_zgemm_noalloc:
+ * 8 KiB of FP64 partials per workgroup (16 KiB with C_bb) plus the line
+ * twiddles (nmesh * 8 bytes, rounded up to 256). */
+#define CNPC_NOALLOC_SCRATCH_BYTES ((size_t)(8u << 20) + (size_t)(64u << 10))
+
+/* cnpc_contract_zgemm without any allocation: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under
+ * graph capture, serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, kernels and results as cnpc_contract_zgemm; the partial tiles
+ * and twiddles live in a static device array of CNPC_NOALLOC_SCRATCH_BYTES
+ * per device (set up by cnpc_ctx_create) instead of the context's scratch
+ * area.  nblk <= 0 takes as many workgroups as fit that array; an explicit
+ * nblk that does not fit, or a grid whose twiddles do not, is refused (status
+ * 1, message in cnpc_last_error).  Every call on a device shares its array:
+ * calls that overlap on one device (two streams, two host threads) race. */
+int cnpc_contract_zgemm_noalloc(cnpc_ctx* ctx, int nmesh, double boxsize, double kmax, const float* G, const void* HA,
+                                int na, int sa_slots, const int* kz_a, const void* HB, int nb, int sb_slots,
+                                const int* kz_b, int nblk, double* C, double* C_bb);
 
 /* *ok = 1 when cnpc_contract_zgemm accepts this grid and these row counts
  * on the current device (LDS line buffers and FFT staging), else 0. */
