@@ -180,3 +180,86 @@ def test_pk_survey_fast_modes_distant_observer(poles, mode):
     if mode == "fast":
         want -= {f"P{p}" for p in fast}
     assert keys == want
+
+
+def _lattice_randoms(n, L, center, sub=4):
+    """A periodic lattice of spacing H/sub filling the box: every paint
+    window sees the same weights around every mesh point, so the painted
+    randoms are exactly uniform (no k != 0 modes)."""
+    H = L / n
+    g = (np.arange(sub * n) - sub * n // 2) * (H / sub)
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    out = np.zeros(X.size, dtype=[("Position", "f8", (3,)), ("WEIGHT", "f8"), ("WEIGHT_FKP", "f8"), ("NZ", "f8")])
+    out["Position"] = np.column_stack([X.ravel(), Y.ravel(), Z.ravel()]) + np.asarray(center)
+    out["WEIGHT"], out["WEIGHT_FKP"], out["NZ"] = 1.0, 1.0, 1e-3
+    return out
+
+
+def test_N_field_closed_forms():
+    """N = D_{(W W_fkp)^2}/dV + alpha^2 R_{(W W_fkp)^2}/dV
+    (``mesh_generator.py:1007-1128``): mass conservation with non-trivial
+    weights pins the squares and alpha^2; one data object on a mesh point
+    (CIC) over lattice randoms pins the cell and the 1/dV."""
+    n, L = 16, 200.0
+    vpc = L ** 3 / n ** 3
+    cats = _far_catalogs(1e6, 21, n_d=400, n_r=1500, L=L)
+    d, r = cats["data_a"], cats["randoms_a"]
+    bc = S.survey_boxcenter(r)
+    alpha = 0.37
+    for interlaced in (False, True):
+        N = S.get_N_field(cats, "a", alpha, n, L, "tsc", interlaced, bc)
+        want = np.sum((d["WEIGHT"] * d["WEIGHT_FKP"]) ** 2) + alpha ** 2 * np.sum((r["WEIGHT"] * r["WEIGHT_FKP"]) ** 2)
+        np.testing.assert_allclose(np.sum(np.real(N)) * vpc, want, rtol=1e-12)
+    rl = _lattice_randoms(n, L, (0.0, 0.0, 1e6))
+    bc = S.survey_boxcenter(rl)
+    one = np.zeros(1, dtype=rl.dtype)
+    one["Position"] = bc + np.array([3, -5, 2]) * (L / n)
+    one["WEIGHT"], one["WEIGHT_FKP"] = 1.5, 0.8
+    N = np.real(S.get_N_field({"data_a": one, "randoms_a": rl}, "a", alpha, n, L, "cic", False, bc))
+    uniform = alpha ** 2 * len(rl) / L ** 3  # lattice randoms of unit weight
+    exp = np.full((n, n, n), uniform)
+    exp[3, -5, 2] += (1.5 * 0.8) ** 2 / vpc
+    np.testing.assert_allclose(N, exp, rtol=1e-12, atol=1e-12 * np.max(exp))
+
+
+@pytest.mark.parametrize("angu", [[0, 0, 0], [2, 0, 2]])
+@pytest.mark.parametrize("sn", ["ana", "fft"])
+def test_bk_survey_shot_noise_distant_observer_equals_box(angu, sn):
+    """Distant observer, unit weights and lattice randoms (exactly uniform
+    paint): F = D/dV - alpha R/dV and the N-field D/dV + alpha^2 R/dV share
+    every k != 0 mode, y_LM(xhat) -> delta_M0 and S_LM -> delta_M0 S_0.  Then
+    the survey S1-like term (``clustering_estimator.py:2231-2320``), Q_L
+    (``:2322-2401``) with the analytic S3 (``:2403-2491``), and the FFT S3
+    (``:2510-2781``) reduce to the box SN1, SN2 and S3 of the same field
+    with SN0 = S_0 (``:1386-1468``, ``:1081-1384``; pinned in
+    test_oracle_box.py).  k_min = 0.01 k_f, so the box's k3 bins (from 2 k_min)
+    hold the same modes as the k bins the survey S1 uses."""
+    n, L, D = 16, 200.0, 1e8
+    kf = 2 * np.pi / L
+    rng = np.random.default_rng(4)
+    rl = _lattice_randoms(n, L, (0.0, 0.0, D))
+    dat = np.zeros(1500, dtype=rl.dtype)
+    dat["Position"] = rng.random((1500, 3)) * (0.9 * L) + np.array([0.0, 0.0, D]) - 0.45 * L
+    dat["WEIGHT"], dat["WEIGHT_FKP"], dat["NZ"] = 1.0, 1.0, 1e-3
+    cats = {"data_a": dat, "randoms_a": rl}
+    kmin, kmax, K = 0.01 * kf, 3.61 * kf, 3
+    attrs, fa, _, _ = S.get_mesh_bk_survey(cats, "auto", "aaa", angu, n, L, "tsc", False, "particle")
+    sa = dict(attrs, angu_config=angu, boxsize=[L] * 3, nmesh=[n] * 3, k_min=kmin, k_max=kmax, k_bins=K,
+              sampler="tsc", interlaced=False, tracer_type="aaa", data_vector_mode="full", shotnoise_mode=sn)
+    sur = S.calculate_bk_sugi_survey(fa, None, None, "auto", sa, cats)
+    S0 = complex(np.atleast_1d(attrs["S_LM_a"])[0])
+    assert abs(S0.imag) < 1e-6 * abs(S0.real)
+    NZ = 1.0
+    rbox = (np.real(fa) + NZ) * L ** 3 / n ** 3
+    bsa = dict(sa, NZ_a=NZ, N_gal_a=S0.real)
+    box = B.calculate_bk_sugi_box(rbox, None, None, "auto", bsa)
+    terms = ["SN0", "SN1", "SN2", "SN3"]
+    scale = max(np.max(np.abs(sur["SN_terms"][t] * sur["I_norm"])) for t in terms)
+    for t in terms:
+        got = np.asarray(sur["SN_terms"][t] * sur["I_norm"])
+        exp = np.asarray(box["SN_terms"][t] * box["I_norm"])
+        np.testing.assert_allclose(got, exp, rtol=0, atol=2e-6 * scale, err_msg=t)
+    # the terms are not all trivially zero
+    assert np.max(np.abs(sur["SN_terms"]["SN3"] * sur["I_norm"])) > 1e-3 * scale
+    if angu == [2, 0, 2]:
+        assert np.max(np.abs(sur["SN_terms"]["SN1"] * sur["I_norm"])) > 1e-3 * scale
