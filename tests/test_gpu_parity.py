@@ -2,8 +2,8 @@
 on identical seeded inputs.
 
 Tolerances: FP64 path 1e-10 relative per bin (relative to max|B| for
-bispectrum entries, which cross zero); FP32 + split-bf16 path 1e-4 relative to
-max|B| per multipole.
+bispectrum entries, which cross zero); FP32 + split-bf16 / fp16 path 5e-6
+relative to max|B| per multipole (split-fp8 3e-5).
 """
 import numpy as np
 import pytest
@@ -393,11 +393,11 @@ def test_zgemm_fused_matches_fp64_contraction(gpu_ctx, n, L, kmax):
 
 def test_bk_box_fused_and_separate_z_agree(gpu_ctx):
     """The estimator with the fused z pass equals the one with real shells in
-    HBM (both split-bf16) to 1e-6 of max|B| and the FP64 oracle to 1e-4."""
+    HBM (both split-bf16) to 1e-6 of max|B| and the FP64 oracle to 5e-6."""
     got_f, ref = _bk_case(gpu_ctx, (2, 0, 2), "full", "bf16x3", prec=1)
     got_s, _ = _bk_case(gpu_ctx, (2, 0, 2), "full", "bf16x3", prec=1, zsynth="separate")
     assert rel_to_max(got_f["Bk_raw"], got_s["Bk_raw"]) < 1e-6
-    assert rel_to_max(got_f["Bk_raw"], ref["Bk_raw"]) < 1e-4
+    assert rel_to_max(got_f["Bk_raw"], ref["Bk_raw"]) < 5e-6
 
 
 def test_bk_box_filter_method_matches_oracle(gpu_ctx):
@@ -414,13 +414,14 @@ def test_bk_box_blocked_matches_full(gpu_ctx):
 @pytest.mark.parametrize("contraction", ["bf16x3", "fp16x3", "fp8x6"])
 def test_bk_box_fp32_split_error(gpu_ctx, angu, contraction):
     """Split-precision contractions against the FP64 oracle, relative to
-    max|B|: 1e-4 for the bf16 / fp16 splits (three 8-11 bit pieces' worth);
-    1e-3 for the fp8 split (three 4-bit pieces)."""
+    max|B| (FP32 paint / FFT / shells included): 5e-6 for the bf16 / fp16
+    splits (measured 2.7e-7 - 8.6e-7), 3e-5 for the fp8 split (three 4-bit
+    pieces; measured 2e-6 - 6.6e-6)."""
     got, ref = _bk_case(gpu_ctx, angu, "full", contraction, prec=1, n=48, L=800.0, k=(0.01, 0.09, 8),
                         npart=60000)
     err = rel_to_max(got["Bk_raw"], ref["Bk_raw"])
     _log(f"{contraction} {angu} error rel max|B| = {err:.3e}")
-    assert err < (1e-3 if contraction == "fp8x6" else 1e-4)
+    assert err < (3e-5 if contraction == "fp8x6" else 5e-6)
 
 
 def test_gemm_modes_agree_with_pairwise(gpu_ctx):

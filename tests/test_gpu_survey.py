@@ -1,6 +1,6 @@
 """Survey-geometry parity: the HIP path against the oracle on identical seeded
 catalogs (FP64: 1e-10 relative per bin, or relative to max|value| for bins
-that cross zero; FP32 fields with split-bf16 contraction: 1e-4 of max|B|)."""
+that cross zero; FP32 fields with split-bf16 contraction: 5e-6 of max|B|)."""
 import numpy as np
 import pytest
 
@@ -163,7 +163,8 @@ def test_bk_survey_vs_oracle(gpu_ctx, corr, tracer, angu, dvm, ns, sn, sampler, 
 
 
 def test_bk_survey_fp32_split_bf16(gpu_ctx):
-    """FP32 fields + split-bf16 MFMA contraction against the FP64 oracle."""
+    """FP32 fields + split-bf16 MFMA contraction against the FP64 oracle:
+    5e-6 of max|B|."""
     from cosmonpc_amd import survey as GS
     from oracle.cnpc_ref import survey as S
 
@@ -179,7 +180,9 @@ def test_bk_survey_fp32_split_bf16(gpu_ctx):
     cats = oracle_survey_cats(rows)
     oattrs, oa, ob, oc = S.get_mesh_bk_survey(cats, "auto", "aaa", angu, N, L, "tsc", True, "particle")
     ores = S.calculate_bk_sugi_survey(oa, ob, oc, "auto", dict(oattrs, **kw), cats)
-    assert rel_to_max(res["Bk_raw"], ores["Bk_raw"]) < 1e-4
+    err = rel_to_max(res["Bk_raw"], ores["Bk_raw"])
+    print(f"survey bk fp32 split-bf16 rel max|B| {err:.2e}", flush=True)
+    assert err < 5e-6
 
 
 def test_run_stats_survey_boundary(gpu_ctx, tmp_path):
