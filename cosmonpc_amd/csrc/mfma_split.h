@@ -37,6 +37,29 @@ __device__ __forceinline__ void block_reduce_store(const float (&acc)[2][2][4], 
   __syncthreads();  // red may be reused by a second call
 }
 
+// The same for FP64 per-wave accumulators.
+__device__ __forceinline__ void block_reduce_store64(const double (&acc)[2][2][4], double* out, int w_lo = 0,
+                                                     int w_hi = 1 << 20) {
+  __shared__ double red64[32 * 32];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int i = threadIdx.x; i < 1024; i += blockDim.x) red64[i] = 0.0;
+  __syncthreads();
+  const int nw = (int)(blockDim.x >> 6) < w_hi ? (int)(blockDim.x >> 6) : w_hi;
+  for (int ww = w_lo; ww < nw; ++ww) {
+    if (w == ww) {
+      for (int bi = 0; bi < 2; ++bi)
+        for (int bj = 0; bj < 2; ++bj)
+          for (int j = 0; j < 4; ++j) {
+            const int r = 16 * bi + 4 * (lane >> 4) + j, c = 16 * bj + (lane & 15);
+            red64[r * 32 + c] += acc[bi][bj][j];
+          }
+    }
+    __syncthreads();
+  }
+  for (int i = threadIdx.x; i < 1024; i += blockDim.x) out[i] = red64[i];
+  __syncthreads();
+}
+
 template <bool F16>
 __device__ __forceinline__ void split8(const float (&v)[8], bf16x8& hi, bf16x8& lo, f16x8& hh, f16x8& ll) {
 #pragma unroll

@@ -264,6 +264,282 @@ __global__ void __launch_bounds__(THREADS) zgemm_bf16x3_kernel(ZGemmArgs g) {
   }
 }
 
+// ---------------------------------------------------------------- fixed plan
+// Grids n = N1 * N2 with both factors composite radices (e.g. 400 = 20 x
+// 20): the line FFT runs as two in-place passes over a padded layout, and
+// the contraction sums over z in the order the FFT leaves it (G is staged in
+// the same order), so no pass sorts its output and no pass needs a barrier
+// between its reads and its writes.
+//
+// A line holds N1 rows of RW = N2 + 1 complex: input mode k = k2 + N2 k1 sits
+// at position k2 + RW k1.
+//   pass 1, butterfly (line, k2): the N1-point inverse DFT over k1 (stride
+//     RW), times e^{2 pi i k2 z1 / n}, written back in place (row z1);
+//   pass 2, butterfly (line, z1): the N2-point inverse DFT over the
+//     contiguous row z1, written back in place: z = z1 + N1 z2 at position
+//     z2 + RW z1 (real part = first shell, imaginary part = second shell).
+// The pad column (k2 = N2) is never read by the passes and G is zero there
+// and past the KP = N1 RW positions, so the MFMA phase reads its operands
+// without masks (the pad and the last chunk's overhang hold finite values).
+#ifndef ZGF_THREADS_DEF
+#define ZGF_THREADS_DEF 768
+#endif
+constexpr int ZGF_THREADS = ZGF_THREADS_DEF;  // threads of the fixed-plan kernels
+
+template <int N1, int N2>
+struct ZgFixed {
+  static constexpr int n = N1 * N2;
+  static constexpr int RW = N2 + 1;
+  static constexpr int KP = N1 * RW;            // positions per line
+  static constexpr int NCH = (KP + 31) / 32;    // 32-wide chunks per row
+  static constexpr int GLS = NCH * 32;          // G row stride in LDS
+  static constexpr int LSC = KP + 2;            // line pitch (complex)
+  static constexpr int TAIL = 2 * (GLS - KP);   // floats read past the last line
+  __device__ static int pos(int k) { return (k % N2) + RW * (k / N2); }
+};
+
+template <int N1, int N2>
+size_t zgemm_fixed_lds_bytes(int nlines_alloc) {
+  typedef ZgFixed<N1, N2> F;
+  return sizeof(float2) * (size_t)F::n + sizeof(float) * 2 * (size_t)F::GLS +
+         sizeof(float2) * (size_t)nlines_alloc * F::LSC + sizeof(float) * (size_t)F::TAIL;
+}
+
+// 8 consecutive positions of an interleaved line: real and imaginary parts
+__device__ __forceinline__ void zgf_row8(const float2* p, float (&re)[8], float (&im)[8]) {
+  const float4* q = (const float4*)p;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const float4 v = q[j];
+    re[2 * j] = v.x;
+    im[2 * j] = v.y;
+    re[2 * j + 1] = v.z;
+    im[2 * j + 1] = v.w;
+  }
+}
+
+// The operands of one round, fetched into registers (MAXF items per thread,
+// item t = threadIdx.x + it * THREADS -> line l = t / Q, mode kz = t % Q):
+// B == A, the float4 of shells 2l and 2l + 1 (both rows of the tile); B !=
+// A, one row's (re, im) of A_l and B_l.
+template <bool BR, int MAXF>
+struct ZgfFetch {
+  typedef typename std::conditional<BR, float4, float2>::type T;
+  T a[MAXF], b[MAXF];
+};
+
+template <int THREADS, bool BR, int MAXF>
+__device__ __forceinline__ void zgf_issue(ZgfFetch<BR, MAXF>& f, const ZGemmArgs& g, const int* qa, const int* qb,
+                                          int64_t tile, int r, int total, float inv_q) {
+  const int Q = g.Q;
+  const float4* TA = g.HA + tile * g.sa_slots * Q;
+  const float4* TB = g.HB + tile * g.sb_slots * Q;
+#pragma unroll
+  for (int it = 0; it < MAXF; ++it) {
+    const int t = threadIdx.x + it * THREADS;
+    const int l = fdiv(t, Q, inv_q), kz = t - l * Q;
+    if constexpr (BR) {
+      float4 a = make_float4(0.f, 0.f, 0.f, 0.f), b = a;
+      if (t < total) {
+        const int s0 = 2 * l, s1 = s0 + 1;
+        if (s0 < g.na && kz < qa[s0]) a = TA[s0 * Q + kz];
+        if (s1 < g.na && kz < qa[s1]) b = TA[s1 * Q + kz];
+      }
+      f.a[it] = a;
+      f.b[it] = b;
+    } else {
+      float2 a = make_float2(0.f, 0.f), b = a;
+      if (t < total) {
+        if (l < g.na && kz < qa[l]) a = ((const float2*)(TA + l * Q + kz))[r];
+        if (l < g.nb && kz < qb[l]) b = ((const float2*)(TB + l * Q + kz))[r];
+      }
+      f.a[it] = a;
+      f.b[it] = b;
+    }
+  }
+}
+
+template <int THREADS, bool BR, bool BB, int N1, int N2, int MAXF>
+__global__ void __launch_bounds__(THREADS) zgemm_fixed_kernel(ZGemmArgs g) {
+  typedef ZgFixed<N1, N2> F;
+  constexpr int n = F::n, RW = F::RW, GLS = F::GLS, NCH = F::NCH, LSC = F::LSC;
+  const int Q = g.Q, R = Q - 1, half = n >> 1, L = g.nlines;
+  extern __shared__ unsigned char smem[];
+  float2* T1 = (float2*)smem;            // e^{2 pi i k2 z1 / n} at z1 N2 + k2
+  float* Gs = (float*)(T1 + n);          // G of the tile's rows in FFT order (zero at pads and past KP)
+  float2* buf = (float2*)(Gs + 2 * GLS); // the lines, LSC complex each
+  const int lalloc = BR ? L + 1 : 32;
+  for (int j = threadIdx.x; j < n; j += THREADS) {
+    const int z1 = j / N2, k2 = j - z1 * N2;
+    T1[j] = g.tw[(k2 * z1) % n];
+  }
+  for (int t = threadIdx.x; t < 2 * GLS; t += THREADS) Gs[t] = 0.f;
+  {
+    float* fb = (float*)buf;
+    for (int t = threadIdx.x; t < lalloc * 2 * LSC + F::TAIL; t += THREADS) fb[t] = 0.f;
+  }
+  __shared__ int qa[32], qb[32];
+  if (threadIdx.x < 32) {
+    qa[threadIdx.x] = g.qa[threadIdx.x];
+    qb[threadIdx.x] = g.qb[threadIdx.x];
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  constexpr int nw = THREADS / 64;
+  const bool wbb = BB && w >= (nw / 2);
+  const int wq = BB ? (wbb ? w - nw / 2 : w) : w;
+  constexpr int nwq = BB ? nw / 2 : nw;
+  const int rr = lane & 15, kg = lane >> 4;
+  const int Lh = (g.na + 1) >> 1;
+  const int nz0 = n - 2 * R - 1;  // zero modes Q .. n - Q of a line
+  const float inv_z = 1.0f / (float)(nz0 > 0 ? nz0 : 1), inv_q = 1.0f / (float)Q;
+  float acc[2][2][4];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) acc[i][j][q] = 0.f;
+  f16x8 fdh, fdl;
+  constexpr int rounds = BR ? 1 : 2;
+  const int total = BR ? Lh * Q : L * Q;  // fetched items per round
+  for (int64_t tile = blockIdx.x; tile < g.ntiles; tile += gridDim.x) {
+    const int x = (int)(tile / half), yp = (int)(tile - (int64_t)x * half);
+    for (int r0 = 0; r0 < rounds; ++r0) {
+      // every load of the round issued before the first is used: one HBM
+      // latency per round
+      ZgfFetch<BR, MAXF> pf;
+      zgf_issue<THREADS>(pf, g, qa, qb, tile, r0, total, inv_q);
+      // zero modes of every line (the previous FFT overwrote them)
+      for (int t = threadIdx.x; t < L * nz0; t += THREADS) {
+        const int l = fdiv(t, nz0, inv_z), j = t - l * nz0;
+        buf[l * LSC + F::pos(Q + j)] = make_float2(0.f, 0.f);
+      }
+      // the round's G row(s) in FFT order: z = z1 + N1 z2 at z2 + RW z1
+      {
+        const float* Grow = g.G + ((int64_t)x * n + 2 * yp) * n + (BR ? 0 : r0 * n);
+        for (int t = threadIdx.x; t < (BR ? 2 : 1) * n; t += THREADS) {
+          const int r = BR ? (t >= n ? 1 : 0) : r0, z = t - (BR && t >= n ? n : 0);
+          const int z2 = z / N1, z1 = z - z2 * N1;
+          Gs[r * GLS + z2 + RW * z1] = Grow[t];
+        }
+      }
+      // the round's lines from the fetched registers
+#pragma unroll
+      for (int it = 0; it < MAXF; ++it) {
+        const int t = threadIdx.x + it * THREADS;
+        if (t < total) {
+          const int l = fdiv(t, Q, inv_q), kz = t - l * Q;
+          if constexpr (BR) {
+            const float4 a = pf.a[it], b = pf.b[it];
+#pragma unroll
+            for (int r = 0; r < 2; ++r) {
+              const float2 p = r ? make_float2(a.z, a.w) : make_float2(a.x, a.y);
+              const float2 q = r ? make_float2(b.z, b.w) : make_float2(b.x, b.y);
+              float2* Ln = buf + (r * Lh + l) * LSC;
+              if (kz == 0) {  // c2r reads only the real part of the zero mode
+                Ln[0] = make_float2(p.x, q.x);
+              } else {
+                Ln[F::pos(kz)] = make_float2(p.x - q.y, p.y + q.x);
+                Ln[F::pos(n - kz)] = make_float2(p.x + q.y, q.x - p.y);
+              }
+            }
+          } else {
+            const float2 p = pf.a[it], q = pf.b[it];
+            float2* Ln = buf + l * LSC;
+            if (kz == 0) {
+              Ln[0] = make_float2(p.x, q.x);
+            } else {
+              Ln[F::pos(kz)] = make_float2(p.x - q.y, p.y + q.x);
+              Ln[F::pos(n - kz)] = make_float2(p.x + q.y, q.x - p.y);
+            }
+          }
+        }
+      }
+      __syncthreads();
+      // pass 1: N1-point DFTs down the columns, twiddled, in place
+      for (int t = threadIdx.x; t < L * N2; t += THREADS) {
+        const int l = t / N2, k2 = t - l * N2;
+        float2* col = buf + l * LSC + k2;
+        float2 u[N1];
+#pragma unroll
+        for (int k1 = 0; k1 < N1; ++k1) u[k1] = col[RW * k1];
+        SmallDft<N1>::run(u);
+        col[0] = u[0];
+#pragma unroll
+        for (int z1 = 1; z1 < N1; ++z1) col[RW * z1] = cmul(u[z1], T1[z1 * N2 + k2]);
+      }
+      __syncthreads();
+      // pass 2: N2-point DFTs along the rows, in place
+      for (int t = threadIdx.x; t < L * N1; t += THREADS) {
+        const int l = t / N1, z1 = t - l * N1;
+        float2* row = buf + l * LSC + RW * z1;
+        float2 u[N2];
+#pragma unroll
+        for (int k2 = 0; k2 < N2; ++k2) u[k2] = row[k2];
+        SmallDft<N2>::run(u);
+#pragma unroll
+        for (int z2 = 0; z2 < N2; ++z2) row[z2] = u[z2];
+      }
+      __syncthreads();
+      constexpr int nchunk = BR ? 2 * NCH : NCH;
+      for (int c = wq; c < nchunk; c += nwq) {
+        const int r = BR ? (c >= NCH ? 1 : 0) : r0;
+        const int z0 = 32 * (c - ((BR && r) ? NCH : 0)) + 8 * kg;
+        bf16x8 ah[2], al[2], bh[2], bl[2];
+        float vg[8];
+        {
+          const float4* gp = (const float4*)(Gs + r * GLS + z0);
+          const float4 g0 = gp[0], g1 = gp[1];
+          vg[0] = g0.x; vg[1] = g0.y; vg[2] = g0.z; vg[3] = g0.w;
+          vg[4] = g1.x; vg[5] = g1.y; vg[6] = g1.z; vg[7] = g1.w;
+        }
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int i = 16 * h + rr;
+          float re[8], im[8], a[8], b[8];
+          if constexpr (BR) {  // row i: part i & 1 of line r Lh + i / 2 (rows past the lines: the zero line L)
+            const int line = (i >> 1) < Lh ? r * Lh + (i >> 1) : L;
+            zgf_row8(buf + line * LSC + z0, re, im);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+              a[j] = (i & 1) ? im[j] : re[j];
+              b[j] = a[j] * vg[j];
+            }
+          } else {  // line i: A_i real, B_i imaginary
+            zgf_row8(buf + i * LSC + z0, re, im);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+              a[j] = wbb ? im[j] : re[j];
+              b[j] = im[j] * vg[j];
+            }
+          }
+          split8<false>(a, ah[h], al[h], fdh, fdl);
+          split8<false>(b, bh[h], bl[h], fdh, fdl);
+        }
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int j = 0; j < 2; ++j) {
+            f32x4 cacc = {acc[i][j][0], acc[i][j][1], acc[i][j][2], acc[i][j][3]};
+            cacc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], cacc, 0, 0, 0);
+            cacc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], cacc, 0, 0, 0);
+            cacc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], cacc, 0, 0, 0);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) acc[i][j][q] = cacc[q];
+          }
+      }
+      __syncthreads();  // the next round overwrites the lines and G
+    }
+  }
+  if (BB) {
+    block_reduce_store(acc, g.partial + (int64_t)blockIdx.x * 1024, 0, nw / 2);
+    block_reduce_store(acc, g.partial_bb + (int64_t)blockIdx.x * 1024, nw / 2, nw);
+  } else {
+    block_reduce_store(acc, g.partial + (int64_t)blockIdx.x * 1024);
+  }
+}
+
 // Launch shapes.  B == A (kernel BR = true): 2 * ceil(na/2) lines per round
 // (both rows) and one zero line.  B != A: max(na, nb) lines per round, 32
 // lines of LDS (the rows beyond stay zero).  768 threads, one workgroup per
@@ -290,8 +566,9 @@ int zgemm_supported(const SynthArgs& a, int na, int nb, int same, size_t lds_lim
   return zgemm_lds_bytes(n, zg_lines_alloc(L, same)) + 32 * 32 * sizeof(double) <= lds_limit ? 1 : 0;
 }
 
-// workgroups per CU the launch shape allows
-int zgemm_blocks_per_cu(int) { return 1; }
+// default workgroups per CU (one is resident at a time): four keep each
+// wave's FP32 sums a quarter as long as one, at the same launch time
+int zgemm_blocks_per_cu(int) { return 4; }
 
 template <int T, bool BR, bool BB = false>
 static hipError_t zg_launch(const ZGemmArgs& g, int nblk, size_t lds, hipStream_t st) {
@@ -311,6 +588,40 @@ static hipError_t zg_launch(const ZGemmArgs& g, int nblk, size_t lds, hipStream_
   }
   hipLaunchKernelGGL((zgemm_bf16x3_kernel<T, BR, BB>), dim3(nblk), dim3(T), lds, st, g);
   return hipGetLastError();
+}
+
+template <int T, bool BR, bool BB, int N1, int N2, int MAXF>
+static hipError_t zgf_launch_f(const ZGemmArgs& g, int nblk, size_t lds, hipStream_t st) {
+  static size_t allowed[CNPC_MAX_DEVICES] = {};
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= CNPC_MAX_DEVICES) return hipErrorInvalidDevice;
+  if (lds > allowed[dev]) {
+    e = hipFuncSetAttribute((const void*)zgemm_fixed_kernel<T, BR, BB, N1, N2, MAXF>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    allowed[dev] = lds;
+  }
+  hipLaunchKernelGGL((zgemm_fixed_kernel<T, BR, BB, N1, N2, MAXF>), dim3(nblk), dim3(T), lds, st, g);
+  return hipGetLastError();
+}
+
+// items a thread fetches per round: B == A, ceil(na / 2) Q float4 pairs;
+// B != A, max(na, nb) Q float2 pairs (at most 4 per thread)
+static int zgf_maxf(int Q, int na, int nb, int same) {
+  const int total = same ? ((na + 1) / 2) * Q : (na > nb ? na : nb) * Q;
+  return total <= 2 * ZGF_THREADS ? 2 : total <= 4 * ZGF_THREADS ? 4 : total <= 8 * ZGF_THREADS ? 8 : 0;
+}
+
+template <int T, bool BR, bool BB, int N1, int N2>
+static hipError_t zgf_launch(const ZGemmArgs& g, int nblk, size_t lds, hipStream_t st) {
+  switch (zgf_maxf(g.Q, g.na, g.nb, BR ? 1 : 0)) {
+    case 2: return zgf_launch_f<T, BR, BB, N1, N2, 2>(g, nblk, lds, st);
+    case 4: return zgf_launch_f<T, BR, BB, N1, N2, 4>(g, nblk, lds, st);
+    case 8: return zgf_launch_f<T, BR, BB, N1, N2, 8>(g, nblk, lds, st);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 hipError_t launch_zgemm(const SynthArgs& a, int Q, const void* HA, int na, int sa_slots, const int* qa,
@@ -334,14 +645,24 @@ hipError_t launch_zgemm(const SynthArgs& a, int Q, const void* HA, int na, int s
   g.tw = (const float2*)tw;
   g.partial = partial;
   g.partial_bb = partial + (int64_t)nblk * 1024;  // used by BB launches only
-  const size_t lds = zgemm_lds_bytes(a.n, zg_lines_alloc(g.nlines, same));
   hipError_t e;
-  if (same)
-    e = zg_launch<ZG_THREADS, true>(g, nblk, lds, st);
-  else if (Cbb)
-    e = zg_launch<ZG_THREADS, false, true>(g, nblk, lds, st);
-  else
-    e = zg_launch<ZG_THREADS, false>(g, nblk, lds, st);
+  if (a.n == 400 && zgf_maxf(Q, na, g.nb, same) > 0) {  // fixed plan 400 = 20 x 20
+    const size_t lds = zgemm_fixed_lds_bytes<20, 20>(zg_lines_alloc(g.nlines, same));
+    if (same)
+      e = zgf_launch<ZGF_THREADS, true, false, 20, 20>(g, nblk, lds, st);
+    else if (Cbb)
+      e = zgf_launch<ZGF_THREADS, false, true, 20, 20>(g, nblk, lds, st);
+    else
+      e = zgf_launch<ZGF_THREADS, false, false, 20, 20>(g, nblk, lds, st);
+  } else {
+    const size_t lds = zgemm_lds_bytes(a.n, zg_lines_alloc(g.nlines, same));
+    if (same)
+      e = zg_launch<ZG_THREADS, true>(g, nblk, lds, st);
+    else if (Cbb)
+      e = zg_launch<ZG_THREADS, false, true>(g, nblk, lds, st);
+    else
+      e = zg_launch<ZG_THREADS, false>(g, nblk, lds, st);
+  }
   if (e != hipSuccess) return e;
   e = launch_gemm_finalize(partial, nblk, 0, 0, na, g.nb, 1.0, C, g.nb, st);
   if (e != hipSuccess || !Cbb) return e;

@@ -206,15 +206,10 @@ def _contract(ctx, mode, prec, G, R1, R2, pairs):
 
 
 def _zgemm_blocks(ctx, zargs, with_bb=False):
-    """Workgroups of the fused contraction (1, 2 or 4 per CU), autotuned per
-    grid and row counts."""
-    from . import autotune
-
-    n, rows_a, rows_b, same = zargs[0], zargs[5], zargs[8], zargs[7] is None
-    ncu = torch.cuda.get_device_properties(ctx.device).multi_processor_count
-    shape = (n, rows_a, rows_b, same) + (("bb",) if with_bb else ())
-    return autotune.choose("zgemm_blocks", shape, [ncu, 2 * ncu, 4 * ncu],
-                           lambda c: ctx.contract_zgemm(*zargs, nblk=c, with_bb=with_bb))
+    """Workgroups of the fused contraction: 0 lets the library take four per
+    CU (a quarter of the tiles per workgroup keeps each wave's FP32 sums
+    short; the launch time is flat from one to four per CU)."""
+    return 0
 
 
 class _ZRequest:

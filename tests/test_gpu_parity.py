@@ -310,7 +310,8 @@ def test_shell_synth_matches_filter_c2r(gpu_ctx, prec, n, L, kmax):
             assert np.max(np.abs(ah - bh)) <= tol * np.max(np.abs(bh)), (ell, m, src is None)
 
 
-@pytest.mark.parametrize("n,L,kmax", [(40, 500.0, 0.12), (48, 600.0, 0.13), (80, 1000.0, 0.15), (200, 1000.0, 0.3)])
+@pytest.mark.parametrize("n,L,kmax", [(40, 500.0, 0.12), (48, 600.0, 0.13), (80, 1000.0, 0.15), (200, 1000.0, 0.3),
+                                      (400, 2000.0, 0.31)])
 def test_zgemm_fused_matches_fp64_contraction(gpu_ctx, n, L, kmax):
     """Fused z synthesis + split-bf16 GEMM (half-synthesised shells in slots)
     against the FP64 real shells contracted by the FP64 GEMM: B == A with an
