@@ -337,6 +337,82 @@ __device__ void stockham_lds(C* buf, const C* Wt, int n, int ls, int nl, const S
   }
 }
 
+// ------------------------------------------------------------ fixed plans
+// n = N1 * N2 with both factors radices of SmallDft (e.g. 400 = 20 x 20): an
+// inverse FFT as two in-place passes over a padded line of N1 rows of RW =
+// N2 + 1 complex.  Input mode k = k2 + N2 k1 sits at position k2 + RW k1.
+//   pass 1, butterfly (line, k2): the N1-point DFT over k1 (stride RW),
+//     times e^{2 pi i k2 z1 / n}, written back in place (row z1);
+//   pass 2, butterfly (line, z1): the N2-point DFT over the contiguous row
+//     z1, written back in place: output z = z1 + N1 z2 at position z2 + RW z1.
+// Each butterfly reads and writes only its own positions, so a pass needs no
+// barrier between its reads and writes; rows start RW (odd for even N2)
+// entries apart, which spreads pass 2 over the LDS banks.  The pad column
+// (k2 = N2) is never read.  Consumers either read the output in this order
+// (the fused contraction, with G staged alike) or gather it by opos(z).
+template <int N1, int N2>
+struct FixedPlan {
+  static constexpr int n = N1 * N2;
+  static constexpr int RW = N2 + 1;
+  static constexpr int KP = N1 * RW;  // positions per line
+  __device__ static int ipos(int k) { return (k % N2) + RW * (k / N2); }  // input mode k
+  __device__ static int opos(int z) { return (z / N1) + RW * (z % N1); }  // output z
+};
+
+// twiddles of pass 1: T1[z1 N2 + k2] = e^{2 pi i k2 z1 / n}, from the table
+// tw[j] = e^{2 pi i j / n}
+template <int N1, int N2, typename C>
+__device__ __forceinline__ void fixed_twiddles(C* __restrict__ T1, const C* __restrict__ tw) {
+  constexpr int n = N1 * N2;
+  for (int j = threadIdx.x; j < n; j += blockDim.x) {
+    const int z1 = j / N2, k2 = j - z1 * N2;
+    T1[j] = tw[(k2 * z1) % n];
+  }
+}
+
+// Both passes over nl lines of pitch lsc (complex); the caller synchronises
+// before (inputs written) and after (outputs read).  qz > 0: the modes qz ..
+// n - qz are zero and not read (whatever the positions hold); qz <= 0: every
+// mode is read.  Thread t takes line t % nl of butterfly t / nl: lanes of a
+// wave walk the lines first, whose pitch (lsc = 6 mod 32 for the fused
+// contraction's lines) spreads them over the LDS banks.
+template <int N1, int N2, int THREADS, typename C>
+__device__ __forceinline__ void fixed_fft_inplace(C* __restrict__ buf, const C* __restrict__ T1, int nl, int lsc,
+                                                  int qz = 0) {
+  constexpr int RW = N2 + 1, n = N1 * N2;
+  const float inv_nl = 1.0f / (float)nl;
+  for (int t = threadIdx.x; t < nl * N2; t += THREADS) {
+    const int k2 = fdiv(t, nl, inv_nl), l = t - k2 * nl;
+    C* col = buf + l * lsc + k2;
+    C u[N1];
+#pragma unroll
+    for (int k1 = 0; k1 < N1; ++k1) {
+      const int k = k2 + N2 * k1;
+      if (qz > 0 && k >= qz && k <= n - qz) {
+        u[k1].x = 0;
+        u[k1].y = 0;
+      } else {
+        u[k1] = col[RW * k1];
+      }
+    }
+    SmallDft<N1>::run(u);
+    col[0] = u[0];
+#pragma unroll
+    for (int z1 = 1; z1 < N1; ++z1) col[RW * z1] = cmul(u[z1], T1[z1 * N2 + k2]);
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < nl * N1; t += THREADS) {
+    const int z1 = fdiv(t, nl, inv_nl), l = t - z1 * nl;
+    C* row = buf + l * lsc + RW * z1;
+    C u[N2];
+#pragma unroll
+    for (int k2 = 0; k2 < N2; ++k2) u[k2] = row[k2];
+    SmallDft<N2>::run(u);
+#pragma unroll
+    for (int z2 = 0; z2 < N2; ++z2) row[z2] = u[z2];
+  }
+}
+
 // butterflies per thread of a pass (bounds the registers of the staged pass)
 template <typename T> struct MaxB { static constexpr int v = 4; };
 

@@ -279,6 +279,121 @@ __global__ void __launch_bounds__(SYNTH_THREADS) zfft_kernel(SynthArgs a, int NL
   }
 }
 
+// ------------------------------------------------------------ fixed plans
+// The x and y passes of the half synthesis for grids n = N1 * N2 with the
+// two-pass in-place FFT of lds_fft.h (FixedPlan): inputs are scattered to
+// their padded positions, outputs gathered back in natural order, so U and
+// H are the same as with the generic passes.  NL lines per workgroup, line
+// pitch LSC = KP + 2 complex.
+template <int N1, int N2>
+struct SynthFixed {
+  typedef FixedPlan<N1, N2> P;
+  static constexpr int LSC = P::KP + 2;
+};
+
+// Mode k of every line of the block into its padded position: thread t takes
+// k = t, t + THREADS, ... of every line (no divisions by the line length),
+// all loads of a thread issued before the first LDS write.  get(l, f) is the
+// value of frequency f in [-n/2, n/2) of line l (zero outside the shell).
+template <int N1, int N2, int THREADS, int NL, typename Get>
+__device__ __forceinline__ void fixed_load_lines(float2* __restrict__ buf, int nl, Get get) {
+  typedef FixedPlan<N1, N2> F;
+  constexpr int n = F::n, LSC = SynthFixed<N1, N2>::LSC, KPT = (n + THREADS - 1) / THREADS;
+  float2 v[NL][KPT];
+#pragma unroll
+  for (int l = 0; l < NL; ++l)
+#pragma unroll
+    for (int i = 0; i < KPT; ++i) {
+      const int k = threadIdx.x + i * THREADS;
+      v[l][i] = make_float2(0.f, 0.f);
+      if (l < nl && k < n) v[l][i] = get(l, k < (n >> 1) ? k : k - n);
+    }
+#pragma unroll
+  for (int l = 0; l < NL; ++l)
+#pragma unroll
+    for (int i = 0; i < KPT; ++i) {
+      const int k = threadIdx.x + i * THREADS;
+      if (l < nl && k < n) buf[l * LSC + F::ipos(k)] = v[l][i];
+    }
+}
+
+template <int N1, int N2, int THREADS, int NL>
+__global__ void __launch_bounds__(THREADS) xfft_fixed_kernel(SynthArgs a, const float2* __restrict__ V,
+                                                             const signed char* __restrict__ bins,
+                                                             const float2* __restrict__ twg, float2* __restrict__ U) {
+  typedef FixedPlan<N1, N2> F;
+  constexpr int n = F::n, LSC = SynthFixed<N1, N2>::LSC;
+  const int R = a.R, P = 2 * R + 1, Q = R + 1;
+  const int ky0 = blockIdx.x * NL, kz = blockIdx.y;
+  const int nl = min(NL, P - ky0);
+  extern __shared__ unsigned char smem[];
+  float2* T1 = (float2*)smem;
+  float2* buf = T1 + n;
+  fixed_twiddles<N1, N2>(T1, twg);
+  const int64_t lbase = ((int64_t)kz * P + ky0) * P;
+  for (int s = 0; s < a.nshell; ++s) {
+    const signed char want = (signed char)(a.bin0 + s);
+    // lines with kz or every ky of the block beyond the shell's radius are
+    // zero for this shell: skipped (the y pass never reads them)
+    const int rs = a.rs[s];
+    if (kz > rs || ky0 - R > rs || ky0 + nl - 1 - R < -rs) continue;
+    fixed_load_lines<N1, N2, THREADS, NL>(buf, nl, [&](int l, int f) {
+      float2 v = make_float2(0.f, 0.f);
+      if (f >= -R && f <= R) {
+        const int64_t j = lbase + (int64_t)l * P + f + R;
+        if (bins[j] == want) v = V[j];
+      }
+      return v;
+    });
+    __syncthreads();
+    fixed_fft_inplace<N1, N2, THREADS>(buf, T1, nl, LSC);
+    __syncthreads();
+    // U[s][x][kz][ky]: consecutive threads take consecutive ky of one x
+    float2* dst = U + ((int64_t)s * n * Q + kz) * P + ky0;
+    for (int t = threadIdx.x; t < NL * n; t += THREADS) {
+      const int x = t / NL, l = t - x * NL;
+      if (l < nl) dst[(int64_t)x * Q * P + l] = buf[l * LSC + F::opos(x)];
+    }
+    __syncthreads();
+  }
+}
+
+// y pass into the half-synthesised layout H[x][y/2][slot0 + s][kz][y%2]
+template <int N1, int N2, int THREADS, int NL>
+__global__ void __launch_bounds__(THREADS) yfft_fixed_kernel(SynthArgs a, const float2* __restrict__ U,
+                                                             const float2* __restrict__ twg, float2* __restrict__ H,
+                                                             int nslots, int slot0) {
+  typedef FixedPlan<N1, N2> F;
+  constexpr int n = F::n, LSC = SynthFixed<N1, N2>::LSC;
+  const int R = a.R, P = 2 * R + 1, Q = R + 1;
+  const int64_t plane = blockIdx.x;
+  const int kz0 = blockIdx.y * NL;
+  const int nl = min(NL, Q - kz0);
+  const int s = (int)(plane / n), x = (int)(plane - (int64_t)s * n);
+  // kz lines beyond the shell's radius are zero: not computed, not written
+  const int rs = a.rs[s];
+  if (kz0 > rs) return;
+  extern __shared__ unsigned char smem[];
+  float2* T1 = (float2*)smem;
+  float2* buf = T1 + n;
+  fixed_twiddles<N1, N2>(T1, twg);
+  const float2* src = U + (plane * Q + kz0) * P;
+  fixed_load_lines<N1, N2, THREADS, NL>(buf, nl, [&](int l, int f) {
+    return (f >= -rs && f <= rs && kz0 + l <= rs) ? src[l * P + f + R] : make_float2(0.f, 0.f);
+  });
+  __syncthreads();
+  fixed_fft_inplace<N1, N2, THREADS>(buf, T1, nl, LSC);
+  __syncthreads();
+  // each y pair writes 2 nl consecutive entries: thread t -> pair t / (2 NL),
+  // entry (l, y % 2) = t % (2 NL)
+  float2* dst = H + (((int64_t)x * (n >> 1) * nslots + slot0 + s) * Q + kz0) * 2;
+  const int64_t ystride = (int64_t)nslots * Q * 2;
+  for (int t = threadIdx.x; t < NL * n; t += THREADS) {
+    const int yp = t / (2 * NL), e = t - yp * (2 * NL), l = e >> 1, j = e & 1;
+    if (l < nl) dst[yp * ystride + e] = buf[l * LSC + F::opos(2 * yp + j)];
+  }
+}
+
 template <typename T>
 __global__ void twiddle_kernel(int n, typename cplx<T>::type* tw) {
   for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
@@ -312,6 +427,12 @@ int synth_lines_per_block(int prec, const SynthArgs& a) {
   const int cap = 16 * SYNTH_THREADS / 256;
   return nl > cap ? cap : nl;
 }
+
+// lines per workgroup of the fixed-plan passes (400 = 20 x 20): twiddles +
+// NL padded lines within the LDS budget
+#ifndef CNPC_SYNTH_FIXED_NL
+#define CNPC_SYNTH_FIXED_NL 8
+#endif
 
 template <typename T>
 static hipError_t launch_synth_t(const SynthArgs& a, const BinEdges& E, const PolySpec& Wp, const void* D, void* V,
@@ -364,10 +485,19 @@ hipError_t launch_synth_half(const SynthArgs& a, const BinEdges& E, const PolySp
     hipLaunchKernelGGL(cube_kernel<float>, dim3(grid_for(cube, 256, 1 << 14)), blk, 0, st, a, E, Wp, (const C*)D,
                        (C*)V, bins);
   }
-  const int NL = synth_lines_per_block(1, a);
-  const size_t lds = sizeof(C) * ((size_t)n + (size_t)NL * (n + 1));
   const int64_t planes = (int64_t)a.nshell * n;
   const dim3 fblk(SYNTH_THREADS);
+  if (n == 400) {  // fixed plan 400 = 20 x 20
+    constexpr int NL = CNPC_SYNTH_FIXED_NL;
+    const size_t lds = sizeof(C) * ((size_t)n + (size_t)NL * SynthFixed<20, 20>::LSC);
+    hipLaunchKernelGGL((xfft_fixed_kernel<20, 20, SYNTH_THREADS, NL>), dim3((P + NL - 1) / NL, Q), fblk, lds, st, a,
+                       (const C*)V, bins, (const C*)tw, (C*)work);
+    hipLaunchKernelGGL((yfft_fixed_kernel<20, 20, SYNTH_THREADS, NL>), dim3((unsigned)planes, (Q + NL - 1) / NL),
+                       fblk, lds, st, a, (const C*)work, (const C*)tw, (C*)out, nslots, slot0);
+    return hipGetLastError();
+  }
+  const int NL = synth_lines_per_block(1, a);
+  const size_t lds = sizeof(C) * ((size_t)n + (size_t)NL * (n + 1));
   hipLaunchKernelGGL(xfft_kernel<float>, dim3((P + NL - 1) / NL, Q), fblk, lds, st, a, NL, (const C*)V, bins,
                      (const C*)tw, (C*)work);
   hipLaunchKernelGGL(yfft_kernel<float>, dim3((unsigned)planes, (Q + NL - 1) / NL), fblk, lds, st, a, NL,

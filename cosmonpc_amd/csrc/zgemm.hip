@@ -287,15 +287,13 @@ __global__ void __launch_bounds__(THREADS) zgemm_bf16x3_kernel(ZGemmArgs g) {
 constexpr int ZGF_THREADS = ZGF_THREADS_DEF;  // threads of the fixed-plan kernels
 
 template <int N1, int N2>
-struct ZgFixed {
-  static constexpr int n = N1 * N2;
-  static constexpr int RW = N2 + 1;
-  static constexpr int KP = N1 * RW;            // positions per line
-  static constexpr int NCH = (KP + 31) / 32;    // 32-wide chunks per row
-  static constexpr int GLS = NCH * 32;          // G row stride in LDS
-  static constexpr int LSC = KP + 2;            // line pitch (complex)
-  static constexpr int TAIL = 2 * (GLS - KP);   // floats read past the last line
-  __device__ static int pos(int k) { return (k % N2) + RW * (k / N2); }
+struct ZgFixed : FixedPlan<N1, N2> {
+  typedef FixedPlan<N1, N2> B;
+  static constexpr int NCH = (B::KP + 31) / 32;    // 32-wide chunks per row
+  static constexpr int GLS = NCH * 32;             // G row stride in LDS
+  static constexpr int LSC = B::KP + 2;            // line pitch (complex)
+  static constexpr int TAIL = 2 * (GLS - B::KP);   // floats read past the last line
+  __device__ static int pos(int k) { return B::ipos(k); }
 };
 
 template <int N1, int N2>
@@ -363,16 +361,13 @@ template <int THREADS, bool BR, bool BB, int N1, int N2, int MAXF>
 __global__ void __launch_bounds__(THREADS) zgemm_fixed_kernel(ZGemmArgs g) {
   typedef ZgFixed<N1, N2> F;
   constexpr int n = F::n, RW = F::RW, GLS = F::GLS, NCH = F::NCH, LSC = F::LSC;
-  const int Q = g.Q, R = Q - 1, half = n >> 1, L = g.nlines;
+  const int Q = g.Q, half = n >> 1, L = g.nlines;
   extern __shared__ unsigned char smem[];
   float2* T1 = (float2*)smem;            // e^{2 pi i k2 z1 / n} at z1 N2 + k2
   float* Gs = (float*)(T1 + n);          // G of the tile's rows in FFT order (zero at pads and past KP)
   float2* buf = (float2*)(Gs + 2 * GLS); // the lines, LSC complex each
   const int lalloc = BR ? L + 1 : 32;
-  for (int j = threadIdx.x; j < n; j += THREADS) {
-    const int z1 = j / N2, k2 = j - z1 * N2;
-    T1[j] = g.tw[(k2 * z1) % n];
-  }
+  fixed_twiddles<N1, N2>(T1, g.tw);
   for (int t = threadIdx.x; t < 2 * GLS; t += THREADS) Gs[t] = 0.f;
   {
     float* fb = (float*)buf;
@@ -391,8 +386,7 @@ __global__ void __launch_bounds__(THREADS) zgemm_fixed_kernel(ZGemmArgs g) {
   constexpr int nwq = BB ? nw / 2 : nw;
   const int rr = lane & 15, kg = lane >> 4;
   const int Lh = (g.na + 1) >> 1;
-  const int nz0 = n - 2 * R - 1;  // zero modes Q .. n - Q of a line
-  const float inv_z = 1.0f / (float)(nz0 > 0 ? nz0 : 1), inv_q = 1.0f / (float)Q;
+  const float inv_q = 1.0f / (float)Q;
   float acc[2][2][4];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
@@ -410,11 +404,6 @@ __global__ void __launch_bounds__(THREADS) zgemm_fixed_kernel(ZGemmArgs g) {
       // latency per round
       ZgfFetch<BR, MAXF> pf;
       zgf_issue<THREADS>(pf, g, qa, qb, tile, r0, total, inv_q);
-      // zero modes of every line (the previous FFT overwrote them)
-      for (int t = threadIdx.x; t < L * nz0; t += THREADS) {
-        const int l = fdiv(t, nz0, inv_z), j = t - l * nz0;
-        buf[l * LSC + F::pos(Q + j)] = make_float2(0.f, 0.f);
-      }
       // the round's G row(s) in FFT order: z = z1 + N1 z2 at z2 + RW z1
       {
         const float* Grow = g.G + ((int64_t)x * n + 2 * yp) * n + (BR ? 0 : r0 * n);
@@ -457,30 +446,9 @@ __global__ void __launch_bounds__(THREADS) zgemm_fixed_kernel(ZGemmArgs g) {
         }
       }
       __syncthreads();
-      // pass 1: N1-point DFTs down the columns, twiddled, in place
-      for (int t = threadIdx.x; t < L * N2; t += THREADS) {
-        const int l = t / N2, k2 = t - l * N2;
-        float2* col = buf + l * LSC + k2;
-        float2 u[N1];
-#pragma unroll
-        for (int k1 = 0; k1 < N1; ++k1) u[k1] = col[RW * k1];
-        SmallDft<N1>::run(u);
-        col[0] = u[0];
-#pragma unroll
-        for (int z1 = 1; z1 < N1; ++z1) col[RW * z1] = cmul(u[z1], T1[z1 * N2 + k2]);
-      }
-      __syncthreads();
-      // pass 2: N2-point DFTs along the rows, in place
-      for (int t = threadIdx.x; t < L * N1; t += THREADS) {
-        const int l = t / N1, z1 = t - l * N1;
-        float2* row = buf + l * LSC + RW * z1;
-        float2 u[N2];
-#pragma unroll
-        for (int k2 = 0; k2 < N2; ++k2) u[k2] = row[k2];
-        SmallDft<N2>::run(u);
-#pragma unroll
-        for (int z2 = 0; z2 < N2; ++z2) row[z2] = u[z2];
-      }
+      // the line FFTs, in place (z = z1 + N1 z2 at position z2 + RW z1)
+      // (modes Q .. n - Q are zero: pass 1 does not read their positions)
+      fixed_fft_inplace<N1, N2, THREADS>(buf, T1, L, LSC, Q);
       __syncthreads();
       constexpr int nchunk = BR ? 2 * NCH : NCH;
       for (int c = wq; c < nchunk; c += nwq) {
