@@ -46,7 +46,7 @@ SIGNATURES = {
     "cnpc_ctx_hold": [_P, _I],
     "cnpc_ctx_clear_plans": [_P],
     "cnpc_paint": [_P, _P, _P, _L, _I, _D, _I, _I, _P, _I, _P, _P],
-    "cnpc_paint_batch": [_P, _P, _P, _P, _I, _L, _I, _D, _I, _I, _P, _I, _P, _P],
+    "cnpc_paint_batch": [_P, _P, _P, _P, _I, _L, _L, _I, _D, _I, _I, _P, _I, _P, _P],
     "cnpc_fft_r2c": [_P, _I, _I, _I, _P, _P],
     "cnpc_fft_c2r": [_P, _I, _I, _I, _P, _P],
     "cnpc_prepare_spectrum": [_P, _I, _D, _I, _P, _P, _I, _I, _I, _D, _D, _I, _P],

@@ -304,9 +304,9 @@ int cnpc_paint(cnpc_ctx* ctx, const double* pos, const double* w, int64_t np, in
   REQUIRE(np >= 0 && (np == 0 || (pos && w)), "cnpc_paint: null particle arrays");
   REQUIRE(mesh0 && (!interlaced || mesh1), "cnpc_paint: null mesh");
   if (paint_tiled_ok(n) && np < ((int64_t)1 << 31)) {  // sorted, LDS-tiled, writes every cell
-    if (int rc = ensure_pwork(ctx, paint_tiled_work_bytes(np, n))) return rc;
-    CHECK_HIP(launch_paint_tiled(pos, w, np, n, boxsize, window, center, interlaced ? 2 : 1, prec, mesh0, mesh1,
-                                 ctx->pwork, ctx->stream));
+    if (int rc = ensure_pwork(ctx, paint_tiled_work_bytes(np, n, 1))) return rc;
+    CHECK_HIP(launch_paint_tiled(pos, w, nullptr, 1, np, np, n, boxsize, window, center, interlaced ? 2 : 1, prec,
+                                 mesh0, mesh1, ctx->pwork, ctx->stream));
     return 0;
   }
   const int64_t bytes = (int64_t)n * n * n * (prec == 0 ? 8 : 4);
@@ -318,7 +318,7 @@ int cnpc_paint(cnpc_ctx* ctx, const double* pos, const double* w, int64_t np, in
 }
 
 int cnpc_paint_batch(cnpc_ctx* ctx, const double* pos, const double* w, const int64_t* offsets, int nbatch,
-                     int64_t max_count, int n, double boxsize, int window, int interlaced, const double* center, int prec,
+                     int64_t total_count, int64_t max_count, int n, double boxsize, int window, int interlaced, const double* center, int prec,
                      void* mesh0, void* mesh1) {
   REQUIRE(ctx, "null context");
   REQUIRE(n >= 2 && n % 2 == 0 && n <= 4096, "cnpc_paint_batch: nmesh must be even and in [2, 4096]");
@@ -328,6 +328,16 @@ int cnpc_paint_batch(cnpc_ctx* ctx, const double* pos, const double* w, const in
   REQUIRE(nbatch >= 1 && nbatch <= 65535 && max_count >= 0, "cnpc_paint_batch: bad batch size");
   REQUIRE(offsets && mesh0 && (!interlaced || mesh1), "cnpc_paint_batch: null offsets or meshes");
   REQUIRE(max_count == 0 || (pos && w), "cnpc_paint_batch: null particle arrays");
+  REQUIRE(total_count >= max_count, "cnpc_paint_batch: total_count below max_count");
+  if (paint_tiled_ok(n) && total_count >= 0 && total_count < ((int64_t)1 << 31) &&
+      (int64_t)nbatch * (n / 8) * (n / 8) * (n / 8) * 8 < ((int64_t)1 << 31)) {
+    // sorted, LDS-tiled, every cell of every mesh written once
+    if (int rc = ensure_pwork(ctx, paint_tiled_work_bytes(total_count, n, nbatch))) return rc;
+    CHECK_HIP(launch_paint_tiled(pos, w, offsets, nbatch, total_count, max_count, n, boxsize, window, center,
+                                 interlaced ? 2 : 1, prec, mesh0, interlaced ? mesh1 : nullptr, ctx->pwork,
+                                 ctx->stream));
+    return 0;
+  }
   const int64_t bytes = (int64_t)nbatch * n * n * n * (prec == 0 ? 8 : 4);
   if (int rc = cnpc_zero(ctx, mesh0, bytes)) return rc;
   if (interlaced)

@@ -312,10 +312,10 @@ hipError_t launch_synth(int prec, const SynthArgs& a, const BinEdges& E, const P
                         signed char* bins, void* tw, void* work, void* out, hipStream_t st);
 
 int paint_tiled_ok(int n);
-size_t paint_tiled_work_bytes(int64_t np, int n);
-hipError_t launch_paint_tiled(const double* pos, const double* w, int64_t np, int n, double boxsize, int order,
-                              const double* center, int ncopy, int prec, void* m0, void* m1, void* work,
-                              hipStream_t st);
+size_t paint_tiled_work_bytes(int64_t np, int n, int nbatch);
+hipError_t launch_paint_tiled(const double* pos, const double* w, const int64_t* off, int nbatch, int64_t np,
+                              int64_t np_max, int n, double boxsize, int order, const double* center, int ncopy,
+                              int prec, void* m0, void* m1, void* work, hipStream_t st);
 hipError_t launch_paint(const double* pos, const double* w, int64_t np, int n, double boxsize, int order,
                         const double* center, int ncopy, int prec, void* m0, void* m1, hipStream_t st);
 hipError_t launch_paint_batch(const double* pos, const double* w, const int64_t* off, int nb, int64_t np_max, int n,

@@ -104,7 +104,8 @@ class GpuContext:
         if interlaced:
             self._check_mesh(mesh1, n, prec, nb)
         off_d = torch.from_numpy(off).to(self.device)
-        check(self.lib.cnpc_paint_batch(self.h, _ptr(pos), _ptr(w), _ptr(off_d), nb, int(np.diff(off).max()), n,
+        check(self.lib.cnpc_paint_batch(self.h, _ptr(pos), _ptr(w), _ptr(off_d), nb, int(off[-1]),
+                                        int(np.diff(off).max()), n,
                                         float(boxsize), WINDOW[sampler], int(bool(interlaced)), None, prec,
                                         _ptr(mesh0), _ptr(mesh1 if interlaced else None)), "cnpc_paint_batch")
 

@@ -83,14 +83,17 @@ int cnpc_ctx_hold(cnpc_ctx* ctx, int delta);
 int cnpc_paint(cnpc_ctx* ctx, const double* pos_xyz, const double* weight, int64_t np, int nmesh, double boxsize,
                int window, int interlaced, const double* boxcenter, int prec, void* mesh0, void* mesh1);
 
-/* Paint nbatch catalogs in one launch: catalog b is particles
- * [offsets[b], offsets[b+1]) of the concatenated arrays (offsets: device
- * int64, nbatch + 1 entries; max_count >= the largest catalog) and lands in
- * mesh b of the mesh0 (and mesh1) batches, nbatch * nmesh^3 values each.
- * Zero-fills the meshes first.  The multi-mock form of pm_painter
+/* Paint nbatch catalogs at once: catalog b is particles [offsets[b],
+ * offsets[b+1]) of the concatenated arrays (offsets: device int64, nbatch +
+ * 1 entries; total_count = offsets[nbatch], max_count >= the largest
+ * catalog) and lands in mesh b of the mesh0 (and mesh1) batches, nbatch *
+ * nmesh^3 values each.  For nmesh % 8 == 0 every catalog takes the sorted,
+ * LDS-tiled path of cnpc_paint in the same launches (every cell written
+ * once, no global atomics); other meshes are zero-filled and painted by
+ * per-particle atomic scatter.  The multi-mock form of pm_painter
  * (mesh_generator.py:172-224) for runs over many catalogs. */
 int cnpc_paint_batch(cnpc_ctx* ctx, const double* pos_xyz, const double* weight, const int64_t* offsets, int nbatch,
-                     int64_t max_count, int nmesh, double boxsize, int window, int interlaced,
+                     int64_t total_count, int64_t max_count, int nmesh, double boxsize, int window, int interlaced,
                      const double* boxcenter, int prec, void* mesh0, void* mesh1);
 
 /* Unnormalised 3-D real-to-half-spectrum forward FFT and its inverse, batch

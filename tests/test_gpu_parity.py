@@ -66,32 +66,46 @@ def test_paint_wraps_and_empty(gpu_ctx):
 
 
 @pytest.mark.parametrize("sampler", WINDOWS)
-def test_paint_tiled_matches_scatter(gpu_ctx, sampler):
-    """Tiled paint (cnpc_paint at n % 8 == 0) == scatter paint (the batch
-    entry with one catalog) on a dense clump inside one tile, particles on
-    tile and box edges and a uniform background; FP64 to 1e-13 of max, FP32
-    to 5e-5 (summation order of ~3000 weights per cell)."""
+@pytest.mark.parametrize("n", [64, 40, 36])
+def test_paint_tiled_dense_clump_and_batch(gpu_ctx, sampler, n):
+    """Tiled paint (16^3 tiles at n = 64, 8^3 at n = 40; n = 36 takes the
+    atomic scatter paths) on a dense clump
+    inside one tile, particles on tile and box edges and a uniform
+    background, against the oracle's paint: FP64 to 1e-12 of max, FP32 to
+    5e-5 (the LDS additions of ~3000 weights per cell in another order).
+    The batch entry (the same catalog between an empty and a small one, one
+    set of launches) must give the same meshes, and the empty catalog zero
+    meshes."""
     import torch
 
-    n, L = 64, 640.0
+    C, _, _ = _oracle()
+    L = 10.0 * n
     rng = np.random.default_rng(5)
-    clump = 200.0 + rng.random((3000, 3)) * 4.0           # one 8^3 tile (80 Mpc/h)
-    edges = np.array([[0.0, 0.0, 0.0], [639.999, 639.999, 639.999], [80.0, 160.0, 239.99], [79.999, 80.0, 0.0]])
+    clump = 3.0 * n + rng.random((3000, 3)) * 4.0
+    edges = np.array([[0.0, 0.0, 0.0], [L - 1e-3] * 3, [80.0, 160.0, L - 0.01], [79.999, 80.0, 0.0],
+                      [159.999, 159.999, 159.999]])
     pos = np.concatenate([clump, edges, rng.random((20000, 3)) * L])
     w = 0.5 + rng.random(len(pos))
+    small = rng.random((37, 3)) * L
+    ws = 0.5 + rng.random(37)
+    ref0 = C.paint(pos, w, n, L, sampler).real
+    ref1 = C.paint(pos, w, n, L, sampler, shift=0.5).real
+    refs0 = C.paint(small, ws, n, L, sampler).real
+    refs1 = C.paint(small, ws, n, L, sampler, shift=0.5).real
     ctx = gpu_ctx
     pd, wd = torch.from_numpy(pos).cuda(), torch.from_numpy(w).cuda()
+    bpos = torch.from_numpy(np.concatenate([pos, small])).cuda()
+    bw = torch.from_numpy(np.concatenate([w, ws])).cuda()
     for prec in (0, 1):
+        tol = 1e-12 if prec == 0 else 5e-5
         a0, a1 = ctx.mesh(n, prec), ctx.mesh(n, prec)
         ctx.paint(pd, wd, n, L, sampler, True, prec, a0, a1)
-        b0, b1 = ctx.mesh(n, prec, 1), ctx.mesh(n, prec, 1)
-        ctx.paint_batch(pd, wd, [0, len(pos)], n, L, sampler, True, prec, b0, b1)
-        # FP32: the two paths add the ~3000 clump weights of a cell in different
-        # orders (atomics), so they agree to ~N eps, not to eps
-        tol = 1e-13 if prec == 0 else 5e-5
-        for a, b in ((a0, b0[0]), (a1, b1[0])):
-            a, b = a.double().cpu().numpy(), b.double().cpu().numpy()
-            assert np.max(np.abs(a - b)) <= tol * np.max(np.abs(b)), (sampler, prec)
+        b0, b1 = ctx.mesh(n, prec, 3), ctx.mesh(n, prec, 3)
+        ctx.paint_batch(bpos, bw, [0, 0, len(pos), len(pos) + len(small)], n, L, sampler, True, prec, b0, b1)
+        for got, exp in ((a0, ref0), (a1, ref1), (b0[1], ref0), (b1[1], ref1), (b0[2], refs0), (b1[2], refs1)):
+            got = got.double().cpu().numpy()
+            assert np.max(np.abs(got - exp)) <= tol * np.max(np.abs(exp)), (sampler, prec)
+        assert b0[0].abs().max().item() == 0 and b1[0].abs().max().item() == 0
 
 
 def test_fft_roundtrip_and_normalisation(gpu_ctx):
