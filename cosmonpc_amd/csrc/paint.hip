@@ -6,13 +6,13 @@
 // pm.affine.shift(0.5) at mesh_generator.py:201), periodic wrap.
 //
 // Meshes with n % 8 == 0 (16 <= n <= 4096) take the sorted, LDS-tiled
-// owner-computes path below, for one catalog or a batch; the others take the
-// scatter path: one thread per particle paints both the plain and the
-// shifted copy with global atomics, so each particle's 32 bytes are read
-// once.  Cell index and fractional offset always
-// come from the FP64 position (a float32 position in a 2000 Mpc/h box would
-// carry ~3e-5 cell of error); weights are formed in FP64 and rounded to the
-// mesh type for the atomic add.
+// owner-computes path below, for one catalog or a batch (bitwise
+// reproducible: fixed-point tile sums); the others take the scatter path:
+// one thread per particle paints both the plain and the shifted copy with
+// global atomics, so each particle's 32 bytes are read once.  Cell index and
+// fractional offset always come from the FP64 position (a float32 position
+// in a 2000 Mpc/h box would carry ~3e-5 cell of error); weights are formed
+// in FP64.
 #include "cnpc_common.h"
 
 namespace cnpc {
@@ -180,9 +180,10 @@ hipError_t launch_paint_batch(const double* pos, const double* w, const int64_t*
 // contributes classes with the x bit, ...), and writes the tile once with
 // plain stores (also the zero fill).  No global atomics on the mesh, and a
 // particle is read by 1 + (tiles its windows reach) workgroups: about 1.9
-// reads per particle at PT = 16 instead of 8.  The weights are the same
-// FP64 products as paint_one, rounded to the mesh type before the (LDS)
-// addition.
+// reads per particle at PT = 16 instead of 8.  Window weights are formed
+// in FP64 per axis, rounded to the mesh type and multiplied in it; the tile
+// sums them in 64-bit fixed point (below), so the result does not depend on
+// the order of the additions and the mesh is bitwise reproducible.
 constexpr int PAINT_TILE_THREADS = 256;
 
 // first window cell of both copies along one axis, relative to the mesh
@@ -318,8 +319,8 @@ struct PaintSeg {
   int src[14];
 };
 
-__host__ __device__ inline void paint_segments(PaintSeg& g) {
-  g.n = 0;
+__host__ __device__ constexpr PaintSeg paint_segments() {
+  PaintSeg g{};
   for (int s = 0; s < 8; ++s) {
     int c = 0;
     while (c < 8) {
@@ -336,84 +337,202 @@ __host__ __device__ inline void paint_segments(PaintSeg& g) {
       c = e + 1;
     }
   }
+  return g;
 }
 
+constexpr PaintSeg kPaintSeg = paint_segments();
+static_assert(kPaintSeg.n == 14, "27 neighbour/class combinations form 14 runs");
+
+// Lane layout of the accumulation step: KL lanes per particle (one per z
+// offset of the window), PG particles per wave instruction; WS = stride of a
+// particle's staged values (odd, so lane-strided stores hit distinct banks).
+template <int ORDER>
+struct TileLanes {
+  static constexpr int KL = ORDER == 1 ? 1 : ORDER == 2 ? 2 : 4;
+  static constexpr int PG = 64 / KL;
+  static constexpr int WS = (3 * ORDER) | 1;
+};
+
+// The tile sums in 64-bit fixed point: contribution v becomes round(v * 2^S)
+// and the LDS additions are integer (LDS integer atomics run at full rate;
+// floating-point LDS atomics do not), exact and order-independent, so the
+// painted mesh is bitwise reproducible.  Per tile, S = OFF - e with
+// 2^e > (particles read) * max|w| >= any cell's |sum|: FP64 meshes keep
+// OFF = 61 (resolution below 2^-61 of that bound), FP32 meshes OFF = 50,
+// which lets a contribution convert with one add of 1.5 * 2^52.
+template <typename T>
+struct TileFixed;
+template <>
+struct TileFixed<float> {
+  static constexpr int OFF = 50;
+  __device__ static long long cvt(float v) {
+    return __double_as_longlong((double)v + 6755399441055744.0) - 0x4338000000000000LL;
+  }
+};
+template <>
+struct TileFixed<double> {
+  static constexpr int OFF = 61;
+  __device__ static long long cvt(double v) { return __double2ll_rn(v); }
+};
+
+// One workgroup per (catalog, tile); the copies are painted one after the
+// other through one accumulator.  Each wave takes 64 particles at a time:
+// one lane per particle forms the FP64 windows of the copy and stages the
+// x weights times mass times 2^S, the y and z weights (all rounded to T) and
+// the window's position in the tile; then KL lanes per particle, one per z
+// offset, add the ORDER^2 products of the particle's (x, y) cells.
 template <typename T, int ORDER, int PT>
 __global__ void __launch_bounds__(PAINT_TILE_THREADS) paint_tile_kernel(
     const double* __restrict__ spos, const double* __restrict__ sw, const int* __restrict__ start, int n, int nt,
     double scale, double cx, double cy, double cz, int ncopy, T* __restrict__ mesh0, T* __restrict__ mesh1) {
-  constexpr int PT3 = PT * PT * PT;
-  __shared__ T acc[2 * PT3];
+  using L = TileLanes<ORDER>;
+  constexpr int PT3 = PT * PT * PT, NW = PAINT_TILE_THREADS / 64;
+  constexpr int DEAD = 0xFFFFFF;  // staged position of an empty slot: outside every tile
+  __shared__ unsigned long long acc[PT3];
+  __shared__ T stw[NW][64 * L::WS];
+  __shared__ int stp[NW][64];
+  __shared__ int seg_start[14], seg_count[14];
+  __shared__ double wred[NW];
   const int nt3 = nt * nt * nt;
-  const int b = blockIdx.x / nt3, tile = blockIdx.x - b * nt3;
+  // consecutive workgroups land on different XCDs: give each XCD a
+  // contiguous run of tiles, so neighbouring tiles share an L2
+  int bid = blockIdx.x;
+  if (gridDim.x % 8 == 0) bid = (bid & 7) * (gridDim.x >> 3) + (bid >> 3);
+  const int b = bid / nt3, tile = bid - b * nt3;
   const int tz = tile % nt, ty = (tile / nt) % nt, tx = tile / (nt * nt);
   const int x0 = tx * PT, y0 = ty * PT, z0 = tz * PT;
-  for (int i = threadIdx.x; i < ncopy * PT3; i += blockDim.x) acc[i] = (T)0;
-  // the ranges to read, as one virtual range
-  PaintSeg g;
-  paint_segments(g);
-  int sb[14], pre[15];
-  pre[0] = 0;
-  for (int q = 0; q < g.n; ++q) {
-    const int s = g.src[q];
+  for (int i = threadIdx.x; i < PT3; i += blockDim.x) acc[i] = 0ull;
+  // the particle ranges to read (runs of consecutive classes, one source tile each)
+  if (threadIdx.x < 14) {
+    const int q = threadIdx.x;
+    int s = 0, first = 0, len = 0;
+#pragma unroll
+    for (int j = 0; j < 14; ++j)
+      if (j == q) {
+        s = kPaintSeg.src[j];
+        first = kPaintSeg.first[j];
+        len = kPaintSeg.len[j];
+      }
     const int sx = (s & 4) ? (tx == 0 ? nt - 1 : tx - 1) : tx;
     const int sy = (s & 2) ? (ty == 0 ? nt - 1 : ty - 1) : ty;
     const int sz = (s & 1) ? (tz == 0 ? nt - 1 : tz - 1) : tz;
-    const int k0 = (((b * nt + sx) * nt + sy) * nt + sz) * 8 + g.first[q];
-    sb[q] = start[k0];
-    pre[q + 1] = pre[q] + (start[k0 + g.len[q]] - sb[q]);
+    const int k0 = (((b * nt + sx) * nt + sy) * nt + sz) * 8 + first;
+    const int s0 = start[k0];
+    seg_start[q] = s0;
+    seg_count[q] = start[k0 + len] - s0;
   }
   __syncthreads();
-  const int total = pre[g.n];
-  for (int v = threadIdx.x; v < total; v += blockDim.x) {
-    int q = 0;
-    while (v >= pre[q + 1]) ++q;
-    const int64_t p = (int64_t)sb[q] + (v - pre[q]);
-    const double x = spos[3 * p + 0] - cx;
-    const double y = spos[3 * p + 1] - cy;
-    const double z = spos[3 * p + 2] - cz;
-    const double m = sw[p];
-    for (int c = 0; c < ncopy; ++c) {
-      const double sh = 0.5 * c;
-      double wx[ORDER], wy[ORDER], wz[ORDER];
-      int ax = window<ORDER>(__dadd_rn(__dmul_rn(x, scale), sh), wx);
-      int ay = window<ORDER>(__dadd_rn(__dmul_rn(y, scale), sh), wy);
-      int az = window<ORDER>(__dadd_rn(__dmul_rn(z, scale), sh), wz);
-      ax %= n; if (ax < 0) ax += n;
-      ay %= n; if (ay < 0) ay += n;
-      az %= n; if (az < 0) az += n;
-      // offsets of the first window cell from the tile origin, in [0, n)
-      int rx = ax - x0; if (rx < 0) rx += n;
-      int ry = ay - y0; if (ry < 0) ry += n;
-      int rz = az - z0; if (rz < 0) rz += n;
-      T* a = acc + c * PT3;
+  int sst[14], scnt[14], total = 0;
 #pragma unroll
-      for (int i = 0; i < ORDER; ++i) {
-        int ix = rx + i; if (ix >= n) ix -= n;
-        if (ix >= PT) continue;
-        const double mx = m * wx[i];
+  for (int q = 0; q < 14; ++q) {
+    sst[q] = seg_start[q];
+    scnt[q] = seg_count[q];
+    total += scnt[q];
+  }
+  auto particle = [&](int v) -> int64_t {
+    int64_t p = -1;
+    int run = 0;
 #pragma unroll
-        for (int j = 0; j < ORDER; ++j) {
-          int iy = ry + j; if (iy >= n) iy -= n;
-          if (iy >= PT) continue;
-          const double mxy = mx * wy[j];
+    for (int q = 0; q < 14; ++q) {
+      if (v >= run && v < run + scnt[q]) p = (int64_t)sst[q] + (v - run);
+      run += scnt[q];
+    }
+    return p;
+  };
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  // the tile's scale: max |w| over the particles read
+  double wmax = 0.0;
+  for (int v = threadIdx.x; v < total; v += blockDim.x) wmax = fmax(wmax, fabs(sw[particle(v)]));
 #pragma unroll
-          for (int k = 0; k < ORDER; ++k) {
-            int iz = rz + k; if (iz >= n) iz -= n;
-            if (iz >= PT) continue;
-            atomicAdd(a + (ix * PT + iy) * PT + iz, (T)(mxy * wz[k]));
+  for (int o = 32; o > 0; o >>= 1) wmax = fmax(wmax, __shfl_xor(wmax, o));
+  if (lane == 0) wred[wv] = wmax;
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < NW; ++q) wmax = fmax(wmax, wred[q]);
+  double bound = (double)total * wmax;
+  if (!(bound <= 1e300)) bound = 1e300;
+  int e = 0;
+  frexp(bound, &e);
+  const int S = TileFixed<T>::OFF - e;
+  const int kk = lane % L::KL, slot = lane / L::KL;
+  const int kw = kk < ORDER ? kk : ORDER - 1;
+  T* const st = stw[wv];
+  int* const sp = stp[wv];
+  const int64_t n3 = (int64_t)n * n * n;
+  for (int c = 0; c < ncopy; ++c) {
+    const double sh = 0.5 * c;
+    for (int base = wv * 64; base < total; base += NW * 64) {
+      const int64_t p = particle(base + lane);
+      if (p >= 0) {
+        const double x = spos[3 * p + 0] - cx, y = spos[3 * p + 1] - cy, z = spos[3 * p + 2] - cz;
+        const double m = sw[p];
+        double wx[ORDER], wy[ORDER], wz[ORDER];
+        int r[3];
+        r[0] = window<ORDER>(__dadd_rn(__dmul_rn(x, scale), sh), wx);
+        r[1] = window<ORDER>(__dadd_rn(__dmul_rn(y, scale), sh), wy);
+        r[2] = window<ORDER>(__dadd_rn(__dmul_rn(z, scale), sh), wz);
+        const int o[3] = {x0, y0, z0};
+        int pk = 0;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          // first window cell relative to the tile, in [-4, PT]
+          int rr = r[d] % n;
+          if (rr < 0) rr += n;
+          rr -= o[d];
+          if (rr < 0) rr += n;
+          if (rr >= n - 4) rr -= n;
+          pk |= (rr + 4) << (8 * d);
+        }
+        T* w = st + lane * L::WS;
+#pragma unroll
+        for (int i = 0; i < ORDER; ++i) {
+          w[i] = (T)ldexp(m * wx[i], S);
+          w[ORDER + i] = (T)wy[i];
+          w[2 * ORDER + i] = (T)wz[i];
+        }
+        sp[lane] = pk;
+      } else {
+        sp[lane] = DEAD;
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      const int cnt = min(64, total - base);
+      for (int g = 0; g < cnt; g += L::PG) {
+        const int q = g + slot;
+        const int pk = sp[q];
+        const T* w = st + q * L::WS;
+        T mx[ORDER], wy[ORDER];
+#pragma unroll
+        for (int i = 0; i < ORDER; ++i) {
+          mx[i] = w[i];
+          wy[i] = w[ORDER + i];
+        }
+        const T wz = w[2 * ORDER + kw];
+        const int rx = (pk & 255) - 4, ry = ((pk >> 8) & 255) - 4, iz = ((pk >> 16) & 255) - 4 + kk;
+        const bool zok = kk < ORDER && (unsigned)iz < (unsigned)PT;
+        unsigned long long* a = acc + (rx * PT + ry) * PT + iz;
+#pragma unroll
+        for (int i = 0; i < ORDER; ++i) {
+#pragma unroll
+          for (int j = 0; j < ORDER; ++j) {
+            if (zok && (unsigned)(rx + i) < (unsigned)PT && (unsigned)(ry + j) < (unsigned)PT)
+              atomicAdd(a + (i * PT + j) * PT, (unsigned long long)TileFixed<T>::cvt(mx[i] * wy[j] * wz));
           }
         }
       }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
-  }
-  __syncthreads();
-  const int64_t n3 = (int64_t)n * n * n;
-  for (int t = threadIdx.x; t < ncopy * PT3; t += blockDim.x) {
-    const int c = t / PT3, cell = t - c * PT3;
-    const int ix = cell / (PT * PT), iy = (cell / PT) % PT, iz = cell % PT;
+    __syncthreads();
     T* mesh = (c == 0 ? mesh0 : mesh1) + b * n3;
-    mesh[((int64_t)(x0 + ix) * n + (y0 + iy)) * n + z0 + iz] = acc[t];
+    for (int t = threadIdx.x; t < PT3; t += blockDim.x) {
+      const int ix = t / (PT * PT), iy = (t / PT) % PT, iz = t % PT;
+      mesh[((int64_t)(x0 + ix) * n + (y0 + iy)) * n + z0 + iz] = (T)ldexp((double)(long long)acc[t], -S);
+      acc[t] = 0ull;
+    }
+    __syncthreads();
   }
 }
 
