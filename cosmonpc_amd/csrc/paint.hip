@@ -365,15 +365,33 @@ struct TileFixed;
 template <>
 struct TileFixed<float> {
   static constexpr int OFF = 50;
-  __device__ static long long cvt(float v) {
-    return __double_as_longlong((double)v + 6755399441055744.0) - 0x4338000000000000LL;
+  // round(a * b) for a = mx * wz (exact in FP64) and b = wy: one FMA onto
+  // 1.5 * 2^52 leaves the integer in the low mantissa bits
+  __device__ static long long cvt(double a, double b) {
+    return __double_as_longlong(fma(a, b, 6755399441055744.0)) - 0x4338000000000000LL;
   }
 };
 template <>
 struct TileFixed<double> {
   static constexpr int OFF = 61;
-  __device__ static long long cvt(double v) { return __double2ll_rn(v); }
+  __device__ static long long cvt(double a, double b) { return __double2ll_rn(a * b); }
 };
+
+// window<ORDER> with the PCS sixths as a multiplication by 1/6 (within an
+// ulp of the division)
+template <int ORDER>
+__device__ __forceinline__ int window_tile(double u, double* w) {
+  if (ORDER != 4) return window<ORDER>(u, w);
+  constexpr double sixth = 1.0 / 6.0;
+  const double i1 = floor(u);
+  const double f = u - i1;
+  const double g = 1.0 - f;
+  w[0] = g * g * g * sixth;
+  w[1] = (4.0 - 6.0 * f * f + 3.0 * f * f * f) * sixth;
+  w[2] = (4.0 - 6.0 * g * g + 3.0 * g * g * g) * sixth;
+  w[3] = f * f * f * sixth;
+  return (int)i1 - 1;
+}
 
 // One workgroup per (catalog, tile); the copies are painted one after the
 // other through one accumulator.  Each wave takes 64 particles at a time:
@@ -459,18 +477,35 @@ __global__ void __launch_bounds__(PAINT_TILE_THREADS) paint_tile_kernel(
   T* const st = stw[wv];
   int* const sp = stp[wv];
   const int64_t n3 = (int64_t)n * n * n;
+  // the wave's first 64 particles stay in registers for the second copy
+  const int64_t p_first = particle(wv * 64 + lane);
+  double xf = 0.0, yf = 0.0, zf = 0.0, mf = 0.0;
+  if (p_first >= 0) {
+    xf = spos[3 * p_first + 0] - cx;
+    yf = spos[3 * p_first + 1] - cy;
+    zf = spos[3 * p_first + 2] - cz;
+    mf = sw[p_first];
+  }
   for (int c = 0; c < ncopy; ++c) {
     const double sh = 0.5 * c;
     for (int base = wv * 64; base < total; base += NW * 64) {
-      const int64_t p = particle(base + lane);
+      int64_t p = p_first;
+      double x = xf, y = yf, z = zf, m = mf;
+      if (base != wv * 64) {
+        p = particle(base + lane);
+        if (p >= 0) {
+          x = spos[3 * p + 0] - cx;
+          y = spos[3 * p + 1] - cy;
+          z = spos[3 * p + 2] - cz;
+          m = sw[p];
+        }
+      }
       if (p >= 0) {
-        const double x = spos[3 * p + 0] - cx, y = spos[3 * p + 1] - cy, z = spos[3 * p + 2] - cz;
-        const double m = sw[p];
         double wx[ORDER], wy[ORDER], wz[ORDER];
         int r[3];
-        r[0] = window<ORDER>(__dadd_rn(__dmul_rn(x, scale), sh), wx);
-        r[1] = window<ORDER>(__dadd_rn(__dmul_rn(y, scale), sh), wy);
-        r[2] = window<ORDER>(__dadd_rn(__dmul_rn(z, scale), sh), wz);
+        r[0] = window_tile<ORDER>(__dadd_rn(__dmul_rn(x, scale), sh), wx);
+        r[1] = window_tile<ORDER>(__dadd_rn(__dmul_rn(y, scale), sh), wy);
+        r[2] = window_tile<ORDER>(__dadd_rn(__dmul_rn(z, scale), sh), wz);
         const int o[3] = {x0, y0, z0};
         int pk = 0;
 #pragma unroll
@@ -502,13 +537,13 @@ __global__ void __launch_bounds__(PAINT_TILE_THREADS) paint_tile_kernel(
         const int q = g + slot;
         const int pk = sp[q];
         const T* w = st + q * L::WS;
-        T mx[ORDER], wy[ORDER];
+        const double wz = (double)w[2 * ORDER + kw];
+        double mxz[ORDER], wy[ORDER];
 #pragma unroll
         for (int i = 0; i < ORDER; ++i) {
-          mx[i] = w[i];
-          wy[i] = w[ORDER + i];
+          mxz[i] = (double)w[i] * wz;
+          wy[i] = (double)w[ORDER + i];
         }
-        const T wz = w[2 * ORDER + kw];
         const int rx = (pk & 255) - 4, ry = ((pk >> 8) & 255) - 4, iz = ((pk >> 16) & 255) - 4 + kk;
         const bool zok = kk < ORDER && (unsigned)iz < (unsigned)PT;
         unsigned long long* a = acc + (rx * PT + ry) * PT + iz;
@@ -517,7 +552,7 @@ __global__ void __launch_bounds__(PAINT_TILE_THREADS) paint_tile_kernel(
 #pragma unroll
           for (int j = 0; j < ORDER; ++j) {
             if (zok && (unsigned)(rx + i) < (unsigned)PT && (unsigned)(ry + j) < (unsigned)PT)
-              atomicAdd(a + (i * PT + j) * PT, (unsigned long long)TileFixed<T>::cvt(mx[i] * wy[j] * wz));
+              atomicAdd(a + (i * PT + j) * PT, (unsigned long long)TileFixed<T>::cvt(mxz[i], wy[j]));
           }
         }
       }
@@ -526,10 +561,16 @@ __global__ void __launch_bounds__(PAINT_TILE_THREADS) paint_tile_kernel(
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
     __syncthreads();
-    T* mesh = (c == 0 ? mesh0 : mesh1) + b * n3;
-    for (int t = threadIdx.x; t < PT3; t += blockDim.x) {
-      const int ix = t / (PT * PT), iy = (t / PT) % PT, iz = t % PT;
-      mesh[((int64_t)(x0 + ix) * n + (y0 + iy)) * n + z0 + iz] = (T)ldexp((double)(long long)acc[t], -S);
+    // write the tile (XP x planes per pass) and clear the accumulator
+    constexpr int XP = PAINT_TILE_THREADS / (PT * PT);
+    static_assert(XP >= 1 && PT % XP == 0 && XP * PT * PT == PAINT_TILE_THREADS, "whole x planes per pass");
+    const int rem = threadIdx.x % (PT * PT), ix0 = threadIdx.x / (PT * PT);
+    T* mp = (c == 0 ? mesh0 : mesh1) + b * n3 + ((int64_t)(x0 + ix0) * n + y0 + rem / PT) * n + z0 + rem % PT;
+    const int64_t step = (int64_t)XP * n * n;
+#pragma unroll 4
+    for (int ix = ix0; ix < PT; ix += XP, mp += step) {
+      const int t = ix * PT * PT + rem;
+      *mp = (T)ldexp((double)(long long)acc[t], -S);
       acc[t] = 0ull;
     }
     __syncthreads();
