@@ -12,10 +12,12 @@ MFMA contraction with the z pass fused in, to the two 30 x 30 signal
 matrices.  bins/step = 2 x 465.
 
 Also reported (outside the timed steps): the same stage with the real shells
-written to HBM and contracted apart, the plain FP64 pairwise path and the
-FP64 MFMA GEMM path on the same catalog, the split-bf16 error relative to
-max|B| against the FP64 result, per-kernel roofline timings, and the CPU
-baseline (the FP64 numpy oracle on a bounded sample, extrapolated).
+written to HBM and contracted apart, the split-fp16 and split-fp8 variants,
+the plain FP64 pairwise path and the FP64 MFMA GEMM path on the same
+catalog, each split variant's error relative to max|B| against the FP64
+result, per-kernel roofline timings, the batched-mocks comparison (64 box
+mocks at 256^3, batched driver against the per-mock loop) and the CPU
+baseline (the FP64 numpy oracle timed on one full angular configuration).
 """
 import argparse
 import json
@@ -28,6 +30,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+MFMA_BF16_PEAK_TFLOPS = 2500.0
 
 
 def log(msg):
@@ -45,6 +48,7 @@ def parse():
     p.add_argument("--no-fp64", action="store_true", help="skip the FP64 reference paths")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     p.add_argument("--no-graph", action="store_true", help="time the eager launches only (no hipGraph replay)")
+    p.add_argument("--no-mocks", action="store_true", help="skip the batched-mocks comparison of the default run")
     p.add_argument("--roofline-only", action="store_true")
     p.add_argument("--workload", choices=("metric", "mocks", "kernels"), default="metric",
                    help="mocks: 64 box mocks (pk + bk_sugi diagonal at 256^3), batched driver vs per-mock loop; "
@@ -65,6 +69,8 @@ def main():
         return bench_mocks(args)
     if args.workload == "kernels":
         return bench_kernels(args)
+    # the mock catalogs are drawn in worker processes before anything touches the GPU
+    mock_cats = None if args.no_mocks or args.mocks < 1 else mock_catalogs(args.mocks)
     import numpy as np
     import torch
 
@@ -212,6 +218,14 @@ def main():
     torch.cuda.synchronize()
     t_fp8 = time.perf_counter() - t0
     log(f"fp8x6 (separate z pass): {t_fp8 * 1e3:.1f} ms/step -> {bins_per_step / t_fp8:.1f} bins/s")
+    # the FP16 split variant (hi + lo halves, three products; separate z pass)
+    signal_step(d32, 1, "fp16x3")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res_fp16 = signal_step(d32, 1, "fp16x3")
+    torch.cuda.synchronize()
+    t_fp16 = time.perf_counter() - t0
+    log(f"fp16x3 (separate z pass): {t_fp16 * 1e3:.1f} ms/step -> {bins_per_step / t_fp16:.1f} bins/s")
 
     # ---------------------------------------------------------- per-kernel roofline
     m, nmax = contraction_grid(n, L, kmax, "bf16x3")
@@ -221,7 +235,8 @@ def main():
     log(f"kernels (grid {m}^3): {json.dumps(roof['stages'])}")
 
     extra = {**graph_info, "separate_z_ms_per_step": t_sep * 1e3, "separate_z_bins_per_s": bins_per_step / t_sep,
-             "fp8x6_ms_per_step": t_fp8 * 1e3, "fp8x6_bins_per_s": bins_per_step / t_fp8}
+             "fp8x6_ms_per_step": t_fp8 * 1e3, "fp8x6_bins_per_s": bins_per_step / t_fp8,
+             "fp16x3_ms_per_step": t_fp16 * 1e3, "fp16x3_bins_per_s": bins_per_step / t_fp16}
     if not args.no_fp64:
         paint_field(ctx, pos_d, w_d, n, L, sampler, True, 0)
         torch.cuda.synchronize()
@@ -244,14 +259,19 @@ def main():
             log(f"fp64 {mode}: {dt * 1e3:.1f} ms/step -> {bins_per_step / dt:.1f} bins/s")
         ref = res["pairwise"]
         extra["speedup_vs_fp64_plain"] = extra["pairwise_ms_per_step"] / (t_step * 1e3)
-        for a, r64, rb, rs, r8, rg in zip(angus, ref, res_bf16, res_sep, res_fp8, res["gemm_fp64"]):
+        for a, r64, rb, rs, r8, r16, rg in zip(angus, ref, res_bf16, res_sep, res_fp8, res_fp16, res["gemm_fp64"]):
             tag = "".join(map(str, a))
             extra[f"err_bf16x3_B{tag}"] = float(np.max(np.abs(rb - r64)) / np.max(np.abs(r64)))
             extra[f"err_bf16x3_separate_B{tag}"] = float(np.max(np.abs(rs - r64)) / np.max(np.abs(r64)))
             extra[f"err_fp8x6_B{tag}"] = float(np.max(np.abs(r8 - r64)) / np.max(np.abs(r64)))
+            extra[f"err_fp16x3_B{tag}"] = float(np.max(np.abs(r16 - r64)) / np.max(np.abs(r64)))
             extra[f"err_gemm_fp64_B{tag}"] = float(np.max(np.abs(rg - r64)) / np.max(np.abs(r64)))
         del d64
-    cpu = None if args.no_cpu else cpu_baseline(n, L, k_edge, npairs)
+    del d32
+    if mock_cats is not None:
+        extra["mocks"] = mock_comparison(args, mock_cats)
+        del mock_cats
+    cpu = None if args.no_cpu else cpu_baseline(n, L, k_edge)
     line = {
         "metric": "bk_sugi full-mode bins/s @512^3, split-bf16 MFMA vs FP64 plain (<=1e-4 rel)",
         "value": value,
@@ -373,11 +393,33 @@ def kernel_roofline(ctx, delta, n, L, k_edge, torch, np, A):
     for v in stages.values():
         v["GBps"] = v["bytes"] / v["ms_per_launch"] / 1e6
     # the measured step runs 15 synthesis launches (both weights) and one
-    # two-output contraction
+    # two-output contraction.  Algorithmic FLOPs of a contraction: 2 K^2 n^3
+    # per output matrix (the G and Ylm products are counted as bytes only)
+    stages["zgemm_bf16x3"]["flops"] = 2.0 * K * K * N3
+    stages["zgemm_bf16x3_bb"]["flops"] = 4.0 * K * K * N3
+    stages["gemm_x3_bf16"]["flops"] = 2.0 * K * K * N3
     dom = max(("shell_synth_half_batch4", "zgemm_bf16x3_bb"), key=lambda k: stages[k]["total_ms"])
-    ach = stages[dom]["GBps"]
-    roof = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
-            "traffic": measured_traffic(dom), "kernel": dom, "algorithmic_bytes_per_launch": stages[dom]["bytes"]}
+    st = stages[dom]
+    ach = st["GBps"]
+    # the bounding roof: the larger of the two lower bounds on the launch
+    # time (bytes at the HBM peak, FLOPs at the dense bf16 MFMA peak)
+    t_hbm = st["bytes"] / (HBM_PEAK_GBS * 1e9)
+    t_mfma = st.get("flops", 0.0) / (MFMA_BF16_PEAK_TFLOPS * 1e12)
+    tflops = st.get("flops", 0.0) / (st["ms_per_launch"] * 1e-3) / 1e12
+    if t_mfma > t_hbm:
+        roof = {"bound": "mfma", "achieved": tflops, "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": tflops / MFMA_BF16_PEAK_TFLOPS}
+    else:
+        roof = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS}
+    roof.update({"traffic": measured_traffic(dom), "kernel": dom, "algorithmic_bytes_per_launch": st["bytes"],
+                 "traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE of this kernel (profiles/"
+                                   "kernels_512_hbm_traffic_pmc.txt), not measured in this run"})
+    if "flops" in st:
+        roof["mfma"] = {"algorithmic_flops_per_launch": st["flops"], "achieved_tflops": tflops,
+                        "peak_tflops_bf16_dense": MFMA_BF16_PEAK_TFLOPS, "frac": tflops / MFMA_BF16_PEAK_TFLOPS}
+    util = measured_utilisation(dom)
+    if util:
+        roof["measured_utilisation"] = dict(util, source="rocprofv3 PMC (profiles/kernels_512_pmc_utilisation.txt)")
     del work, shells, G, H
     return {"roofline": roof, "stages": stages}
 
@@ -401,13 +443,39 @@ def measured_traffic(kernel):
     return None
 
 
-def cpu_baseline(n, L, k_edge, npairs):
-    """FP64 numpy oracle on a bounded sample of the same signal stage: one
-    Ylm-weighted shell inverse FFT (weights, mask and c2c IFFT on the full
-    complex128 grid, as the oracle and the reference do) and one
-    triple-product sum, extrapolated to (K + 1) transforms and the pair sums
-    of one angular configuration.  It uses the host's cores (at most 16):
-    the elementwise work in x-slabs on a thread pool, the 3-D IFFT with
+def measured_utilisation(stage):
+    """Busy fractions of ``stage``'s kernel from the rocprofv3 PMC runs
+    summarised in profiles/ (None if absent)."""
+    path = os.path.join(ROOT, "profiles", "kernels_512_pmc_utilisation.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        parts = line.split()
+        if len(parts) >= 4 and parts[0] == "utilisation" and parts[1] == stage:
+            return {parts[i]: float(parts[i + 1]) for i in range(2, len(parts) - 1, 2)}
+    return None
+
+
+def host_info():
+    model = platform.processor() or platform.machine()
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"cpu_model": model, "os_cpu_count": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0))}
+
+
+def cpu_baseline(n, L, k_edge):
+    """The FP64 numpy oracle timed on one full angular configuration of the
+    workload, [0,0,0]: the K shell inverse FFTs (mask and c2c IFFT on the
+    full complex128 grid, real parts kept in host memory: K n^3 doubles, the
+    whole block at once), the G_00 transform, and the K (K + 1) / 2
+    triple-product sums sum_x G S_i S_j.  Threads: the host's CPUs available
+    to this process, at most 16 (the CPU share of one GPU): the elementwise
+    work and the sums in x-slabs on a thread pool, the 3-D IFFTs with
     scipy's multi-threaded pocketfft.  The timed operations do not depend on
     the field values, so the input spectrum is seeded noise."""
     from concurrent.futures import ThreadPoolExecutor
@@ -418,40 +486,70 @@ def cpu_baseline(n, L, k_edge, npairs):
     from oracle.cnpc_ref import angular as OA
     from oracle.cnpc_ref import conventions as C
 
-    cores = max(1, min(16, os.cpu_count() or 1))
+    cores = max(1, min(16, len(os.sched_getaffinity(0))))
+    K = len(k_edge) - 1
     rng = np.random.default_rng(0)
     d = rng.standard_normal((n, n, n)) + 1j * rng.standard_normal((n, n, n))
     kx, ky, kz = C.k_axes(n, L)
-    ylm = OA.get_Ylm(2, 0, True)
-    slabs = [(i, min(n, i + max(1, n // cores))) for i in range(0, n, max(1, n // cores))]
+    step = max(1, -(-n // cores))
+    slabs = [(i, min(n, i + step)) for i in range(0, n, step)]
+    kn = np.empty((n, n, n))
+
+    def knorm(sl):
+        kn[sl[0]:sl[1]] = OA.get_kgrid((kx[sl[0]:sl[1]], ky, kz))[1]
+
+    S = np.empty((K, n, n, n))
     w = np.empty_like(d)
-
-    def weight(sl):
-        i0, i1 = sl
-        kg, kn = OA.get_kgrid((kx[i0:i1], ky, kz))
-        m = np.logical_and(kn >= k_edge[10], kn < k_edge[11])
-        w[i0:i1] = d[i0:i1] * ylm(*kg) * m
-
     with ThreadPoolExecutor(cores) as ex:
+        list(ex.map(knorm, slabs))
         t0 = time.perf_counter()
-        list(ex.map(weight, slabs))
-        F = scipy.fft.ifftn(w, workers=cores) * w.size
-        t_shell = time.perf_counter() - t0
-        del d, w
-        G = np.ascontiguousarray(np.real(F))
+        for i in range(K):
+            def shell(sl, i=i):
+                a, b = sl
+                w[a:b] = d[a:b] * np.logical_and(kn[a:b] >= k_edge[i], kn[a:b] < k_edge[i + 1])
+            list(ex.map(shell, slabs))
+            F = scipy.fft.ifftn(w, workers=cores, overwrite_x=False)
+
+            def keep(sl, i=i):
+                S[i, sl[0]:sl[1]] = F[sl[0]:sl[1]].real
+            list(ex.map(keep, slabs))
+            del F
+            if i == 0 or (i + 1) % 10 == 0:
+                log(f"cpu oracle: {i + 1}/{K} shell IFFTs, {time.perf_counter() - t0:.1f}s")
+        F = scipy.fft.ifftn(d, workers=cores)
+        G = np.empty((n, n, n))
+
+        def keep_g(sl):
+            G[sl[0]:sl[1]] = F[sl[0]:sl[1]].real
+        list(ex.map(keep_g, slabs))
+        del F
+        t_shells = time.perf_counter() - t0
+        del w, d, kn
         t0 = time.perf_counter()
-        s = sum(ex.map(lambda sl: np.sum(G[sl[0]:sl[1]] * F[sl[0]:sl[1]] * F[sl[0]:sl[1]]), slabs))
-        t_pair = time.perf_counter() - t0
-    K = len(k_edge) - 1
-    t_cfg = (K + 1) * t_shell + npairs * t_pair
+        B = np.zeros((K, K))
+        GS = np.empty((n, n, n))
+        for i in range(K):
+            def gs(sl, i=i):
+                np.multiply(G[sl[0]:sl[1]], S[i, sl[0]:sl[1]], out=GS[sl[0]:sl[1]])
+            list(ex.map(gs, slabs))
+            for j in range(i, K):
+                B[i, j] = sum(ex.map(lambda sl, j=j: np.dot(GS[sl[0]:sl[1]].ravel(), S[j, sl[0]:sl[1]].ravel()),
+                                     slabs))
+            if (i + 1) % 10 == 0:
+                log(f"cpu oracle: triple sums of {i + 1}/{K} rows, {time.perf_counter() - t0:.1f}s")
+        t_pairs = time.perf_counter() - t0
+    del S, G, GS
+    npairs = K * (K + 1) // 2
+    t_cfg = t_shells + t_pairs
     bins_per_s = npairs / t_cfg
-    log(f"cpu oracle ({cores} threads): shell {t_shell:.1f}s pair {t_pair:.2f}s -> {bins_per_s:.4f} bins/s "
-        f"(check {abs(s):.3e})")
+    log(f"cpu oracle ({cores} threads): {K} shell IFFTs + G {t_shells:.1f}s, {npairs} triple sums {t_pairs:.1f}s "
+        f"-> {bins_per_s:.3f} bins/s (check {abs(B[0, 0]):.3e})")
     return {"value": bins_per_s, "unit": "bins/s", "cores": cores, "kind": "port",
-            "sample": f"one Ylm-weighted shell IFFT ({t_shell:.1f}s) and one triple sum ({t_pair:.2f}s) of the "
-                      f"{n}^3 complex128 numpy oracle ({cores} threads: x-slab thread pool, scipy.fft workers), "
-                      f"extrapolated to {K + 1} IFFTs + {npairs} pair sums per angular configuration; host "
-                      f"{platform.processor() or platform.machine()}"}
+            "sample": f"one full angular configuration ([0,0,0]) of the {n}^3 workload timed end to end with the "
+                      f"complex128 numpy oracle on {cores} threads (x-slab thread pool, scipy.fft workers): "
+                      f"{K} shell IFFTs + the G_00 transform in {t_shells:.1f}s, {npairs} triple sums in "
+                      f"{t_pairs:.1f}s; [2,0,2] would add {K} Ylm-weighted shell IFFTs",
+            "host": host_info()}
 
 
 def bench_kernels(args):
@@ -553,31 +651,37 @@ def _mock_catalog(i):
     return synth.lognormal_box(npart, boxsize=1000.0, ngrid=256, seed=5000 + i)
 
 
-def bench_mocks(args):
-    """Batched mocks: ``args.mocks`` box catalogs, pk poles [0,2,4] and
-    bk_sugi [0,0,0] diagonal (19 bins 0.01-0.20) at 256^3 CIC interlaced,
-    FP64.  One step = every mock through both tasks with the batched driver
+def mock_catalogs(nmocks):
+    """The mock catalogs, drawn in worker processes (call before the GPU is
+    touched: the workers are forked)."""
+    from concurrent.futures import ProcessPoolExecutor
+
+    t = time.time()
+    with ProcessPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
+        cats = list(ex.map(_mock_catalog, range(nmocks)))
+    log(f"{nmocks} mock catalogs ({sum(len(c) for c in cats)} particles) in {time.time() - t:.1f}s")
+    return cats
+
+
+def mock_comparison(args, cats):
+    """Batched mocks: the box catalogs ``cats``, pk poles [0,2,4] and bk_sugi
+    [0,0,0] diagonal (19 bins 0.01-0.20) at 256^3 CIC interlaced, FP64.  One
+    step = every mock through both tasks with the batched driver
     (``run_mocks``: one paint launch and one batched r2c per chunk); the
     reference's way, one ``run_stats`` per mock and task, is timed beside it
     on ``args.loop_mocks`` mocks.  Outputs go to a temporary directory."""
     import shutil
     import tempfile
-    from concurrent.futures import ProcessPoolExecutor
 
     import numpy as np
-
-    t = time.time()
-    # catalogs are drawn in worker processes before this process touches the GPU
-    with ProcessPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
-        cats = list(ex.map(_mock_catalog, range(args.mocks)))
-    log(f"{args.mocks} mock catalogs ({sum(len(c) for c in cats)} particles) in {time.time() - t:.1f}s")
     import torch
 
     from cosmonpc_amd import run_stats
     from cosmonpc_amd.mocks import LAST_TIMINGS, run_mocks
 
+    nm = len(cats)
     tmp = tempfile.mkdtemp(prefix="cnpc_mocks_")
-    paths = [os.path.join(tmp, "mocks", f"mock_{i:03d}.npy") for i in range(args.mocks)]
+    paths = [os.path.join(tmp, "mocks", f"mock_{i:03d}.npy") for i in range(nm)]
     arrays = {p: (c, np.ones(len(c))) for p, c in zip(paths, cats)}
     base = dict(correlation_mode="auto", geometry="box-like", column_names=["x", "y", "z"], rsd=[0, 0, 1],
                 apply_rsd=False, nmesh=[256] * 3, boxsize=[1000.0] * 3, sampler="cic", interlaced=True,
@@ -607,7 +711,7 @@ def bench_mocks(args):
         torch.cuda.synchronize()
         t_b = time.perf_counter() - t0
         timings = dict(LAST_TIMINGS)
-        nloop = min(args.loop_mocks, args.mocks)
+        nloop = min(args.loop_mocks, nm)
         t0 = time.perf_counter()
         loop(os.path.join(tmp, "loop"), paths[:nloop])
         torch.cuda.synchronize()
@@ -625,20 +729,29 @@ def bench_mocks(args):
     finally:
         logging.disable(logging.NOTSET)
         shutil.rmtree(tmp, ignore_errors=True)
-    value = args.mocks / t_b
+    value = nm / t_b
     loop_rate = nloop / t_l
-    log(f"batched: {t_b:.2f}s for {args.mocks} mocks -> {value:.2f} mocks/s ({timings}); "
+    log(f"mocks batched: {t_b:.2f}s for {nm} mocks -> {value:.2f} mocks/s ({timings}); "
         f"per-mock loop: {t_l:.2f}s for {nloop} -> {loop_rate:.2f} mocks/s; max rel diff {worst:.2e}")
+    return {"value": value, "unit": "mocks/s", "mocks": nm, "seconds": t_b, "mock_batch": args.mock_batch,
+            "workload": f"{nm} box mocks 256^3 cic interlaced, box 1000, N ~ U[5e5, 2e6], pk [0,2,4] 30 bins + "
+                        "bk_sugi [0,0,0] diagonal 19 bins, FP64",
+            "per_mock_loop": {"value": loop_rate, "unit": "mocks/s", "mocks": nloop},
+            "speedup_vs_loop": value / loop_rate, "max_rel_diff_vs_loop": worst, "stages_s": timings}
+
+
+def bench_mocks(args):
+    """The batched-mocks comparison on its own, as one JSON line."""
+    cats = mock_catalogs(args.mocks)
+    r = mock_comparison(args, cats)
     line = {
         "metric": "batched box mocks/s (pk [0,2,4] + bk_sugi [0,0,0] diagonal per mock)",
-        "value": value, "unit": "mocks/s", "n_gpus": 1, "steps": 1, "warmup": 1,
-        "ms_per_step": t_b * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "value": r["value"], "unit": "mocks/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": r["seconds"] * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic (seeded lognormal + linear RSD catalogs, one seed per mock)",
-        "config": {"workload": f"{args.mocks} box mocks 256^3 cic interlaced, box 1000, N ~ U[5e5, 2e6], "
-                               "pk 30 bins + bk_sugi [0,0,0] diagonal 19 bins", "mock_batch": args.mock_batch},
-        "per_mock_loop": {"value": loop_rate, "unit": "mocks/s", "mocks": nloop},
-        "speedup_vs_loop": value / loop_rate, "max_rel_diff_vs_loop": worst,
-        "stages_s": timings,
+        "config": {"workload": r["workload"], "mock_batch": args.mock_batch},
+        "per_mock_loop": r["per_mock_loop"], "speedup_vs_loop": r["speedup_vs_loop"],
+        "max_rel_diff_vs_loop": r["max_rel_diff_vs_loop"], "stages_s": r["stages_s"],
     }
     print(json.dumps(line), flush=True)
 
