@@ -193,7 +193,65 @@ template <int N1, int N2> struct CompositeDft {
   }
 };
 template <> struct SmallDft<16> : CompositeDft<4, 4> {};
-template <> struct SmallDft<20> : CompositeDft<4, 5> {};
+
+// u i^m (m a compile-time constant after unrolling)
+template <typename C>
+__device__ __forceinline__ C mul_ipow(C u, int m) {
+  C r;
+  switch (m & 3) {
+    case 0: r = u; break;
+    case 1: r.x = -u.y; r.y = u.x; break;
+    case 2: r.x = -u.x; r.y = -u.y; break;
+    default: r.x = u.y; r.y = -u.x; break;
+  }
+  return r;
+}
+
+// 20-point DFT by the prime-factor (Good-Thomas) algorithm, 20 = 4 x 5 with
+// no twiddles: input q = (5 n1 + 4 n2) mod 20, output k = (5 k1 + 16 k2)
+// mod 20 (k1 = k mod 4, k2 = k mod 5), X[k] = sum_n2 w5^{n2 k2} sum_n1
+// w4^{n1 k1} u[q].  ZMID: the inputs q = 5 .. 14 are zero, and each 4-point
+// DFT has two nonzero inputs (sums of two terms times powers of i).
+template <bool ZMID>
+struct Pfa20 {
+  template <typename C> __device__ __forceinline__ static void run(C (&u)[20]) {
+    C y[5][4];
+#pragma unroll
+    for (int n2 = 0; n2 < 5; ++n2) {
+      if (!ZMID) {
+        C a[4];
+#pragma unroll
+        for (int n1 = 0; n1 < 4; ++n1) a[n1] = u[(5 * n1 + 4 * n2) % 20];
+        SmallDft<4>::run(a);
+#pragma unroll
+        for (int k1 = 0; k1 < 4; ++k1) y[n2][k1] = a[k1];
+      } else {
+        int p0 = -1, p1 = -1;
+#pragma unroll
+        for (int n1 = 0; n1 < 4; ++n1) {
+          const int q = (5 * n1 + 4 * n2) % 20;
+          if (q < 5 || q > 14) {
+            if (p0 < 0) p0 = n1;
+            else p1 = n1;
+          }
+        }
+        const C a0 = u[(5 * p0 + 4 * n2) % 20], a1 = u[(5 * p1 + 4 * n2) % 20];
+#pragma unroll
+        for (int k1 = 0; k1 < 4; ++k1) y[n2][k1] = cadd(mul_ipow(a0, p0 * k1), mul_ipow(a1, p1 * k1));
+      }
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+      C b[5];
+#pragma unroll
+      for (int n2 = 0; n2 < 5; ++n2) b[n2] = y[n2][k1];
+      SmallDft<5>::run(b);
+#pragma unroll
+      for (int k2 = 0; k2 < 5; ++k2) u[(5 * k1 + 16 * k2) % 20] = b[k2];
+    }
+  }
+};
+template <> struct SmallDft<20> : Pfa20<false> {};
 template <> struct SmallDft<25> : CompositeDft<5, 5> {};
 
 // Twiddles of every pass in pass order, pass s (radix R, stride p) at offset
@@ -372,8 +430,9 @@ __device__ __forceinline__ void fixed_twiddles(C* __restrict__ T1, const C* __re
 
 // Both passes over nl lines of pitch lsc (complex); the caller synchronises
 // before (inputs written) and after (outputs read).  qz > 0: the modes qz ..
-// n - qz are zero and not read (whatever the positions hold); qz <= 0: every
-// mode is read.  Thread t takes line t % nl of butterfly t / nl: lanes of a
+// n - qz are zero and not read (whatever the positions hold; for 20 x 20
+// plans with qz <= n / 4 the first pass's 20-point DFTs also skip their
+// arithmetic); qz <= 0: every mode is read.  Thread t takes line t % nl of butterfly t / nl: lanes of a
 // wave walk the lines first, whose pitch (lsc = 6 mod 32 for the fused
 // contraction's lines) spreads them over the LDS banks.
 template <int N1, int N2, int THREADS, typename C>
@@ -385,17 +444,40 @@ __device__ __forceinline__ void fixed_fft_inplace(C* __restrict__ buf, const C* 
     const int k2 = fdiv(t, nl, inv_nl), l = t - k2 * nl;
     C* col = buf + l * lsc + k2;
     C u[N1];
+    bool pruned = false;
+    if constexpr (N1 == 20) {
+      if (qz > 0 && 4 * qz <= n) {
+        // modes k2 + N2 k1 with k1 = 5 .. 14 lie in [qz, n - qz]: zero, not read
+        pruned = true;
 #pragma unroll
-    for (int k1 = 0; k1 < N1; ++k1) {
-      const int k = k2 + N2 * k1;
-      if (qz > 0 && k >= qz && k <= n - qz) {
-        u[k1].x = 0;
-        u[k1].y = 0;
-      } else {
-        u[k1] = col[RW * k1];
+        for (int k1 = 0; k1 < N1; ++k1) {
+          const int k = k2 + N2 * k1;
+          if (k1 >= 5 && k1 <= 14) {
+            u[k1].x = 0;
+            u[k1].y = 0;
+          } else if (k >= qz && k <= n - qz) {
+            u[k1].x = 0;
+            u[k1].y = 0;
+          } else {
+            u[k1] = col[RW * k1];
+          }
+        }
+        Pfa20<true>::run(u);
       }
     }
-    SmallDft<N1>::run(u);
+    if (!pruned) {
+#pragma unroll
+      for (int k1 = 0; k1 < N1; ++k1) {
+        const int k = k2 + N2 * k1;
+        if (qz > 0 && k >= qz && k <= n - qz) {
+          u[k1].x = 0;
+          u[k1].y = 0;
+        } else {
+          u[k1] = col[RW * k1];
+        }
+      }
+      SmallDft<N1>::run(u);
+    }
     col[0] = u[0];
 #pragma unroll
     for (int z1 = 1; z1 < N1; ++z1) col[RW * z1] = cmul(u[z1], T1[z1 * N2 + k2]);

@@ -346,7 +346,7 @@ __global__ void __launch_bounds__(THREADS) xfft_fixed_kernel(SynthArgs a, const 
       return v;
     });
     __syncthreads();
-    fixed_fft_inplace<N1, N2, THREADS>(buf, T1, nl, LSC);
+    fixed_fft_inplace<N1, N2, THREADS>(buf, T1, nl, LSC, R + 1);
     __syncthreads();
     // U[s][x][kz][ky]: consecutive threads take consecutive ky of one x
     float2* dst = U + ((int64_t)s * n * Q + kz) * P + ky0;
@@ -382,7 +382,7 @@ __global__ void __launch_bounds__(THREADS) yfft_fixed_kernel(SynthArgs a, const 
     return (f >= -rs && f <= rs && kz0 + l <= rs) ? src[l * P + f + R] : make_float2(0.f, 0.f);
   });
   __syncthreads();
-  fixed_fft_inplace<N1, N2, THREADS>(buf, T1, nl, LSC);
+  fixed_fft_inplace<N1, N2, THREADS>(buf, T1, nl, LSC, rs + 1);
   __syncthreads();
   // each y pair writes 2 nl consecutive entries: thread t -> pair t / (2 NL),
   // entry (l, y % 2) = t % (2 NL)
