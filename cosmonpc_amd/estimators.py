@@ -212,13 +212,33 @@ def _zgemm_blocks(ctx, zargs, with_bb=False):
     return 0
 
 
+class _Landing:
+    """A device result copied to pinned host memory right behind the kernels
+    that produce it (the copy and an event follow them on the stream), so
+    reading it waits for this result only and not for device work enqueued
+    after it: the next step's launches can be queued before this one is
+    read."""
+
+    def __init__(self, t):
+        self.host = torch.empty(t.shape, dtype=t.dtype, pin_memory=True)
+        self.host.copy_(t, non_blocking=True)
+        self.event = torch.cuda.Event()
+        self.event.record()
+
+    def get(self):
+        self.event.synchronize()
+        return self.host.numpy()
+
+
 class _ZRequest:
     """A fused contraction asked for by a configuration plan; ``value`` is
-    its device result once the batch has run."""
+    its device result once the batch has run, ``landed`` its host copy in
+    flight (None while a graph records: the reader then copies)."""
 
     def __init__(self, zargs):
         self.zargs = zargs
         self.value = None
+        self.landed = None
 
 
 class ZBatch:
@@ -253,6 +273,10 @@ class ZBatch:
         for r in self.requests:
             if r.value is None:
                 r.value = self.ctx.contract_zgemm(*r.zargs, nblk=_zgemm_blocks(self.ctx, r.zargs))
+        if not _RECORDING[0]:
+            for r in self.requests:
+                if torch.is_tensor(r.value) and r.value.is_cuda:
+                    r.landed = _Landing(r.value)
         self.requests = []
 
 
@@ -418,7 +442,7 @@ def _config_plan(ctx, builder, src1, src2, Gs, kinfo, ell_1, m1, ell_2, m2, sym_
             C = 0j
             for Cm, gph in parts:
                 if isinstance(Cm, _ZRequest):
-                    Cm = Cm.value
+                    Cm = Cm.landed.get() if Cm.landed is not None else Cm.value
                 C = C + gph * (Cm.cpu().numpy() if torch.is_tensor(Cm) else Cm)
             for (i, j) in want:
                 s = 0j
