@@ -366,8 +366,10 @@ __global__ void __launch_bounds__(THREADS) yfft_fixed_kernel(SynthArgs a, const 
   typedef FixedPlan<N1, N2> F;
   constexpr int n = F::n, LSC = SynthFixed<N1, N2>::LSC;
   const int R = a.R, P = 2 * R + 1, Q = R + 1;
-  const int64_t plane = blockIdx.x;
-  const int kz0 = blockIdx.y * NL;
+  // grid (kz blocks, planes): the kz blocks of a plane, which write
+  // adjacent segments of H, are dispatched together
+  const int64_t plane = blockIdx.y;
+  const int kz0 = blockIdx.x * NL;
   const int nl = min(NL, Q - kz0);
   const int s = (int)(plane / n), x = (int)(plane - (int64_t)s * n);
   // kz lines beyond the shell's radius are zero: not computed, not written
@@ -433,6 +435,9 @@ int synth_lines_per_block(int prec, const SynthArgs& a) {
 #ifndef CNPC_SYNTH_FIXED_NL
 #define CNPC_SYNTH_FIXED_NL 8
 #endif
+#ifndef CNPC_SYNTH_FIXED_THREADS
+#define CNPC_SYNTH_FIXED_THREADS 256
+#endif
 
 template <typename T>
 static hipError_t launch_synth_t(const SynthArgs& a, const BinEdges& E, const PolySpec& Wp, const void* D, void* V,
@@ -488,12 +493,12 @@ hipError_t launch_synth_half(const SynthArgs& a, const BinEdges& E, const PolySp
   const int64_t planes = (int64_t)a.nshell * n;
   const dim3 fblk(SYNTH_THREADS);
   if (n == 400) {  // fixed plan 400 = 20 x 20
-    constexpr int NL = CNPC_SYNTH_FIXED_NL;
+    constexpr int NL = CNPC_SYNTH_FIXED_NL, FT = CNPC_SYNTH_FIXED_THREADS;
     const size_t lds = sizeof(C) * ((size_t)n + (size_t)NL * SynthFixed<20, 20>::LSC);
-    hipLaunchKernelGGL((xfft_fixed_kernel<20, 20, SYNTH_THREADS, NL>), dim3((P + NL - 1) / NL, Q), fblk, lds, st, a,
+    hipLaunchKernelGGL((xfft_fixed_kernel<20, 20, FT, NL>), dim3((P + NL - 1) / NL, Q), dim3(FT), lds, st, a,
                        (const C*)V, bins, (const C*)tw, (C*)work);
-    hipLaunchKernelGGL((yfft_fixed_kernel<20, 20, SYNTH_THREADS, NL>), dim3((unsigned)planes, (Q + NL - 1) / NL),
-                       fblk, lds, st, a, (const C*)work, (const C*)tw, (C*)out, nslots, slot0);
+    hipLaunchKernelGGL((yfft_fixed_kernel<20, 20, FT, NL>), dim3((Q + NL - 1) / NL, (unsigned)planes), dim3(FT),
+                       lds, st, a, (const C*)work, (const C*)tw, (C*)out, nslots, slot0);
     return hipGetLastError();
   }
   const int NL = synth_lines_per_block(1, a);
