@@ -81,8 +81,25 @@ SIGNATURES = {
 _lib = None
 
 
+HIP_ERROR_OUT_OF_MEMORY = 2  # hipErrorOutOfMemory
+
+
 class CnpcError(RuntimeError):
-    pass
+    """A failed library call; ``status`` is the call's nonzero return code
+    (a hipError_t value)."""
+
+    def __init__(self, msg, status=None):
+        super().__init__(msg)
+        self.status = status
+
+
+def is_out_of_memory(exc):
+    """True for a device allocation failure: a library call that returned
+    hipErrorOutOfMemory, or torch's out-of-memory error."""
+    if isinstance(exc, CnpcError):
+        return exc.status == HIP_ERROR_OUT_OF_MEMORY
+    oom = getattr(__import__("torch"), "OutOfMemoryError", None)
+    return oom is not None and isinstance(exc, oom)
 
 
 def load(path=LIB_PATH):
@@ -108,4 +125,4 @@ def load(path=LIB_PATH):
 def check(rc, what):
     if rc != 0:
         msg = _lib.cnpc_last_error().decode(errors="replace") if _lib is not None else ""
-        raise CnpcError(f"{what} failed (status {rc}): {msg}")
+        raise CnpcError(f"{what} failed (status {rc}): {msg}", rc)

@@ -23,6 +23,7 @@ import numpy as np
 import torch
 
 from . import angular as A
+from ._lib import is_out_of_memory
 from .device import PREC, get_context
 
 LAST_TIMINGS = {}
@@ -319,6 +320,57 @@ def _config_matrix(*args, **kwargs):
     return _config_plan(*args, **kwargs)()
 
 
+def _config_blocks(ctx, shells, Gs, kinfo, src1, src2, comps1, comps2, use_sym, dvm, bs, fused, kmax, mode, prec,
+                   zbatch):
+    """The blocks of rows and columns of ``_config_plan`` at block size
+    ``bs``: (ib, bi, bj, rows2, want, [(result, phase)]) each, their device
+    work enqueued."""
+    n, L, _, _, k_bins = kinfo
+    nc1, nc2 = len(comps1), len(comps2)
+    blocks = []  # (ib, bi, bj, rows2, want, [(result, phase)]) per block of rows and columns
+    for bi in range(0, k_bins, bs):
+        ib = list(range(bi, min(k_bins, bi + bs)))
+        R1 = shells(src1, comps1, ib)
+        jstarts = [bi] if dvm == "diagonal" else list(range(bi if use_sym else 0, k_bins, bs))
+        for bj in jstarts:
+            jb = list(range(bj, min(k_bins, bj + bs)))
+            if bj == bi and use_sym:
+                R2 = None
+            else:
+                # off the diagonal the symmetric case rebuilds leg 1's shells
+                R2 = shells(src1, comps1, jb) if use_sym else shells(src2, comps2, jb)
+            # pairs this block contributes (reference loop order)
+            want = []
+            for i in ib:
+                if dvm == "diagonal":
+                    js = [i]
+                elif use_sym and bj == bi:
+                    js = [j for j in jb if j >= i]
+                else:
+                    js = jb
+                for j in js:
+                    want.append((i, j))
+            rows2 = len(ib) if R2 is None else len(jb)
+            pairs = [(c1 * len(ib) + (i - bi), c2 * rows2 + (j - bj)) for (i, j) in want
+                     for c1 in range(nc1) for c2 in range(nc2)]
+            parts = []
+            for G, gph in Gs:
+                if fused:
+                    zargs = (n, L, kmax, G, R1[0], nc1 * len(ib), R1[1], None if R2 is None else R2[0],
+                             nc2 * rows2, None if R2 is None else R2[1])
+                    # device result, read back by finish()
+                    if zbatch is not None:
+                        parts.append((zbatch.request(zargs), gph))
+                    else:
+                        parts.append((ctx.contract_zgemm(*zargs, nblk=_zgemm_blocks(ctx, zargs)), gph))
+                else:
+                    parts.append((_contract(ctx, mode, prec, G, R1, R2, pairs), gph))
+            blocks.append((ib, bi, bj, rows2, want, parts))
+            del R2
+        del R1
+    return blocks
+
+
 def _config_plan(ctx, builder, src1, src2, Gs, kinfo, ell_1, m1, ell_2, m2, sym_eq, sym_cj, mode, block_size,
                  dvm, conj_w=False, zsynth="fused", shell_cache=None, zbatch=None):
     """3j-free signal matrix of one magnetic configuration,
@@ -394,47 +446,24 @@ def _config_plan(ctx, builder, src1, src2, Gs, kinfo, ell_1, m1, ell_2, m2, sym_
             builder.build(src, poly, bins, out[c * len(bins):(c + 1) * len(bins)])
         return out
 
-    blocks = []  # (ib, bi, bj, rows2, want, [(result, phase)]) per block of rows and columns
-    for bi in range(0, k_bins, bs):
-        ib = list(range(bi, min(k_bins, bi + bs)))
-        R1 = shells(src1, comps1, ib)
-        jstarts = [bi] if dvm == "diagonal" else list(range(bi if use_sym else 0, k_bins, bs))
-        for bj in jstarts:
-            jb = list(range(bj, min(k_bins, bj + bs)))
-            if bj == bi and use_sym:
-                R2 = None
-            else:
-                # off the diagonal the symmetric case rebuilds leg 1's shells
-                R2 = shells(src1, comps1, jb) if use_sym else shells(src2, comps2, jb)
-            # pairs this block contributes (reference loop order)
-            want = []
-            for i in ib:
-                if dvm == "diagonal":
-                    js = [i]
-                elif use_sym and bj == bi:
-                    js = [j for j in jb if j >= i]
-                else:
-                    js = jb
-                for j in js:
-                    want.append((i, j))
-            rows2 = len(ib) if R2 is None else len(jb)
-            pairs = [(c1 * len(ib) + (i - bi), c2 * rows2 + (j - bj)) for (i, j) in want
-                     for c1 in range(nc1) for c2 in range(nc2)]
-            parts = []
-            for G, gph in Gs:
-                if fused:
-                    zargs = (n, L, kmax, G, R1[0], nc1 * len(ib), R1[1], None if R2 is None else R2[0],
-                             nc2 * rows2, None if R2 is None else R2[1])
-                    # device result, read back by finish()
-                    if zbatch is not None:
-                        parts.append((zbatch.request(zargs), gph))
-                    else:
-                        parts.append((ctx.contract_zgemm(*zargs, nblk=_zgemm_blocks(ctx, zargs)), gph))
-                else:
-                    parts.append((_contract(ctx, mode, prec, G, R1, R2, pairs), gph))
-            blocks.append((ib, bi, bj, rows2, want, parts))
-            del R2
-        del R1
+    # a device allocation failure (hipMalloc OOM in the library or torch's)
+    # while the blocks are built: the blocks are rebuilt with half the
+    # block size, down to one bin (not while a graph records)
+    nreq = len(zbatch.requests) if zbatch is not None else 0
+    while True:
+        try:
+            blocks = _config_blocks(ctx, shells, Gs, kinfo, src1, src2, comps1, comps2, use_sym, dvm, bs, fused,
+                                    kmax, mode, prec, zbatch)
+            break
+        except Exception as e:
+            if not is_out_of_memory(e) or bs <= 1 or _RECORDING[0]:
+                raise
+            if zbatch is not None:
+                del zbatch.requests[nreq:]
+            blocks = None
+            torch.cuda.empty_cache()
+            logging.warning(f"device memory exhausted with block_size {bs}: retrying with {bs // 2}")
+            bs //= 2
 
     def finish():
         sub = np.zeros((k_bins, k_bins) if dvm == "full" else k_bins, dtype=np.complex128)

@@ -297,3 +297,62 @@ def test_q_ells_memo_rejects_nearby_keys():
     q1 = A.get_q_ells_eff(1, 2, moved, 2, 0, 2, len(k3), k3)
     np.testing.assert_allclose(q1, R.get_q_ells_eff(1, 2, moved, 2, 0, 2, len(k3), k3), rtol=1e-13, atol=1e-15)
     assert not np.array_equal(q1, q0)
+
+
+@pytest.mark.parametrize("kind", ["library", "torch"])
+def test_block_size_retry_on_device_oom(monkeypatch, kind):
+    """A device allocation failure while a configuration's shell blocks are
+    built (a library call returning hipErrorOutOfMemory, or torch's
+    out-of-memory error) rebuilds the blocks with half the block size, down
+    to one bin; the signal matrix is the one of the unconstrained run.  Other
+    errors, and an out-of-memory error at one bin, propagate."""
+    import torch
+
+    from cosmonpc_amd import estimators
+    from cosmonpc_amd._lib import HIP_ERROR_OUT_OF_MEMORY, CnpcError
+
+    monkeypatch.setattr(estimators, "_fit_block", lambda k_bins, b, ctx, reserve=0.25: k_bins)
+    n, k_bins = 4, 6
+
+    class FakeCtx:
+        def __init__(self, limit, err=None):
+            self.limit, self.err, self.rows = limit, err, []
+
+        def mesh(self, n, prec, rows=None):
+            self.rows.append(rows)
+            if self.err is not None:
+                raise self.err
+            if self.limit is not None and rows > self.limit:
+                if kind == "library":
+                    raise CnpcError("cnpc_shell_synth failed (status 2): injected", HIP_ERROR_OUT_OF_MEMORY)
+                raise torch.OutOfMemoryError("injected")
+            return torch.zeros((rows, n, n, n), dtype=torch.float64)
+
+        def contract_pairs(self, prec, G, R1, R2, ia, ib):
+            return np.array([float((G * R1[a] * R2[b]).sum()) for a, b in zip(ia, ib)])
+
+    class FakeBuilder:
+        method = "filter"
+
+        def build(self, src, poly, bins, out):
+            for r, b in enumerate(bins):
+                out[r] = torch.from_numpy(np.random.default_rng(b).standard_normal(out.shape[1]))
+
+    G = torch.from_numpy(np.random.default_rng(99).standard_normal(n ** 3))
+    kinfo = (n, 100.0, 0, np.linspace(0.01, 0.07, k_bins + 1), k_bins)
+
+    def run(ctx):
+        return estimators._config_plan(ctx, FakeBuilder(), None, None, [(G, 1.0)], kinfo, 0, 0, 0, 0, True, False,
+                                       "pairwise", "full", "full")()
+
+    ref_ctx = FakeCtx(None)
+    ref = run(ref_ctx)
+    assert ref_ctx.rows == [k_bins]
+    ctx = FakeCtx(2)
+    got = run(ctx)
+    assert ctx.rows[:2] == [6, 3] and set(ctx.rows[2:]) == {1}
+    np.testing.assert_allclose(got, ref, rtol=1e-12, atol=1e-12)
+    with pytest.raises(ValueError):
+        run(FakeCtx(None, err=ValueError("not a memory error")))
+    with pytest.raises((CnpcError, torch.OutOfMemoryError)):
+        run(FakeCtx(0))
