@@ -76,6 +76,7 @@ SIGNATURES = {
     "cnpc_particle_sums": [_P, _L, _P, _P, _P, _P, _I, _P, _P],
     "cnpc_pair_weights": [_P, _L, _P, _P, _I, _D, _P],
     "cnpc_radecz_to_xyz": [_P, _L, _P, _P, _P, _D, _P],
+    "cnpc_radecz_to_xyz_flrw": [_P, _L, _P, _P, _P, _P, _D, _P],
 }
 
 _lib = None

@@ -7,8 +7,10 @@ Box geometry: a ``.npy`` 2-D array whose columns are named by
 Survey geometry: FITS binary tables (``:481-563``) read by ``cosmonpc_amd.fits``
 -- redshift cut, completeness weights (``add_completeness_weight``,
 ``:179-282``), RA/DEC/Z -> comoving Cartesian positions on the GPU
-(``ra_dec_z_to_xyz``, ``:589-645``) and the box centre from the first randoms
-catalog (``:565-574``).  HDF5 needs h5py, which is not part of this framework.
+(``ra_dec_z_to_xyz``, ``:589-645``: flat LCDM from the config's h and
+Omega0, else the default Planck18 with photons and neutrinos) and the box
+centre from the first randoms catalog (``:565-574``).  HDF5 needs h5py,
+which is not part of this framework.
 """
 import logging
 import os
@@ -23,6 +25,40 @@ SUPPORTED = {"npy", "fits", "h5", "hdf5"}
 def _ext(path):
     p = path[0] if isinstance(path, list) else path
     return p.split(".")[-1].lower() if p else None
+
+
+# Planck18 as astropy defines it (Planck 2018 results VI, Table 2,
+# TT,TE,EE+lowE+lensing+BAO): flat, H0 = 67.66 km/s/Mpc, Om0 = 0.30966,
+# Tcmb0 = 2.7255 K, Neff = 3.046, neutrino masses (0, 0, 0.06) eV
+PLANCK18 = {"H0": 67.66, "Om0": 0.30966, "Tcmb0": 2.7255, "Neff": 3.046, "m_nu": (0.0, 0.0, 0.06)}
+
+
+def flrw_params(H0, Om0, Tcmb0, Neff, m_nu):
+    """(cosmo[9], c/H0 in Mpc) of a flat FLRW cosmology with photons and
+    neutrinos for ``cnpc_radecz_to_xyz_flrw``.  Omega_gamma = 4 sigma_SB
+    Tcmb0^4 / c^3 / rho_crit; neutrinos: floor(Neff) species sharing Neff,
+    T_nu0 = (4/11)^(1/3) Tcmb0, the massive ones through the fitting
+    function of Komatsu et al. 2011 (eq. 26, with astropy's constants),
+    Omega_de = 1 - Omega_m - Omega_r(z = 0).  SI constants: CODATA 2018 and
+    the IAU 2015 parsec."""
+    c, G = 299792458.0, 6.67430e-11
+    sigma_sb, k_b, ev = 5.670374419e-8, 1.380649e-23, 1.602176634e-19
+    mpc = 3.0856775814913673e22
+    h0 = H0 * 1e3 / mpc
+    rho_crit = 3.0 * h0 ** 2 / (8.0 * np.pi * G)
+    og = 4.0 * sigma_sb * Tcmb0 ** 4 / c ** 3 / rho_crit
+    nspecies = int(np.floor(Neff))
+    nupre = 0.22710731766 * Neff / nspecies  # 7/8 (4/11)^(4/3) per species
+    massive = [m for m in m_nu if m > 0]
+    if len(m_nu) != nspecies or len(massive) > 3:
+        raise ValueError("one neutrino mass per species (at most 3 massive)")
+    t_nu = 0.7137658555036082 * Tcmb0  # (4/11)^(1/3)
+    y = [m * ev / (k_b * t_nu) for m in massive]
+    nless = nspecies - len(massive)
+    nu0 = nupre * (sum((1.0 + (0.3173 * v) ** 1.83) ** 0.54644808743 for v in y) + nless)
+    ode = 1.0 - Om0 - og * (1.0 + nu0)
+    cosmo = [Om0, og, ode, nupre, float(nless), float(len(massive))] + y + [0.0] * (3 - len(y))
+    return cosmo, c / 1e3 / H0
 
 
 def _efunc(z, omega_m):
@@ -180,8 +216,13 @@ def read_survey_catalog(ctx, catalog, column_names, z_range, comp_weight_plan, p
         arr = arr[(arr["Z"] > z_min) & (arr["Z"] < z_max)]
     logging.info(f"Total objects after applying z_range cut: {len(arr)}")
     arr, weight = add_completeness_weight(arr, comp_weight_plan, catalog_type)
-    if not (para_cosmo is not None and "h" in para_cosmo and "Omega0" in para_cosmo):
-        raise NotImplementedError("the default Planck18 cosmology is not available: give cosmology 'h' and 'Omega0'")
+    # 'h' and 'Omega0' given: FlatLambdaCDM(H0 = 100 h, Om0) without
+    # radiation; otherwise the default Planck18, its distances in Mpc times
+    # the config's h (catalog_processor.py:609-632: a missing config or 'h'
+    # fails there as here, on para_cosmo["h"])
+    planck = not (para_cosmo is not None and "h" in para_cosmo and "Omega0" in para_cosmo)
+    if planck:
+        h_cfg = para_cosmo["h"]
     names = arr.dtype.names
     if "WEIGHT_FKP" not in names:
         raise ValueError("WEIGHT_FKP column is missing in the catalog.")
@@ -200,7 +241,13 @@ def read_survey_catalog(ctx, catalog, column_names, z_range, comp_weight_plan, p
     def col(a):
         return torch.from_numpy(np.require(a, dtype=np.float64, requirements=["C", "W"])).to(dev)
 
-    pos = ctx.radecz_to_xyz(col(arr["RA"]), col(arr["DEC"]), col(arr["Z"]), float(para_cosmo["Omega0"]))
+    if planck:
+        logging.info("Using default Planck18 cosmology")
+        p18 = PLANCK18
+        cosmo, dh = flrw_params(p18["H0"], p18["Om0"], p18["Tcmb0"], p18["Neff"], p18["m_nu"])
+        pos = ctx.radecz_to_xyz_flrw(col(arr["RA"]), col(arr["DEC"]), col(arr["Z"]), cosmo, dh * float(h_cfg))
+    else:
+        pos = ctx.radecz_to_xyz(col(arr["RA"]), col(arr["DEC"]), col(arr["Z"]), float(para_cosmo["Omega0"]))
     cat = SurveyCatalog(pos, col(weight), col(arr["WEIGHT_FKP"]), col(nz))
     if boxcenter is None:
         if len(cat) == 0:

@@ -923,7 +923,35 @@ int cnpc_radecz_to_xyz(cnpc_ctx* ctx, int64_t np, const double* ra, const double
   });
   RadeczArgs a = base;
   a.om = omega_m;
+  a.ode = 1.0 - omega_m;
   a.dh = 299792.458 / 100.0;
+  CHECK_HIP(launch_radecz(np, ra, dec, z, a, pos, ctx->stream));
+  return 0;
+}
+
+int cnpc_radecz_to_xyz_flrw(cnpc_ctx* ctx, int64_t np, const double* ra, const double* dec, const double* z,
+                            const double* cosmo, double dh, double* pos) {
+  REQUIRE(ctx, "null context");
+  REQUIRE(cosmo, "radecz_to_xyz_flrw: null cosmology");
+  const int nm = (int)cosmo[5];
+  REQUIRE(np >= 0 && cosmo[0] > 0.0 && cosmo[1] >= 0.0 && cosmo[3] >= 0.0 && cosmo[4] >= 0.0 && nm >= 0 && nm <= 3 &&
+              (double)nm == cosmo[5] && dh > 0.0,
+          "radecz_to_xyz_flrw: bad arguments (Omega_m > 0, Omega_gamma >= 0, 0..3 massive species, dh > 0)");
+  for (int m = 0; m < nm; ++m) REQUIRE(cosmo[6 + m] >= 0.0, "radecz_to_xyz_flrw: negative neutrino mass");
+  REQUIRE(np == 0 || (ra && dec && z && pos), "radecz_to_xyz_flrw: null arrays");
+  if (np == 0) return 0;
+  RadeczArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.nodes = 48;
+  gauss_legendre(a.nodes, a.t, a.w);
+  a.om = cosmo[0];
+  a.og = cosmo[1];
+  a.ode = cosmo[2];
+  a.nupre = cosmo[3];
+  a.nless = cosmo[4];
+  a.nmassive = nm;
+  for (int m = 0; m < nm; ++m) a.y[m] = cosmo[6 + m];
+  a.dh = dh;
   CHECK_HIP(launch_radecz(np, ra, dec, z, a, pos, ctx->stream));
   return 0;
 }

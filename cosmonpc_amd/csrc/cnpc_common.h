@@ -247,9 +247,14 @@ struct KCombineArgs {
 #define CNPC_GL_MAX 64
 struct RadeczArgs {
   int nodes;
-  int pad;
-  double om;  // Omega_m
-  double dh;  // c / (100 km/s/Mpc) in Mpc/h
+  int nmassive;   // massive neutrino species (0..3)
+  double om;      // Omega_m
+  double og;      // Omega_gamma (0: no radiation)
+  double ode;     // Omega_de
+  double nupre;   // 7/8 (4/11)^(4/3) N_eff / N_species
+  double nless;   // massless neutrino species
+  double y[3];    // m c^2 / (k_B T_nu0) of the massive species
+  double dh;      // c / H0 in the output unit
   double t[CNPC_GL_MAX];
   double w[CNPC_GL_MAX];
 };

@@ -364,10 +364,13 @@ __global__ void __launch_bounds__(256) pair_weights_kernel(int64_t np, const dou
 }
 
 // ------------------------------------------------------------- RA/DEC/Z
-// chi(z) = c/(100 km/s/Mpc) * int_0^z dz'/E(z') in Mpc/h with
-// E = sqrt(Om (1+z)^3 + 1 - Om) (flat LCDM, no radiation), by Gauss-Legendre
-// quadrature on [0, z]; x = chi cos(dec) cos(ra), y = chi cos(dec) sin(ra),
-// z = chi sin(dec)  (catalog_processor.py:589-645).
+// chi(z) = dh * int_0^z dz'/E(z') by Gauss-Legendre quadrature on [0, z];
+// x = chi cos(dec) cos(ra), y = chi cos(dec) sin(ra), z = chi sin(dec)
+// (catalog_processor.py:589-645).  Flat FLRW: E^2 = q^3 (Om + Or(q) q) + Ode
+// with q = 1 + z and Or = Og (1 + nu(q)), nu = nupre (sum_i f(y_i / q) +
+// nless), f(y) = (1 + (0.3173 y)^1.83)^(1/1.83) (the massive-neutrino
+// density fit of Komatsu et al. 2011, ApJS 192, 18, eq. 26, as astropy
+// evaluates it); Og = 0 is flat LCDM without radiation.
 __global__ void __launch_bounds__(256) radecz_kernel(int64_t np, const double* __restrict__ ra,
                                                      const double* __restrict__ dec, const double* __restrict__ zz,
                                                      RadeczArgs a, double* __restrict__ pos) {
@@ -378,7 +381,13 @@ __global__ void __launch_bounds__(256) radecz_kernel(int64_t np, const double* _
     for (int j = 0; j < a.nodes; ++j) {
       const double zp = half * (a.t[j] + 1.0);
       const double q = 1.0 + zp;
-      s += a.w[j] / sqrt(a.om * (q * q * q) + (1.0 - a.om));
+      double e2 = a.om * (q * q * q) + a.ode;
+      if (a.og > 0.0) {
+        double nu = a.nless;
+        for (int m = 0; m < a.nmassive; ++m) nu += pow(1.0 + pow(0.3173 * (a.y[m] / q), 1.83), 0.54644808743);
+        e2 += a.og * (1.0 + a.nupre * nu) * (q * q * q * q);
+      }
+      s += a.w[j] / sqrt(e2);
     }
     const double chi = a.dh * (half * s);
     const double r = ra[i] * (M_PI / 180.0), d = dec[i] * (M_PI / 180.0);

@@ -454,6 +454,22 @@ class GpuContext:
               "cnpc_pair_weights")
         return out
 
+    def radecz_to_xyz_flrw(self, ra, dec, z, cosmo, dh):
+        """RA/DEC/Z -> positions in a flat FLRW cosmology with radiation and
+        neutrinos (``cosmo``: the 9 numbers of cnpc_radecz_to_xyz_flrw,
+        ``dh`` = c/H0 in the output unit)."""
+        np_ = ra.shape[0]
+        for t, nm in ((ra, "RA"), (dec, "DEC"), (z, "Z")):
+            self._check_vec(t, np_, nm)
+        c = np.ascontiguousarray(cosmo, dtype=np.float64)
+        if c.shape != (9,):
+            raise ValueError("cosmo must hold 9 numbers")
+        out = torch.empty((np_, 3), dtype=torch.float64, device=self.device)
+        check(self.lib.cnpc_radecz_to_xyz_flrw(self.h, np_, _ptr(ra), _ptr(dec), _ptr(z),
+                                               c.ctypes.data_as(ctypes.c_void_p), float(dh), _ptr(out)),
+              "cnpc_radecz_to_xyz_flrw")
+        return out
+
     def radecz_to_xyz(self, ra, dec, z, omega_m):
         np_ = ra.shape[0]
         for t, nm in ((ra, "RA"), (dec, "DEC"), (z, "Z")):

@@ -322,6 +322,18 @@ int cnpc_pair_weights(cnpc_ctx* ctx, int64_t np, const double* W, const double* 
 int cnpc_radecz_to_xyz(cnpc_ctx* ctx, int64_t np, const double* ra, const double* dec, const double* z, double omega_m,
                        double* pos_xyz);
 
+/* The same in a flat FLRW cosmology with photons and neutrinos (massless
+ * and massive), the default Planck18 branch of ra_dec_z_to_xyz
+ * (catalog_processor.py:609-632: astropy's Planck18 comoving distance in Mpc
+ * times the config's h).  cosmo[9] = {Omega_m, Omega_gamma, Omega_de,
+ * nu_prefac, n_massless_nu, n_massive_nu (0..3), y_1, y_2, y_3}: E(z)^2 =
+ * (1+z)^3 (Omega_m + Omega_gamma (1 + nu(z)) (1+z)) + Omega_de with
+ * nu(z) = nu_prefac (sum_i f(y_i / (1+z)) + n_massless_nu),
+ * f(y) = (1 + (0.3173 y)^1.83)^(1/1.83) (y_i = m_i c^2 / (k_B T_nu0));
+ * dh = c / H0 in the output unit (Mpc/h: c / H0[Mpc] times h). */
+int cnpc_radecz_to_xyz_flrw(cnpc_ctx* ctx, int64_t np, const double* ra, const double* dec, const double* z,
+                            const double* cosmo, double dh, double* pos_xyz);
+
 #ifdef __cplusplus
 }
 #endif

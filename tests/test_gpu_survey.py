@@ -31,6 +31,21 @@ def _close(a, b, tol=TOL, what=""):
     assert err < tol, f"{what}: {err:.3e}"
 
 
+def test_radecz_planck18_default(gpu_ctx):
+    """No 'Omega0' in the cosmology: the default Planck18 branch (photons,
+    two massless and one 0.06 eV neutrino; distances in Mpc times the
+    config's h) against the oracle's FLRW quadrature, to 1e-12."""
+    from cosmonpc_amd.catalog import read_survey_catalog
+    from oracle.cnpc_ref import catalog as OC
+
+    d, _ = survey_rows(3000, 10, 6)
+    for h in (0.6766, 0.7):
+        cat, bc = read_survey_catalog(gpu_ctx, None, ["RA", "DEC", "Z"], (0.45, 0.5), NO_COMP, {"h": h}, "data",
+                                      rows=d)
+        ref = OC.survey_catalog(d, (0.45, 0.5), NO_COMP, {"h": h}, "data")
+        np.testing.assert_allclose(cat.pos.cpu().numpy(), ref["Position"], rtol=1e-12, atol=1e-9)
+
+
 def test_radecz_and_particle_sums(gpu_ctx):
     import torch
 

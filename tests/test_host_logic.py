@@ -356,3 +356,33 @@ def test_block_size_retry_on_device_oom(monkeypatch, kind):
         run(FakeCtx(None, err=ValueError("not a memory error")))
     with pytest.raises((CnpcError, torch.OutOfMemoryError)):
         run(FakeCtx(0))
+
+
+def test_planck18_parameters_and_missing_h():
+    """The product's Planck18 numbers (Omega_gamma, Omega_de, the neutrino
+    prefactor and y of the 0.06 eV species) agree with the oracle's FLRW
+    restatement; without a config, or without 'h', the survey reader fails
+    on para_cosmo["h"] as the reference does (TypeError / KeyError)."""
+    from cosmonpc_amd.catalog import PLANCK18, flrw_params, read_survey_catalog
+    from oracle.cnpc_ref import catalog as OC
+
+    from helpers import NO_COMP, survey_rows
+
+    p = PLANCK18
+    cosmo, dh = flrw_params(p["H0"], p["Om0"], p["Tcmb0"], p["Neff"], p["m_nu"])
+    og = OC.omega_gamma(p["H0"], p["Tcmb0"])
+    np.testing.assert_allclose(cosmo[1], og, rtol=1e-12)
+    onu = og * OC.nu_density_ratio(0.0, p["Tcmb0"], p["Neff"], p["m_nu"])
+    np.testing.assert_allclose(cosmo[2], 1 - p["Om0"] - og - onu, rtol=1e-12)
+    assert cosmo[4:6] == [2.0, 1.0] and cosmo[7:] == [0.0, 0.0]
+    np.testing.assert_allclose(dh, 299792.458 / 67.66, rtol=1e-15)
+    for z in (0.0, 0.7, 3.0):  # E(z) of the product's numbers == the oracle's
+        q = 1 + z
+        nu = cosmo[4] + (1 + (0.3173 * cosmo[6] / q) ** 1.83) ** 0.54644808743
+        e = np.sqrt(cosmo[0] * q ** 3 + cosmo[1] * (1 + cosmo[3] * nu) * q ** 4 + cosmo[2])
+        np.testing.assert_allclose(e, OC.efunc_flrw(z, **OC.P18), rtol=1e-13)
+    d, _ = survey_rows(50, 10, 3)
+    with pytest.raises(TypeError):
+        read_survey_catalog(None, None, ["RA", "DEC", "Z"], None, NO_COMP, None, "data", rows=d)
+    with pytest.raises(KeyError):
+        read_survey_catalog(None, None, ["RA", "DEC", "Z"], None, NO_COMP, {"Omega0": 0.3}, "data", rows=d)

@@ -263,3 +263,44 @@ def test_bk_survey_shot_noise_distant_observer_equals_box(angu, sn):
     assert np.max(np.abs(sur["SN_terms"]["SN3"] * sur["I_norm"])) > 1e-3 * scale
     if angu == [2, 0, 2]:
         assert np.max(np.abs(sur["SN_terms"]["SN1"] * sur["I_norm"])) > 1e-3 * scale
+
+
+def test_planck18_flrw_pins():
+    """The Planck18 branch's E(z) and distances: E(0) = 1 (flat), the photon
+    density 2.4728e-5 h^-2 at 2.7255 K, the massless-neutrino ratio
+    7/8 (4/11)^(4/3) N_eff, the non-relativistic limit of the 0.06 eV
+    species (Omega_nu h^2 = m / 93.14 eV within 1.5%: the per-species
+    N_eff / 3 enters as a factor), the fitting constant 0.3173 against
+    180 zeta(3) / (7 pi^4), and the no-radiation, massless limits against the
+    flat-LCDM and closed-form distances."""
+    from scipy.special import zeta
+
+    p = OC.P18
+    h = p["H0"] / 100
+    og = OC.omega_gamma(p["H0"], p["Tcmb0"])
+    assert abs(og * h * h / 2.4728e-5 - 1) < 2e-4
+    np.testing.assert_allclose(OC.efunc_flrw(0.0, **p), 1.0, rtol=0, atol=1e-15)
+    r0 = OC.nu_density_ratio(0.0, p["Tcmb0"], p["Neff"], (0.0, 0.0, 0.0))
+    np.testing.assert_allclose(r0, 7 / 8 * (4 / 11) ** (4 / 3) * p["Neff"], rtol=1e-12)
+    assert abs(0.3173 / (180 * zeta(3) / (7 * np.pi ** 4)) - 1) < 1e-3
+    massive = OC.nu_density_ratio(0.0, p["Tcmb0"], p["Neff"], p["m_nu"]) - 2 / 3 * r0
+    assert abs(og * massive * h * h / (0.06 / 93.14) - 1) < 0.015
+    # no photons: the flat-LCDM distance (times h / (H0/100) = 1 in Mpc/h)
+    z = np.array([0.1, 0.55, 1.0, 2.5])
+    np.testing.assert_allclose(OC.comoving_distance_flrw(z, 1.0, 100.0, 0.3, 1e-30, 3.046, (0.0, 0.0, 0.0)),
+                               OC.comoving_distance(z, 0.3), rtol=1e-12)
+    # matter + radiation only (a 19 K photon bath, massless neutrinos, so
+    # Om + Or = 1 and no Lambda): with a = 1/(1+z),
+    # chi = c/H0 * 2 (sqrt(Om + Or) - sqrt(Om a + Or)) / Om
+    H0 = 100.0
+    orad = OC.omega_gamma(H0, 19.0) * (1 + 7 / 8 * (4 / 11) ** (4 / 3) * 3.0)
+    om = 1.0 - orad
+    assert 0.05 < orad < 0.5
+    chi = OC.comoving_distance_flrw(z, 1.0, H0, om, 19.0, 3.0, (0.0, 0.0, 0.0))
+    exact = CH * 2.0 * (np.sqrt(om + orad) - np.sqrt(om / (1 + z) + orad)) / om
+    np.testing.assert_allclose(chi, exact, rtol=1e-10)
+    # the Planck18 distances: smaller than the no-radiation ones of the
+    # same Om0 (radiation and massive neutrinos raise E) by a small amount
+    d18 = OC.comoving_distance_flrw(z, h)
+    dl = OC.comoving_distance(z, p["Om0"])
+    assert np.all(d18 < dl) and np.all(dl / d18 - 1 < 2e-3)
