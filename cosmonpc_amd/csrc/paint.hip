@@ -295,12 +295,16 @@ __global__ void __launch_bounds__(256) paint_scan_add_kernel(int* __restrict__ s
   if (b == 0 && threadIdx.x == 0) start[m] = bsum[nb];
 }
 
+// off != nullptr: the particles are [0, off[nbatch]) (a batch of catalogs;
+// np is then an upper bound that sizes the grid)
 __global__ void __launch_bounds__(256) paint_scatter_kernel(const double* __restrict__ pos,
                                                             const double* __restrict__ w, int64_t np,
+                                                            const int64_t* __restrict__ off, int nbatch,
                                                             const int* __restrict__ key,
                                                             const int* __restrict__ start, int* __restrict__ cursor,
                                                             double* __restrict__ spos, double* __restrict__ sw) {
-  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < np; p += (int64_t)gridDim.x * blockDim.x) {
+  const int64_t npe = off ? off[nbatch] : np;
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < npe; p += (int64_t)gridDim.x * blockDim.x) {
     const int k = key[p];
     const int64_t q = (int64_t)start[k] + atomicAdd(cursor + k, 1);
     spos[3 * q + 0] = pos[3 * p + 0];
@@ -652,7 +656,7 @@ static hipError_t launch_paint_tiled_t(const double* pos, const double* w, const
   hipLaunchKernelGGL(paint_scan_top_kernel, dim3(1), dim3(1024), 0, st, bsum, nb);
   hipLaunchKernelGGL(paint_scan_add_kernel, dim3(nb), dim3(256), 0, st, start, m, bsum, nb);
   if (np > 0)
-    hipLaunchKernelGGL(paint_scatter_kernel, grd, blk, 0, st, pos, w, np, key, start, cursor, spos, sw);
+    hipLaunchKernelGGL(paint_scatter_kernel, grd, blk, 0, st, pos, w, np, off, nbatch, key, start, cursor, spos, sw);
 #define CNPC_TILES(O)                                                                                              \
   (PT == 16 ? launch_tiles<T, O, 16>(spos, sw, start, n, nt, nbatch, scale, cx, cy, cz, ncopy, m0, m1, st)         \
             : launch_tiles<T, O, 8>(spos, sw, start, n, nt, nbatch, scale, cx, cy, cz, ncopy, m0, m1, st))

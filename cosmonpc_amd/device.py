@@ -89,23 +89,39 @@ class GpuContext:
         check(self.lib.cnpc_paint(self.h, _ptr(pos), _ptr(w), pos.shape[0], n, float(boxsize), WINDOW[sampler],
                                   int(bool(interlaced)), center, prec, _ptr(mesh0), _ptr(mesh1)), "cnpc_paint")
 
-    def paint_batch(self, pos, w, offsets, n, boxsize, sampler, interlaced, prec, mesh0, mesh1=None):
+    def paint_batch(self, pos, w, offsets, n, boxsize, sampler, interlaced, prec, mesh0, mesh1=None, off_dev=None,
+                    bounds=None):
         """Paint the catalogs [offsets[b], offsets[b+1]) of the concatenated
         device arrays into meshes mesh0[b] (and mesh1[b]); ``offsets`` is a
-        host int sequence of nbatch + 1 entries."""
+        host int sequence of nbatch + 1 entries.  For a call recorded in a
+        graph and replayed with other catalogs: ``off_dev`` is a device int64
+        tensor holding the offsets (rewritten before each replay), ``pos``
+        and ``w`` are capacity-sized buffers, and ``bounds`` = (total, max)
+        upper bounds of the particle count and the largest catalog."""
         off = np.asarray(offsets, dtype=np.int64)
         nb = len(off) - 1
-        if nb < 1 or off[0] != 0 or np.any(np.diff(off) < 0) or off[-1] != pos.shape[0]:
-            raise ValueError("offsets must start at 0, increase and end at the particle count")
+        if nb < 1 or off[0] != 0 or np.any(np.diff(off) < 0):
+            raise ValueError("offsets must start at 0 and increase")
+        if bounds is not None:
+            if off_dev is None or off_dev.dtype != torch.int64 or off_dev.numel() != nb + 1 or not off_dev.is_cuda:
+                raise ValueError("bounds need the device offsets (int64, nbatch + 1 entries)")
+            total, biggest = int(bounds[0]), int(bounds[1])
+            if total > pos.shape[0] or total < off[-1] or biggest < int(np.diff(off).max()) or biggest > total:
+                raise ValueError("bounds must cover the catalogs and fit the particle buffers")
+        elif off[-1] != pos.shape[0]:
+            raise ValueError("offsets must end at the particle count")
         if (pos.dtype != torch.float64 or w.dtype != torch.float64 or pos.dim() != 2 or pos.shape[1] != 3
                 or w.shape != (pos.shape[0],) or not (pos.is_contiguous() and w.is_contiguous() and pos.is_cuda)):
             raise ValueError("positions must be a contiguous (Np, 3) and weights an (Np,) float64 device tensor")
         self._check_mesh(mesh0, n, prec, nb)
         if interlaced:
             self._check_mesh(mesh1, n, prec, nb)
-        off_d = torch.from_numpy(off).to(self.device)
-        check(self.lib.cnpc_paint_batch(self.h, _ptr(pos), _ptr(w), _ptr(off_d), nb, int(off[-1]),
-                                        int(np.diff(off).max()), n,
+        if bounds is None:
+            off_d = torch.from_numpy(off).to(self.device)
+            total, biggest = int(off[-1]), int(np.diff(off).max())
+        else:
+            off_d = off_dev
+        check(self.lib.cnpc_paint_batch(self.h, _ptr(pos), _ptr(w), _ptr(off_d), nb, total, biggest, n,
                                         float(boxsize), WINDOW[sampler], int(bool(interlaced)), None, prec,
                                         _ptr(mesh0), _ptr(mesh1 if interlaced else None)), "cnpc_paint_batch")
 
@@ -137,6 +153,18 @@ class GpuContext:
         if npoly:
             self._check_spec(a, n, prec)
             self._check_spec(b, n, prec)
+        out = self.bin_dev(a, b, n, boxsize, prec, edges, polys, ab_scale, noise_window, noise_amp)
+        return self.split_bins(out.cpu().numpy(), nb, npoly)
+
+    def bin_dev(self, a, b, n, boxsize, prec, edges, polys=(), ab_scale=1.0, noise_window=0, noise_amp=0.0):
+        """``bin`` without the read-back: the device FP64 vector [count, ksum,
+        re (npoly x nb), im (npoly x nb)] (split it with ``split_bins``)."""
+        edges = np.ascontiguousarray(edges, dtype=np.float64)
+        nb = len(edges) - 1
+        npoly = len(polys)
+        if npoly:
+            self._check_spec(a, n, prec)
+            self._check_spec(b, n, prec)
         out = torch.empty((2 + 2 * npoly) * nb, dtype=torch.float64, device=self.device)
         parr = (cnpc_poly * max(npoly, 1))(*polys) if npoly else None
         check(self.lib.cnpc_bin_spectrum(self.h, n, float(boxsize), prec, _ptr(a if npoly else None),
@@ -144,7 +172,12 @@ class GpuContext:
                                          ctypes.cast(parr, ctypes.c_void_p) if parr is not None else None,
                                          float(ab_scale), int(noise_window), float(noise_amp), _ptr(out)),
               "cnpc_bin_spectrum")
-        h = out.cpu().numpy()
+        return out
+
+    @staticmethod
+    def split_bins(h, nb, npoly):
+        """(count, ksum, re[npoly, nb], im[npoly, nb]) of a host copy of a
+        ``bin_dev`` vector."""
         return h[:nb].copy(), h[nb:2 * nb].copy(), h[2 * nb:(2 + npoly) * nb].reshape(npoly, nb), \
             h[(2 + npoly) * nb:].reshape(npoly, nb)
 

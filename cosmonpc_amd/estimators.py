@@ -35,6 +35,33 @@ def _sync_time(ctx):
 
 
 # ----------------------------------------------------------------------- pk
+def _pk_polys(even, rsd):
+    return [A.make_poly(0, [1.0], A.PART_FULL, 1.0, 1.0) if p == 0 else A.legendre_poly(p, rsd) for p in even]
+
+
+def pk_box_device_pass(field, stat_attrs):
+    """The device work of ``calculate_power_spectrum_box`` for an
+    auto-correlation (compensated, interlaced delta_k with k = 0 cleared,
+    then the Legendre-weighted binning), enqueued only: returns the function
+    that reads (count, ksum, re) back for ``precomputed=``.  Nothing in it
+    depends on the catalog beyond the painted field, so it can be recorded in
+    a graph once and replayed for every catalog."""
+    ctx = get_context()
+    poles = stat_attrs["poles"]
+    even = [p for p in poles if not p & 1]
+    rsd = np.array(stat_attrs["rsd"], dtype=float)
+    k_edge = np.linspace(stat_attrs["k_min"], stat_attrs["k_max"], stat_attrs["k_bins"] + 1)
+    comp = 1 if stat_attrs["interlaced"] else 2
+    da = field.delta_k(ctx, comp, zero_k0=True)
+    out = ctx.bin_dev(da, da, field.n, field.boxsize, field.prec, k_edge, _pk_polys(even, rsd))
+    nb, npoly = len(k_edge) - 1, len(even)
+
+    def finish():
+        count, ksum, re, _ = ctx.split_bins(out.cpu().numpy(), nb, npoly)
+        return count, ksum, re
+    return finish
+
+
 def calculate_power_spectrum_box(rfield_a, rfield_b, correlation_mode, stat_attrs, comm=None, **kwargs):
     """P_ell(k) of a periodic box (``clustering_estimator.py:566-746``)."""
     ctx = get_context()
@@ -49,14 +76,17 @@ def calculate_power_spectrum_box(rfield_a, rfield_b, correlation_mode, stat_attr
     k_edge = np.linspace(k_min, k_max, k_bins + 1)
     n, L, prec = rfield_a.n, rfield_a.boxsize, rfield_a.prec
 
-    comp = 1 if interlaced else 2  # sinc^p when interlaced, else sqrt alias sums
-    # the zero mode of field a is cleared (clustering_estimator.py:634-635)
-    da = rfield_a.delta_k(ctx, comp, zero_k0=True)
-    db = rfield_b.delta_k(ctx, comp) if correlation_mode == "cross" else da
-
     even = [p for p in poles if not p & 1]
-    polys = [A.make_poly(0, [1.0], A.PART_FULL, 1.0, 1.0) if p == 0 else A.legendre_poly(p, rsd) for p in even]
-    count, ksum, re, _ = ctx.bin(da, db, n, L, prec, k_edge, polys)
+    precomputed = kwargs.get("precomputed")
+    if precomputed is not None:
+        # (count, ksum, re) of pk_box_device_pass, read back after a replay
+        count, ksum, re = precomputed
+    else:
+        comp = 1 if interlaced else 2  # sinc^p when interlaced, else sqrt alias sums
+        # the zero mode of field a is cleared (clustering_estimator.py:634-635)
+        da = rfield_a.delta_k(ctx, comp, zero_k0=True)
+        db = rfield_b.delta_k(ctx, comp) if correlation_mode == "cross" else da
+        count, ksum, re, _ = ctx.bin(da, db, n, L, prec, k_edge, _pk_polys(even, rsd))
     k_num = count
     with np.errstate(invalid="ignore", divide="ignore"):
         results = {"k_eff": ksum / k_num, "k_num": k_num}
@@ -184,8 +214,9 @@ def _pow2_scale(amax, target=2.0 ** 14):
 
 
 def _contract(ctx, mode, prec, G, R1, R2, pairs):
-    """Real contraction sum_x G R1[a] R2[b]: full matrix (GEMM modes) or the
-    listed (a, b) pairs (pairwise).  R2 None means R2 is R1."""
+    """Real contraction sum_x G R1[a] R2[b]: full matrix (GEMM modes, a
+    device tensor) or the listed (a, b) pairs (pairwise, host).  R2 None
+    means R2 is R1."""
     if mode == "pairwise":
         ia = np.array([p[0] for p in pairs], dtype=np.int32)
         ib = np.array([p[1] for p in pairs], dtype=np.int32)
@@ -202,8 +233,9 @@ def _contract(ctx, mode, prec, G, R1, R2, pairs):
         sa = _pow2_scale(ctx.absmax(R1), target)
         sg = ctx.absmax(G)
         sb = _pow2_scale(sg * ctx.absmax(R1 if R2 is None else R2), target)
-        return ctx.contract_gemm(mode, G, R1, R2, sa, sb).cpu().numpy()
-    return ctx.contract_gemm(mode, G, R1, R2, nblk=_gemm_blocks(ctx, mode, G, R1, R2)).cpu().numpy()
+        return ctx.contract_gemm(mode, G, R1, R2, sa, sb)
+    # device result, read back by the plan's finish()
+    return ctx.contract_gemm(mode, G, R1, R2, nblk=_gemm_blocks(ctx, mode, G, R1, R2))
 
 
 def _zgemm_blocks(ctx, zargs, with_bb=False):
@@ -390,9 +422,9 @@ def _config_plan(ctx, builder, src1, src2, Gs, kinfo, ell_1, m1, ell_2, m2, sym_
 
     The device work is enqueued here; the returned ``finish()`` reads the
     results back and assembles the matrix, and may be called again after the
-    same work is replayed from a graph.  The fused contraction leaves its
-    results on the device until then; the other paths read them back as they
-    go.  ``shell_cache`` (a dict shared by several calls on the same spectra)
+    same work is replayed from a graph.  The GEMM contractions (fused or
+    not) leave their results on the device until then; the pairwise path
+    reads them back as it goes.  ``shell_cache`` (a dict shared by several calls on the same spectra)
     keeps the half-synthesised shells of the fused path, so a leg that
     another configuration already synthesised (same spectrum, weight and
     bins) is not synthesised again; with ``zbatch`` (a ``ZBatch``) the fused
@@ -664,6 +696,79 @@ def bk_box_signals(field, stat_attrs_list):
     return [(n / m) ** 3 * r for r in res]
 
 
+def _sn_poly(L):
+    return A.ylm_poly(L, 0, True) if L != 0 else A.make_poly(0, [1.0], A.PART_FULL, 1.0, 1.0)
+
+
+def _k3_edges(k_min, k_max, k_bins):
+    return np.linspace(2 * k_min, 2 * k_max, k_bins * 2 + 1)[:-1]
+
+
+def bk_box_bin_constants(stat_attrs, n, boxsize, prec):
+    """The catalog-independent binnings of ``calculate_bk_sugi_box``: mode
+    counts and |k| sums on the k and k3 bins, and the k3 bins of the shot
+    noise template C_p/sinc^2p times the Y_L0 weight (what the binning
+    subtracts per unit of SN0)."""
+    ctx = get_context()
+    k_min, k_max, k_bins = stat_attrs["k_min"], stat_attrs["k_max"], stat_attrs["k_bins"]
+    k_edge = np.linspace(k_min, k_max, k_bins + 1)
+    k3_edge = _k3_edges(k_min, k_max, k_bins)
+    p = {"ngp": 1, "cic": 2, "tsc": 3, "pcs": 4}[stat_attrs["sampler"]]
+    count, ksum, _, _ = ctx.bin(None, None, n, boxsize, prec, k_edge)
+    count3, ksum3, _, _ = ctx.bin(None, None, n, boxsize, prec, k3_edge)
+    zero = ctx.spectrum(n, prec)
+    zero.zero_()
+    # the binning returns ab_scale a conj(b) - noise_amp * template: with a
+    # zero spectrum and unit amplitude, minus the template's bins
+    _, _, tmpl, _ = ctx.bin(zero, zero, n, boxsize, prec, k3_edge, [_sn_poly(stat_attrs["angu_config"][2])],
+                            ab_scale=0.0, noise_window=p, noise_amp=1.0)
+    return {"k": (count, ksum), "k3": (count3, ksum3), "noise": -tmpl[0]}
+
+
+def bk_box_device_pass(field, stat_attrs_list, sub0_dev):
+    """The device work of ``calculate_bk_sugi_box`` for the auto-correlation
+    configurations of one signal group ('ana' shot noise), enqueued only:
+    delta_k, the multi-configuration signal pass of ``bk_box_signals`` and
+    the k3 binning of V^2 |delta_k|^2 per configuration.  NZ (``sub0_dev``, a
+    device FP64 scalar) is subtracted from the zero mode on the device, so
+    nothing recorded depends on the catalog and the work can be replayed
+    from a graph.  Returns the function that reads, per configuration,
+    {"signal", "re3"} back (combine with ``bk_box_bin_constants`` into
+    ``precomputed=``)."""
+    ctx = get_context()
+    sa = stat_attrs_list[0]
+    dvm = sa.get("data_vector_mode", "diagonal")
+    block_size = sa.get("block_size", 1 if dvm == "diagonal" else "full")
+    n, Lbox, prec = field.n, field.boxsize, field.prec
+    k_min, k_max, k_bins = sa["k_min"], sa["k_max"], sa["k_bins"]
+    k_edge = np.linspace(k_min, k_max, k_bins + 1)
+    k3_edge = _k3_edges(k_min, k_max, k_bins)
+    contraction = _gemm_mode(sa.get("contraction", "gemm_fp64" if prec == 0 else "bf16x3"), prec)
+    V = float(np.prod(np.array(sa["boxsize"], dtype=float)))
+    vol_per_cell = V / np.prod(np.array(sa["nmesh"]))
+    p = {"ngp": 1, "cic": 2, "tsc": 3, "pcs": 4}[sa["sampler"]]
+    ca = field.delta_k(ctx, 1, scale=1.0 / vol_per_cell, sub0=0.0)
+    ca.view(-1)[0] -= sub0_dev
+    m, nmax = contraction_grid(n, Lbox, k_max, contraction, sa.get("shell_grid", "reduced"))
+    ca_m = reduce_spectra(ctx, [ca], n, prec, m, nmax)[0] if m != n else ca
+    G = bk_box_g00(ctx, ca_m, m, prec)
+    sig = bk_box_signal_multi(ctx, ca_m, G, m, Lbox, prec, k_edge, [tuple(s["angu_config"]) for s in stat_attrs_list],
+                              contraction, dvm, block_size, sa.get("zsynth", "fused"), defer=True,
+                              shell_method=sa.get("shell_method", "synth"))
+    bins3 = [ctx.bin_dev(ca, ca, n, Lbox, prec, k3_edge, [_sn_poly(s["angu_config"][2])], ab_scale=V * V,
+                         noise_window=p, noise_amp=0.0) for s in stat_attrs_list]
+    nb3 = len(k3_edge) - 1
+
+    def finish():
+        res = sig()
+        out = []
+        for r, b3 in zip(res, bins3):
+            _, _, re3, _ = ctx.split_bins(b3.cpu().numpy(), nb3, 1)
+            out.append({"signal": (n / m) ** 3 * r, "re3": re3})
+        return out
+    return finish
+
+
 def calculate_bk_sugi_box(rfield_a, rfield_b, rfield_c, correlation_mode, stat_attrs, comm=None, **kwargs):
     """Sugiyama bispectrum multipole of a periodic box
     (``clustering_estimator.py:749-1555``)."""
@@ -703,20 +808,28 @@ def calculate_bk_sugi_box(rfield_a, rfield_b, rfield_c, correlation_mode, stat_a
     elif dvm != "diagonal":
         raise ValueError("data_vector_mode must be either 'diagonal' or 'full'")
 
-    t0 = _sync_time(ctx)
-    # delta_k = sinc^p-compensated r2c(n/dV - NZ)   (clustering_estimator.py:808-816)
-    ca = rfield_a.delta_k(ctx, 1, scale=1.0 / vol_per_cell, sub0=NZ_a)
-    cb = rfield_b.delta_k(ctx, 1, scale=1.0 / vol_per_cell, sub0=NZ_b) if correlation_mode == "cross" else None
-    cc = (rfield_c.delta_k(ctx, 1, scale=1.0 / vol_per_cell, sub0=NZ_c)
-          if (correlation_mode == "cross" and tracer_type == "abc") else None)
-    # G_00 = Re c2r(delta_k) of the third leg (:844-851)
-    gsrc = ca if correlation_mode == "auto" else (cc if tracer_type == "abc" else cb)
-    count, ksum, _, _ = ctx.bin(None, None, n, Lbox, prec, k_edge)
+    # precomputed: the device results of bk_box_device_pass for this
+    # configuration, read back after a graph replay (auto, 'ana' shot noise)
+    pre = kwargs.get("precomputed")
+    if pre is not None and (correlation_mode != "auto" or sn_mode != "ana"):
+        raise ValueError("precomputed device results cover auto-correlation with shotnoise_mode 'ana' only")
+    t0 = _sync_time(ctx) if pre is None else time.perf_counter()
+    if pre is None:
+        # delta_k = sinc^p-compensated r2c(n/dV - NZ)   (clustering_estimator.py:808-816)
+        ca = rfield_a.delta_k(ctx, 1, scale=1.0 / vol_per_cell, sub0=NZ_a)
+        cb = rfield_b.delta_k(ctx, 1, scale=1.0 / vol_per_cell, sub0=NZ_b) if correlation_mode == "cross" else None
+        cc = (rfield_c.delta_k(ctx, 1, scale=1.0 / vol_per_cell, sub0=NZ_c)
+              if (correlation_mode == "cross" and tracer_type == "abc") else None)
+        # G_00 = Re c2r(delta_k) of the third leg (:844-851)
+        gsrc = ca if correlation_mode == "auto" else (cc if tracer_type == "abc" else cb)
+        count, ksum, _, _ = ctx.bin(None, None, n, Lbox, prec, k_edge)
+    else:
+        count, ksum = pre["k"]
     k_num = count
     with np.errstate(invalid="ignore", divide="ignore"):
         k_eff = ksum / k_num
-    t1 = _sync_time(ctx)
-    signal = kwargs.get("signal")
+    t1 = _sync_time(ctx) if pre is None else time.perf_counter()
+    signal = kwargs.get("signal") if pre is None else pre["signal"]
     # the contraction runs on the reduced grid m (exact up to rounding)
     m, nmax = contraction_grid(n, Lbox, k_max, contraction, stat_attrs.get("shell_grid", "reduced"))
     grid_factor = (n / m) ** 3
@@ -737,15 +850,18 @@ def calculate_bk_sugi_box(rfield_a, rfield_b, rfield_c, correlation_mode, stat_a
         total_res = grid_factor * _bk_signal(ctx, ca_m, src2_m, G, kinfo, mcfg, tjv, ell_1, ell_2, same_tracer,
                                              contraction, block_size, dvm, shell_method=shell_method, zsynth=zsynth)
         del G, ca_m, src2_m, gsrc_m
-    t2 = _sync_time(ctx)
+    t2 = _sync_time(ctx) if pre is None else time.perf_counter()
     LAST_TIMINGS.update({"prepare_s": t1 - t0, "signal_s": t2 - t1, "contraction": contraction, "grid": m})
 
     # ---------------------------------------------------------- shot noise
     k_center = 0.5 * (k_edge[1:] + k_edge[:-1])
-    k3_edge = np.linspace(2 * k_min, 2 * k_max, k_bins * 2 + 1)[:-1]
+    k3_edge = _k3_edges(k_min, k_max, k_bins)
     k3_center = 0.5 * (k3_edge[1:] + k3_edge[:-1])
     k3_bins = len(k3_center)
-    k_num_k3, ksum3, _, _ = ctx.bin(None, None, n, Lbox, prec, k3_edge)
+    if pre is None:
+        k_num_k3, ksum3, _, _ = ctx.bin(None, None, n, Lbox, prec, k3_edge)
+    else:
+        k_num_k3, ksum3 = pre["k3"]
     with np.errstate(invalid="ignore", divide="ignore"):
         k3_eff = ksum3 / k_num_k3
     p = {"ngp": 1, "cic": 2, "tsc": 3, "pcs": 4}[sampler]
@@ -756,10 +872,14 @@ def calculate_bk_sugi_box(rfield_a, rfield_b, rfield_c, correlation_mode, stat_a
     else:
         SN0 = N_gal_a if (correlation_mode == "auto" and [ell_1, ell_2, L] == [0, 0, 0]) else 0.0
         # P = V^2 a conj(b) - SN0 C_p/sinc^2p, times y_L0 if L != 0, on k3 bins (:1405-1434)
-        ypoly = A.ylm_poly(L, 0, True) if L != 0 else A.make_poly(0, [1.0], A.PART_FULL, 1.0, 1.0)
-        bsrc = ca if correlation_mode == "auto" else cb
-        _, _, re3, _ = ctx.bin(ca, bsrc, n, Lbox, prec, k3_edge, [ypoly], ab_scale=boxsize.prod() ** 2,
-                               noise_window=p, noise_amp=SN0 if correlation_mode == "auto" else 0.0)
+        if pre is None:
+            bsrc = ca if correlation_mode == "auto" else cb
+            _, _, re3, _ = ctx.bin(ca, bsrc, n, Lbox, prec, k3_edge, [_sn_poly(L)], ab_scale=boxsize.prod() ** 2,
+                                   noise_window=p, noise_amp=SN0 if correlation_mode == "auto" else 0.0)
+        else:
+            # bins of V^2 |a|^2 and of the C_p/sinc^2p template, recombined
+            # (the binning is linear in the noise amplitude)
+            re3 = pre["re3"] - SN0 * pre["noise"]
         total_sum = re3[0].astype(np.complex128)
         SN1, SN2 = 0.0, 0.0
         if ell_1 == L and ell_2 == 0:

@@ -78,14 +78,18 @@ def run_task(statistic, correlation_mode, geometry, catalogs, **kwargs):
     ``contraction``, ``fields`` = (mesh attrs, field_a) of an already
     painted box catalog (auto-correlation; used by the batched mock driver),
     ``signal`` = the bk box ``total_res`` already computed by a
-    multi-configuration pass (``estimators.bk_box_signals``) and
-    ``gc_collect`` (default True: the reference's gc pass after the pk
+    multi-configuration pass (``estimators.bk_box_signals``),
+    ``precomputed`` = the device results of the estimator's device pass
+    (``pk_box_device_pass`` / ``bk_box_device_pass``, replayed from a graph
+    by the batched mock driver) and ``gc_collect`` (default True: the reference's gc pass after the pk
     estimator, ``task_executor.py:115-116``)."""
     logging.basicConfig(level=logging.INFO, format="%(asctime)s - %(module)s - %(levelname)s - %(message)s")
     logging.info(f"Running {correlation_mode} {statistic} task with catalogs:{catalogs},\n")
     arrays = kwargs.pop("arrays", None)
     painted = kwargs.pop("fields", None)
     signal = kwargs.pop("signal", None)  # bk box total_res from a multi-configuration pass
+    # device results recorded and replayed by the batched mock driver's graph
+    precomputed = kwargs.pop("precomputed", None)
     collect = kwargs.pop("gc_collect", True)  # the batched mock driver skips the per-task gc pass
     if painted is not None and (geometry != "box-like" or correlation_mode != "auto"):
         raise ValueError("pre-painted fields are supported for box-like auto-correlation only")
@@ -123,7 +127,8 @@ def run_task(statistic, correlation_mode, geometry, catalogs, **kwargs):
         if geometry == "survey-like":
             pk_res = calculate_power_spectrum_survey(stat_attrs, fa, fb, correlation_mode, **kwargs)
         else:
-            pk_res = calculate_power_spectrum_box(fa, fb, correlation_mode, stat_attrs, **kwargs)
+            pk_res = calculate_power_spectrum_box(fa, fb, correlation_mode, stat_attrs, precomputed=precomputed,
+                                                  **kwargs)
         del fa, fb
         if collect:
             gc.collect()
@@ -166,7 +171,8 @@ def run_task(statistic, correlation_mode, geometry, catalogs, **kwargs):
         t1 = time.time()
         logging.info(f"Time to create overdensity field(s): {t1 - t0:.2f} seconds")
         if geometry == "box-like":
-            bk_res = calculate_bk_sugi_box(fa, fb, fc, correlation_mode, stat_attrs, signal=signal, **kwargs)
+            bk_res = calculate_bk_sugi_box(fa, fb, fc, correlation_mode, stat_attrs, signal=signal,
+                                           precomputed=precomputed, **kwargs)
         else:
             bk_res = calculate_bk_sugi_survey(fa, fb, fc, correlation_mode, stat_attrs, **kwargs)
         del fa, fb, fc

@@ -76,18 +76,22 @@ int cnpc_ctx_hold(cnpc_ctx* ctx, int delta);
 /* Paint weighted particles onto mesh0 (and the +0.5-cell shifted copy onto
  * mesh1 when interlaced); every cell of the meshes is overwritten.  window:
  * 1 ngp, 2 cic, 3 tsc, 4 pcs.  boxcenter may be NULL.  For nmesh % 8 == 0
- * (16..2048) the particles are sorted by 8^3 tile and each tile is summed in
- * LDS and written once (no global atomics; work memory grows with np in
- * the context); other meshes use per-particle atomic scatter.
+ * (16..4096) the particles are sorted by tile (16^3 cells when nmesh % 16
+ * == 0, else 8^3) and each tile is summed in LDS in 64-bit fixed point and
+ * written once (no global atomics, bitwise reproducible; work memory grows
+ * with np in the context); other meshes use per-particle atomic scatter.
  * Replaces pm.paint in pm_painter (mesh_generator.py:194-205). */
 int cnpc_paint(cnpc_ctx* ctx, const double* pos_xyz, const double* weight, int64_t np, int nmesh, double boxsize,
                int window, int interlaced, const double* boxcenter, int prec, void* mesh0, void* mesh1);
 
 /* Paint nbatch catalogs at once: catalog b is particles [offsets[b],
  * offsets[b+1]) of the concatenated arrays (offsets: device int64, nbatch +
- * 1 entries; total_count = offsets[nbatch], max_count >= the largest
- * catalog) and lands in mesh b of the mesh0 (and mesh1) batches, nbatch *
- * nmesh^3 values each.  For nmesh % 8 == 0 every catalog takes the sorted,
+ * 1 entries) and lands in mesh b of the mesh0 (and mesh1) batches, nbatch *
+ * nmesh^3 values each.  total_count >= offsets[nbatch] and max_count >= the
+ * largest catalog are upper bounds: they size the launches and the work
+ * memory, while the device offsets decide what is painted, so the call can
+ * be recorded once in a graph and replayed with other catalogs within the
+ * same bounds.  For nmesh % 8 == 0 every catalog takes the sorted,
  * LDS-tiled path of cnpc_paint in the same launches (every cell written
  * once, no global atomics); other meshes are zero-filled and painted by
  * per-particle atomic scatter.  The multi-mock form of pm_painter

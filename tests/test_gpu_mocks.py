@@ -122,3 +122,42 @@ def test_run_mocks_shared_bk_signal(gpu_ctx, tmp_path, precision, tol):
             print(f"{cfg['angu_config']} {key} rel {err:.2e}", flush=True)
             assert err < tol, (cfg["angu_config"], key, err)
         np.testing.assert_array_equal(res["nmodes"], ref["nmodes"])
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("fp32", 1e-5)])
+def test_run_mocks_graph_replay(gpu_ctx, tmp_path, precision, tol):
+    """The graph-recorded chunk pipeline (pk [0,2,4] + bk_sugi [0,0,0] and
+    [2,0,2] diagonal with the analytic shot noise): chunks of 2 mocks, so the
+    first chunk is measured by the warm-up before the recording, the second
+    (a 5000-particle mock past the recorded capacity, beside an empty one)
+    is recorded again, and the third (partial) is a replay.  Every output
+    equals the eager driver's (graph=False): FP64 to 1e-10, FP32 to 1e-5 of
+    max|.| (split-bf16 summation order); the empty mock's pk is NaN-free."""
+    from cosmonpc_amd.mocks import LAST_TIMINGS, run_mocks
+
+    arrays = _arrays(tmp_path)
+    paths = list(arrays)
+    pk, bk = _cfgs(tmp_path / "g")
+    bk = dict(bk, data_vector_mode="diagonal", shotnoise_mode="ana", precision=precision)
+    pk = dict(pk, precision=precision)
+    cfgs = [pk] + [dict(bk, angu_config=a) for a in ([0, 0, 0], [2, 0, 2])]
+    print(f"graph mocks {precision}", flush=True)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        got = run_mocks(cfgs, paths, batch=2, arrays=arrays)
+        assert LAST_TIMINGS["graph"] and LAST_TIMINGS["replayed_chunks_s"] > 0
+        ref = run_mocks([dict(c, output_dir=str(tmp_path / "e")) for c in cfgs], paths, batch=2, arrays=arrays,
+                        graph=False)
+    assert not LAST_TIMINGS["graph"]
+    for path in paths:
+        for g, r in zip(got[path], ref[path]):
+            for key in ("P0", "P2", "P4", "B_sugi", "Bk_raw", "Shot_noise", "k_eff", "nmodes", "I_norm"):
+                if key not in r:
+                    continue
+                a, b = np.asarray(g[key]), np.asarray(r[key])
+                if not np.all(np.isfinite(b)):
+                    np.testing.assert_array_equal(np.isfinite(a), np.isfinite(b))
+                    a, b = np.where(np.isfinite(b), a, 0), np.where(np.isfinite(b), b, 0)
+                if np.any(b != 0):
+                    assert rel_to_max(a, b) < tol, (os.path.basename(path), key)
+                else:
+                    assert np.all(a == 0), (os.path.basename(path), key)
