@@ -428,6 +428,11 @@ __device__ __forceinline__ void fixed_twiddles(C* __restrict__ T1, const C* __re
   }
 }
 
+// Workgroup barrier that orders LDS only: waits for this wave's LDS
+// operations, not for its global loads, so loads issued earlier stay in
+// flight across it (a __syncthreads() fence would wait for them too).
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
 // Both passes over nl lines of pitch lsc (complex); the caller synchronises
 // before (inputs written) and after (outputs read).  qz > 0: the modes qz ..
 // n - qz are zero and not read (whatever the positions hold; for 20 x 20
@@ -482,7 +487,7 @@ __device__ __forceinline__ void fixed_fft_inplace(C* __restrict__ buf, const C* 
 #pragma unroll
     for (int z1 = 1; z1 < N1; ++z1) col[RW * z1] = cmul(u[z1], T1[z1 * N2 + k2]);
   }
-  __syncthreads();
+  lds_barrier();
   for (int t = threadIdx.x; t < nl * N1; t += THREADS) {
     const int z1 = fdiv(t, nl, inv_nl), l = t - z1 * nl;
     C* row = buf + l * lsc + RW * z1;

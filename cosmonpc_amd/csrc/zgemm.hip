@@ -397,20 +397,36 @@ __global__ void __launch_bounds__(THREADS) zgemm_fixed_kernel(ZGemmArgs g) {
   f16x8 fdh, fdl;
   constexpr int rounds = BR ? 1 : 2;
   const int total = BR ? Lh * Q : L * Q;  // fetched items per round
-  for (int64_t tile = blockIdx.x; tile < g.ntiles; tile += gridDim.x) {
-    const int x = (int)(tile / half), yp = (int)(tile - (int64_t)x * half);
-    for (int r0 = 0; r0 < rounds; ++r0) {
-      // every load of the round issued before the first is used: one HBM
-      // latency per round
-      ZgfFetch<BR, MAXF> pf;
-      zgf_issue<THREADS>(pf, g, qa, qb, tile, r0, total, inv_q);
+  // Rounds (tile, r0) in order, tile = blockIdx.x, blockIdx.x + gridDim.x,
+  // ...: the operands of the next round (lines and G rows) are loaded into
+  // registers while this round's FFT and MFMAs run (the barriers order LDS
+  // only, so the loads stay in flight across them).
+  constexpr int GPT = ((BR ? 2 : 1) * n + THREADS - 1) / THREADS;  // G values per thread and round
+  ZgfFetch<BR, MAXF> pf;
+  float gv[GPT];
+  auto issue_round = [&](int64_t tl, int rr) {
+    zgf_issue<THREADS>(pf, g, qa, qb, tl, rr, total, inv_q);
+    const int xx = (int)(tl / half), yy = (int)(tl - (int64_t)xx * half);
+    const float* Grow = g.G + ((int64_t)xx * n + 2 * yy) * n + (BR ? 0 : rr * n);
+#pragma unroll
+    for (int i = 0; i < GPT; ++i) {
+      const int t = threadIdx.x + i * THREADS;
+      gv[i] = t < (BR ? 2 : 1) * n ? Grow[t] : 0.f;
+    }
+  };
+  int64_t tile = blockIdx.x;
+  int r0 = 0;
+  if (tile < g.ntiles) issue_round(tile, 0);
+  while (tile < g.ntiles) {
+    {
       // the round's G row(s) in FFT order: z = z1 + N1 z2 at z2 + RW z1
-      {
-        const float* Grow = g.G + ((int64_t)x * n + 2 * yp) * n + (BR ? 0 : r0 * n);
-        for (int t = threadIdx.x; t < (BR ? 2 : 1) * n; t += THREADS) {
+#pragma unroll
+      for (int i = 0; i < GPT; ++i) {
+        const int t = threadIdx.x + i * THREADS;
+        if (t < (BR ? 2 : 1) * n) {
           const int r = BR ? (t >= n ? 1 : 0) : r0, z = t - (BR && t >= n ? n : 0);
           const int z2 = z / N1, z1 = z - z2 * N1;
-          Gs[r * GLS + z2 + RW * z1] = Grow[t];
+          Gs[r * GLS + z2 + RW * z1] = gv[i];
         }
       }
       // the round's lines from the fetched registers
@@ -445,61 +461,72 @@ __global__ void __launch_bounds__(THREADS) zgemm_fixed_kernel(ZGemmArgs g) {
           }
         }
       }
-      __syncthreads();
-      // the line FFTs, in place (z = z1 + N1 z2 at position z2 + RW z1)
-      // (modes Q .. n - Q are zero: pass 1 does not read their positions)
-      fixed_fft_inplace<N1, N2, THREADS>(buf, T1, L, LSC, Q);
-      __syncthreads();
-      constexpr int nchunk = BR ? 2 * NCH : NCH;
-      for (int c = wq; c < nchunk; c += nwq) {
-        const int r = BR ? (c >= NCH ? 1 : 0) : r0;
-        const int z0 = 32 * (c - ((BR && r) ? NCH : 0)) + 8 * kg;
-        bf16x8 ah[2], al[2], bh[2], bl[2];
-        float vg[8];
-        {
-          const float4* gp = (const float4*)(Gs + r * GLS + z0);
-          const float4 g0 = gp[0], g1 = gp[1];
-          vg[0] = g0.x; vg[1] = g0.y; vg[2] = g0.z; vg[3] = g0.w;
-          vg[4] = g1.x; vg[5] = g1.y; vg[6] = g1.z; vg[7] = g1.w;
-        }
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-          const int i = 16 * h + rr;
-          float re[8], im[8], a[8], b[8];
-          if constexpr (BR) {  // row i: part i & 1 of line r Lh + i / 2 (rows past the lines: the zero line L)
-            const int line = (i >> 1) < Lh ? r * Lh + (i >> 1) : L;
-            zgf_row8(buf + line * LSC + z0, re, im);
-#pragma unroll
-            for (int j = 0; j < 8; ++j) {
-              a[j] = (i & 1) ? im[j] : re[j];
-              b[j] = a[j] * vg[j];
-            }
-          } else {  // line i: A_i real, B_i imaginary
-            zgf_row8(buf + i * LSC + z0, re, im);
-#pragma unroll
-            for (int j = 0; j < 8; ++j) {
-              a[j] = wbb ? im[j] : re[j];
-              b[j] = im[j] * vg[j];
-            }
-          }
-          split8<false>(a, ah[h], al[h], fdh, fdl);
-          split8<false>(b, bh[h], bl[h], fdh, fdl);
-        }
-#pragma unroll
-        for (int i = 0; i < 2; ++i)
-#pragma unroll
-          for (int j = 0; j < 2; ++j) {
-            f32x4 cacc = {acc[i][j][0], acc[i][j][1], acc[i][j][2], acc[i][j][3]};
-            cacc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], cacc, 0, 0, 0);
-            cacc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], cacc, 0, 0, 0);
-            cacc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], cacc, 0, 0, 0);
-#pragma unroll
-            for (int q = 0; q < 4; ++q) acc[i][j][q] = cacc[q];
-          }
-      }
-      __syncthreads();  // the next round overwrites the lines and G
     }
+    // the next round's loads
+    int64_t ntile = tile;
+    int nr0 = r0 + 1;
+    if (nr0 == rounds) {
+      nr0 = 0;
+      ntile += gridDim.x;
+    }
+    if (ntile < g.ntiles) issue_round(ntile, nr0);
+    lds_barrier();
+    // the line FFTs, in place (z = z1 + N1 z2 at position z2 + RW z1)
+    // (modes Q .. n - Q are zero: pass 1 does not read their positions)
+    fixed_fft_inplace<N1, N2, THREADS>(buf, T1, L, LSC, Q);
+    lds_barrier();
+    constexpr int nchunk = BR ? 2 * NCH : NCH;
+    for (int c = wq; c < nchunk; c += nwq) {
+      const int r = BR ? (c >= NCH ? 1 : 0) : r0;
+      const int z0 = 32 * (c - ((BR && r) ? NCH : 0)) + 8 * kg;
+      bf16x8 ah[2], al[2], bh[2], bl[2];
+      float vg[8];
+      {
+        const float4* gp = (const float4*)(Gs + r * GLS + z0);
+        const float4 g0 = gp[0], g1 = gp[1];
+        vg[0] = g0.x; vg[1] = g0.y; vg[2] = g0.z; vg[3] = g0.w;
+        vg[4] = g1.x; vg[5] = g1.y; vg[6] = g1.z; vg[7] = g1.w;
+      }
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int i = 16 * h + rr;
+        float re[8], im[8], a[8], b[8];
+        if constexpr (BR) {  // row i: part i & 1 of line r Lh + i / 2 (rows past the lines: the zero line L)
+          const int line = (i >> 1) < Lh ? r * Lh + (i >> 1) : L;
+          zgf_row8(buf + line * LSC + z0, re, im);
+#pragma unroll
+          for (int j = 0; j < 8; ++j) {
+            a[j] = (i & 1) ? im[j] : re[j];
+            b[j] = a[j] * vg[j];
+          }
+        } else {  // line i: A_i real, B_i imaginary
+          zgf_row8(buf + i * LSC + z0, re, im);
+#pragma unroll
+          for (int j = 0; j < 8; ++j) {
+            a[j] = wbb ? im[j] : re[j];
+            b[j] = im[j] * vg[j];
+          }
+        }
+        split8<false>(a, ah[h], al[h], fdh, fdl);
+        split8<false>(b, bh[h], bl[h], fdh, fdl);
+      }
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          f32x4 cacc = {acc[i][j][0], acc[i][j][1], acc[i][j][2], acc[i][j][3]};
+          cacc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], cacc, 0, 0, 0);
+          cacc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], cacc, 0, 0, 0);
+          cacc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], cacc, 0, 0, 0);
+#pragma unroll
+          for (int q = 0; q < 4; ++q) acc[i][j][q] = cacc[q];
+        }
+    }
+    lds_barrier();  // the next round overwrites the lines and G
+    tile = ntile;
+    r0 = nr0;
   }
+  __syncthreads();
   if (BB) {
     block_reduce_store(acc, g.partial + (int64_t)blockIdx.x * 1024, 0, nw / 2);
     block_reduce_store(acc, g.partial_bb + (int64_t)blockIdx.x * 1024, nw / 2, nw);
