@@ -331,20 +331,44 @@ __global__ void __launch_bounds__(THREADS) xfft_fixed_kernel(SynthArgs a, const 
   float2* buf = T1 + n;
   fixed_twiddles<N1, N2>(T1, twg);
   const int64_t lbase = ((int64_t)kz * P + ky0) * P;
-  for (int s = 0; s < a.nshell; ++s) {
-    const signed char want = (signed char)(a.bin0 + s);
-    // lines with kz or every ky of the block beyond the shell's radius are
-    // zero for this shell: skipped (the y pass never reads them)
+  // lines with kz or every ky of the block beyond a shell's radius are zero
+  // for that shell: skipped (the y pass never reads them)
+  auto live = [&](int s) {
     const int rs = a.rs[s];
-    if (kz > rs || ky0 - R > rs || ky0 + nl - 1 - R < -rs) continue;
-    fixed_load_lines<N1, N2, THREADS, NL>(buf, nl, [&](int l, int f) {
-      float2 v = make_float2(0.f, 0.f);
-      if (f >= -R && f <= R) {
+    return !(kz > rs || ky0 - R > rs || ky0 + nl - 1 - R < -rs);
+  };
+  bool any = false;
+  for (int s = 0; s < a.nshell; ++s) any = any || live(s);
+  if (!any) return;
+  // the block's cube lines (values and bins) are read once, into registers,
+  // and masked to each shell from there
+  constexpr int KPT = (n + THREADS - 1) / THREADS;
+  float2 cv[NL][KPT];
+  int cb[NL][KPT];
+#pragma unroll
+  for (int l = 0; l < NL; ++l)
+#pragma unroll
+    for (int i = 0; i < KPT; ++i) {
+      const int k = threadIdx.x + i * THREADS;
+      const int f = k < (n >> 1) ? k : k - n;
+      cv[l][i] = make_float2(0.f, 0.f);
+      cb[l][i] = -1;
+      if (l < nl && k < n && f >= -R && f <= R) {
         const int64_t j = lbase + (int64_t)l * P + f + R;
-        if (bins[j] == want) v = V[j];
+        cb[l][i] = bins[j];
+        cv[l][i] = V[j];
       }
-      return v;
-    });
+    }
+  for (int s = 0; s < a.nshell; ++s) {
+    if (!live(s)) continue;
+    const int want = a.bin0 + s;
+#pragma unroll
+    for (int l = 0; l < NL; ++l)
+#pragma unroll
+      for (int i = 0; i < KPT; ++i) {
+        const int k = threadIdx.x + i * THREADS;
+        if (l < nl && k < n) buf[l * LSC + F::ipos(k)] = cb[l][i] == want ? cv[l][i] : make_float2(0.f, 0.f);
+      }
     __syncthreads();
     fixed_fft_inplace<N1, N2, THREADS>(buf, T1, nl, LSC, R + 1);
     __syncthreads();
