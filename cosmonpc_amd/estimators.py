@@ -135,27 +135,32 @@ class ShellBuilder:
         self._tuned = False
 
     def _tune_batch(self, delta, poly, bins, out, slot0):
-        """Shells per x/y synthesis launch: 4 or 8, timed per shell on the
-        first run of at least 8 bins (the autotuner caches the choice per
-        grid, radius and bin count)."""
+        """Shells per x/y synthesis launch: 4, 8 or 16 (as many as the bins
+        allow), each timed on the whole run of ``bins`` in launches of that
+        size, on the first run of at least 8 bins (the autotuner caches the
+        choice per grid, radius and bin count; the x pass reads the mode
+        cube once per launch)."""
         self._tuned = True
         if len(bins) < 8:
             return
         from . import autotune
 
         shape = (self.n, int(math.ceil(self.edges[-1] / (2 * math.pi / self.L))), len(self.edges) - 1)
+        cands = [c for c in (4, 8, 16) if c <= len(bins)]
         best = autotune.lookup("synth_half_batch", shape)
-        work8 = None
+        work = None
         if best is None:
-            work8 = self.ctx.spectrum(self.n, 1, 8)
+            work = self.ctx.spectrum(self.n, 1, cands[-1])
 
             def run(c):
-                self.ctx.shell_synth_half(delta, self.n, self.L, self.edges, bins[0], c, poly, work8, out, slot0)
+                for d in range(0, len(bins), c):
+                    self.ctx.shell_synth_half(delta, self.n, self.L, self.edges, bins[0] + d, min(c, len(bins) - d),
+                                              poly, work, out, slot0 + d, reuse_cube=d > 0)
 
-            best = autotune.choose("synth_half_batch", shape, [4, 8], run, units=lambda c: c)
+            best = autotune.choose("synth_half_batch", shape, cands, run)
         if best != self.batch:
             self.batch = best
-            self.work = work8 if (best == 8 and work8 is not None) else self.ctx.spectrum(self.n, self.prec, best)
+            self.work = work if work is not None else self.ctx.spectrum(self.n, self.prec, best)
 
     def build(self, delta, poly, bins, out):
         """out[r] = shell of bins[r] (bins must be consecutive)."""
