@@ -60,19 +60,45 @@ __device__ __forceinline__ void block_reduce_store64(const double (&acc)[2][2][4
   __syncthreads();
 }
 
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2v __attribute__((ext_vector_type(2)));
+typedef unsigned u32x4v __attribute__((ext_vector_type(4)));
+
+// two FP32 values rounded to nearest-even bf16 in one register (element 0 in
+// the low half): one v_cvt_pk_bf16_f32
+__device__ __forceinline__ unsigned pack_bf16x2(float a, float b) {
+  const f32x2v v = {a, b};
+  return __builtin_bit_cast(unsigned, __builtin_convertvector(v, bf16x2));
+}
+
+// hi = bf16(v), lo = bf16(v - hi) for 8 values, two per conversion: the FP32
+// value of each hi comes from its packed bits by a shift or a mask, so a
+// value costs half a conversion for hi, one bit operation, one subtraction
+// and half a conversion for lo.
+__device__ __forceinline__ void split8_bf16(const float (&v)[8], bf16x8& hi, bf16x8& lo) {
+  u32x4v h, l;
+#pragma unroll
+  for (int p = 0; p < 4; ++p) {
+    const unsigned hp = pack_bf16x2(v[2 * p], v[2 * p + 1]);
+    const float h0 = __uint_as_float(hp << 16), h1 = __uint_as_float(hp & 0xffff0000u);
+    h[p] = hp;
+    l[p] = pack_bf16x2(v[2 * p] - h0, v[2 * p + 1] - h1);
+  }
+  hi = __builtin_bit_cast(bf16x8, h);
+  lo = __builtin_bit_cast(bf16x8, l);
+}
+
 template <bool F16>
 __device__ __forceinline__ void split8(const float (&v)[8], bf16x8& hi, bf16x8& lo, f16x8& hh, f16x8& ll) {
+  if (F16) {
 #pragma unroll
-  for (int j = 0; j < 8; ++j) {
-    if (F16) {
+    for (int j = 0; j < 8; ++j) {
       _Float16 h = (_Float16)v[j];
       hh[j] = h;
       ll[j] = (_Float16)(v[j] - (float)h);
-    } else {
-      __bf16 h = (__bf16)v[j];
-      hi[j] = h;
-      lo[j] = (__bf16)(v[j] - (float)h);
     }
+  } else {
+    split8_bf16(v, hi, lo);
   }
 }
 

@@ -381,20 +381,16 @@ __global__ void __launch_bounds__(THREADS) zgemm_fixed_kernel(ZGemmArgs g) {
   __syncthreads();
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   constexpr int nw = THREADS / 64;
-  const bool wbb = BB && w >= (nw / 2);
-  const int wq = BB ? (wbb ? w - nw / 2 : w) : w;
-  constexpr int nwq = BB ? nw / 2 : nw;
   const int rr = lane & 15, kg = lane >> 4;
   const int Lh = (g.na + 1) >> 1;
   const float inv_q = 1.0f / (float)Q;
-  float acc[2][2][4];
+  float acc[2][2][4], acc2[2][2][4];  // acc2: B^T G B (BB only)
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j)
 #pragma unroll
-      for (int q = 0; q < 4; ++q) acc[i][j][q] = 0.f;
-  f16x8 fdh, fdl;
+      for (int q = 0; q < 4; ++q) acc[i][j][q] = acc2[i][j][q] = 0.f;
   constexpr int rounds = BR ? 1 : 2;
   const int total = BR ? Lh * Q : L * Q;  // fetched items per round
   // Rounds (tile, r0) in order, tile = blockIdx.x, blockIdx.x + gridDim.x,
@@ -476,7 +472,51 @@ __global__ void __launch_bounds__(THREADS) zgemm_fixed_kernel(ZGemmArgs g) {
     fixed_fft_inplace<N1, N2, THREADS>(buf, T1, L, LSC, Q);
     lds_barrier();
     constexpr int nchunk = BR ? 2 * NCH : NCH;
-    for (int c = wq; c < nchunk; c += nwq) {
+    if constexpr (BB) {
+      // every wave takes chunks c = w, w + nw, ... and feeds both outputs:
+      // line i holds A_i (real part) and B_i (imaginary part); B_j G is
+      // split once for both, A_i and B_i once each
+      for (int c = w; c < nchunk; c += nw) {
+        const int z0 = 32 * c + 8 * kg;
+        float vg[8];
+        {
+          const float4* gp = (const float4*)(Gs + r0 * GLS + z0);
+          const float4 g0 = gp[0], g1 = gp[1];
+          vg[0] = g0.x; vg[1] = g0.y; vg[2] = g0.z; vg[3] = g0.w;
+          vg[4] = g1.x; vg[5] = g1.y; vg[6] = g1.z; vg[7] = g1.w;
+        }
+        bf16x8 gh[2], gl[2], ah[2], al[2], bh[2], bl[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          float re[8], im[8], b[8];
+          zgf_row8(buf + (16 * h + rr) * LSC + z0, re, im);
+#pragma unroll
+          for (int j = 0; j < 8; ++j) b[j] = im[j] * vg[j];
+          split8_bf16(b, gh[h], gl[h]);
+          split8_bf16(re, ah[h], al[h]);
+          split8_bf16(im, bh[h], bl[h]);
+        }
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int j = 0; j < 2; ++j) {
+            f32x4 c1 = {acc[i][j][0], acc[i][j][1], acc[i][j][2], acc[i][j][3]};
+            f32x4 c2 = {acc2[i][j][0], acc2[i][j][1], acc2[i][j][2], acc2[i][j][3]};
+            c1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], gh[j], c1, 0, 0, 0);
+            c2 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[i], gh[j], c2, 0, 0, 0);
+            c1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], gl[j], c1, 0, 0, 0);
+            c2 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bh[i], gl[j], c2, 0, 0, 0);
+            c1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], gh[j], c1, 0, 0, 0);
+            c2 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl[i], gh[j], c2, 0, 0, 0);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+              acc[i][j][q] = c1[q];
+              acc2[i][j][q] = c2[q];
+            }
+          }
+      }
+    } else {
+    for (int c = w; c < nchunk; c += nw) {
       const int r = BR ? (c >= NCH ? 1 : 0) : r0;
       const int z0 = 32 * (c - ((BR && r) ? NCH : 0)) + 8 * kg;
       bf16x8 ah[2], al[2], bh[2], bl[2];
@@ -503,12 +543,12 @@ __global__ void __launch_bounds__(THREADS) zgemm_fixed_kernel(ZGemmArgs g) {
           zgf_row8(buf + i * LSC + z0, re, im);
 #pragma unroll
           for (int j = 0; j < 8; ++j) {
-            a[j] = wbb ? im[j] : re[j];
+            a[j] = re[j];
             b[j] = im[j] * vg[j];
           }
         }
-        split8<false>(a, ah[h], al[h], fdh, fdl);
-        split8<false>(b, bh[h], bl[h], fdh, fdl);
+        split8_bf16(a, ah[h], al[h]);
+        split8_bf16(b, bh[h], bl[h]);
       }
 #pragma unroll
       for (int i = 0; i < 2; ++i)
@@ -522,14 +562,15 @@ __global__ void __launch_bounds__(THREADS) zgemm_fixed_kernel(ZGemmArgs g) {
           for (int q = 0; q < 4; ++q) acc[i][j][q] = cacc[q];
         }
     }
+    }
     lds_barrier();  // the next round overwrites the lines and G
     tile = ntile;
     r0 = nr0;
   }
   __syncthreads();
   if (BB) {
-    block_reduce_store(acc, g.partial + (int64_t)blockIdx.x * 1024, 0, nw / 2);
-    block_reduce_store(acc, g.partial_bb + (int64_t)blockIdx.x * 1024, nw / 2, nw);
+    block_reduce_store(acc, g.partial + (int64_t)blockIdx.x * 1024);
+    block_reduce_store(acc2, g.partial_bb + (int64_t)blockIdx.x * 1024);
   } else {
     block_reduce_store(acc, g.partial + (int64_t)blockIdx.x * 1024);
   }
