@@ -518,8 +518,9 @@ int cnpc_shell_synth_half(cnpc_ctx* ctx, int n, double boxsize, const void* D, c
   merge_radices(a);
   REQUIRE(synth_lines_per_block(1, a) >= 1, "shell_synth_half: mesh too large for the LDS line buffers");
   const size_t P = 2 * (size_t)R + 1, Q = (size_t)R + 1, cube = P * Q * P;
-  // work holds U (nshell n P Q complex), at most nshell half spectra
-  REQUIRE(P * Q <= (size_t)n * (n / 2 + 1), "shell_synth_half: k_max too large for the work area");
+  // work holds U (nshell n P Q complex; P rounded up to whole blocks of 16
+  // lines), at most nshell half spectra
+  REQUIRE((P + 15) / 16 * 16 * Q <= (size_t)n * (n / 2 + 1), "shell_synth_half: k_max too large for the work area");
   const size_t vbytes = (cube * 8 + 255) / 256 * 256, bbytes = (cube + 255) / 256 * 256;
   REQUIRE(!reuse_cube || ctx->synth_bytes >= vbytes + bbytes + (size_t)n * 8,
           "shell_synth_half: reuse_cube without a previous cube of this shape");

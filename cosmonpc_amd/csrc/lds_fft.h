@@ -437,10 +437,13 @@ __device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(
 // before (inputs written) and after (outputs read).  qz > 0: the modes qz ..
 // n - qz are zero and not read (whatever the positions hold; for 20 x 20
 // plans with qz <= n / 4 the first pass's 20-point DFTs also skip their
-// arithmetic); qz <= 0: every mode is read.  Thread t takes line t % nl of butterfly t / nl: lanes of a
+// arithmetic); qz <= 0: every mode is read.  BANDZ (20 x 20, qz <= n / 4):
+// the caller has written zeros at the modes qz .. n/4 - 1 and 3n/4 .. n - qz,
+// so the first pass reads its ten band inputs (k1 = 0 .. 4, 15 .. 19)
+// without per-mode conditions.  Thread t takes line t % nl of butterfly t / nl: lanes of a
 // wave walk the lines first, whose pitch (lsc = 6 mod 32 for the fused
 // contraction's lines) spreads them over the LDS banks.
-template <int N1, int N2, int THREADS, typename C>
+template <int N1, int N2, int THREADS, bool BANDZ = false, typename C>
 __device__ __forceinline__ void fixed_fft_inplace(C* __restrict__ buf, const C* __restrict__ T1, int nl, int lsc,
                                                   int qz = 0) {
   constexpr int RW = N2 + 1, n = N1 * N2;
@@ -460,7 +463,7 @@ __device__ __forceinline__ void fixed_fft_inplace(C* __restrict__ buf, const C* 
           if (k1 >= 5 && k1 <= 14) {
             u[k1].x = 0;
             u[k1].y = 0;
-          } else if (k >= qz && k <= n - qz) {
+          } else if (!BANDZ && k >= qz && k <= n - qz) {
             u[k1].x = 0;
             u[k1].y = 0;
           } else {

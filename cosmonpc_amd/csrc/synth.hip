@@ -10,7 +10,9 @@
 //                  0 <= n_z <= R, plus the bin of every mode; shared by all
 //                  shells of the call;
 //  2. xfft_kernel  x-FFT (length N) of every (kz, ky) line, masked to shell
-//                  s: U[s][x][kz][ky];
+//                  s: U[s][x][kz][ky] (the fixed 20 x 20 plans write
+//                  U[s][kz][ky block][x][ky % block], one contiguous run per
+//                  workgroup);
 //  3. yfft_kernel  y-FFT of the P non-zero ky of every (s, x, kz) line
 //                  (contiguous in U): T[s][x][kz][y];
 //  4. zfft_kernel  z-c2r of the Q non-zero kz of every (s, x, y) row, two rows
@@ -372,18 +374,19 @@ __global__ void __launch_bounds__(THREADS) xfft_fixed_kernel(SynthArgs a, const 
     __syncthreads();
     fixed_fft_inplace<N1, N2, THREADS>(buf, T1, nl, LSC, R + 1);
     __syncthreads();
-    // U[s][x][kz][ky]: consecutive threads take consecutive ky of one x
-    float2* dst = U + ((int64_t)s * n * Q + kz) * P + ky0;
+    // U[s][kz][ky block][x][ky % NL] (the fixed plans' layout): one
+    // contiguous run of n NL entries per workgroup and shell
+    float2* dst = U + (((int64_t)s * Q + kz) * gridDim.x + blockIdx.x) * n * NL;
     for (int t = threadIdx.x; t < NL * n; t += THREADS) {
       const int x = t / NL, l = t - x * NL;
-      if (l < nl) dst[(int64_t)x * Q * P + l] = buf[l * LSC + F::opos(x)];
+      if (l < nl) dst[t] = buf[l * LSC + F::opos(x)];
     }
     __syncthreads();
   }
 }
 
 // y pass into the half-synthesised layout H[x][y/2][slot0 + s][kz][y%2]
-template <int N1, int N2, int THREADS, int NL>
+template <int N1, int N2, int THREADS, int NL, int NLX>
 __global__ void __launch_bounds__(THREADS) yfft_fixed_kernel(SynthArgs a, const float2* __restrict__ U,
                                                              const float2* __restrict__ twg, float2* __restrict__ H,
                                                              int nslots, int slot0) {
@@ -403,9 +406,15 @@ __global__ void __launch_bounds__(THREADS) yfft_fixed_kernel(SynthArgs a, const 
   float2* T1 = (float2*)smem;
   float2* buf = T1 + n;
   fixed_twiddles<N1, N2>(T1, twg);
-  const float2* src = U + (plane * Q + kz0) * P;
+  // line l: U[s][kz0 + l][ky / NLX][x][ky % NLX] (the x pass's layout, NLX
+  // lines per block)
+  const int nbx = (P + NLX - 1) / NLX;
+  const float2* src = U + (((int64_t)s * Q + kz0) * nbx * n + x) * NLX;
   fixed_load_lines<N1, N2, THREADS, NL>(buf, nl, [&](int l, int f) {
-    return (f >= -rs && f <= rs && kz0 + l <= rs) ? src[l * P + f + R] : make_float2(0.f, 0.f);
+    const int ky = f + R;
+    return (f >= -rs && f <= rs && kz0 + l <= rs)
+               ? src[((int64_t)l * nbx + ky / NLX) * n * NLX + ky % NLX]
+               : make_float2(0.f, 0.f);
   });
   __syncthreads();
   fixed_fft_inplace<N1, N2, THREADS>(buf, T1, nl, LSC, rs + 1);
@@ -461,6 +470,13 @@ int synth_lines_per_block(int prec, const SynthArgs& a) {
 #endif
 #ifndef CNPC_SYNTH_FIXED_THREADS
 #define CNPC_SYNTH_FIXED_THREADS 256
+#endif
+// the x pass's own shape (lines per workgroup, threads)
+#ifndef CNPC_SYNTH_FIXED_NL_X
+#define CNPC_SYNTH_FIXED_NL_X CNPC_SYNTH_FIXED_NL
+#endif
+#ifndef CNPC_SYNTH_FIXED_THREADS_X
+#define CNPC_SYNTH_FIXED_THREADS_X CNPC_SYNTH_FIXED_THREADS
 #endif
 
 template <typename T>
@@ -518,10 +534,12 @@ hipError_t launch_synth_half(const SynthArgs& a, const BinEdges& E, const PolySp
   const dim3 fblk(SYNTH_THREADS);
   if (n == 400) {  // fixed plan 400 = 20 x 20
     constexpr int NL = CNPC_SYNTH_FIXED_NL, FT = CNPC_SYNTH_FIXED_THREADS;
+    constexpr int NLX = CNPC_SYNTH_FIXED_NL_X, FTX = CNPC_SYNTH_FIXED_THREADS_X;
+    const size_t ldsx = sizeof(C) * ((size_t)n + (size_t)NLX * SynthFixed<20, 20>::LSC);
     const size_t lds = sizeof(C) * ((size_t)n + (size_t)NL * SynthFixed<20, 20>::LSC);
-    hipLaunchKernelGGL((xfft_fixed_kernel<20, 20, FT, NL>), dim3((P + NL - 1) / NL, Q), dim3(FT), lds, st, a,
+    hipLaunchKernelGGL((xfft_fixed_kernel<20, 20, FTX, NLX>), dim3((P + NLX - 1) / NLX, Q), dim3(FTX), ldsx, st, a,
                        (const C*)V, bins, (const C*)tw, (C*)work);
-    hipLaunchKernelGGL((yfft_fixed_kernel<20, 20, FT, NL>), dim3((Q + NL - 1) / NL, (unsigned)planes), dim3(FT),
+    hipLaunchKernelGGL((yfft_fixed_kernel<20, 20, FT, NL, NLX>), dim3((Q + NL - 1) / NL, (unsigned)planes), dim3(FT),
                        lds, st, a, (const C*)work, (const C*)tw, (C*)out, nslots, slot0);
     return hipGetLastError();
   }

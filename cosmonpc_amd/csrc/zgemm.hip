@@ -60,6 +60,7 @@ constexpr int ZG_TAIL_PAD = 32;
 
 struct ZGemmArgs {
   int n, Q;                 // grid, non-zero kz per line (R + 1)
+  int Qi;                   // fixed plan: fetched modes per line (Q, or n/4 + 1 when 4Q <= n)
   int na, nb;               // rows of A and B (<= 32)
   int sa_slots, sb_slots;   // slots per tile in HA / HB
   int same;                 // B is A
@@ -316,43 +317,76 @@ __device__ __forceinline__ void zgf_row8(const float2* p, float (&re)[8], float 
   }
 }
 
-// The operands of one round, fetched into registers (MAXF items per thread,
-// item t = threadIdx.x + it * THREADS -> line l = t / Q, mode kz = t % Q):
-// B == A, the float4 of shells 2l and 2l + 1 (both rows of the tile); B !=
-// A, one row's (re, im) of A_l and B_l.
+// The operands of one round are fetched into registers, MAXF items per
+// thread: item t = threadIdx.x + it * THREADS is line l = t / Q, mode kz =
+// t % Q.  B == A: the float4 of shells 2l and 2l + 1 (both rows of the
+// tile); B != A: one row's (re, im) of A_l and B_l.  What an item reads and
+// where it lands in LDS is the same every round, so it is computed once:
+//   lds word: float2 index of mode kz in its line (bits 0-13), of mode n - kz
+//     (bits 16-29; for kz = 0 the pad column of the line's first row, which
+//     the passes never read) and kz == 0 (bit 31); items past the round's
+//     lines land on the pad column of line 0 (they carry zeros);
+//   fetch word: byte offset of the item in the tile's run of HA (and HB)
+//     (bits 0-16), bit 30 set when the A slot holds no row kz, bit 31 when
+//     the B slot holds none: masking with ZG_A_MASK / ZG_B_MASK leaves the
+//     offset, or an offset beyond the run, which the buffer load answers
+//     with zeros.
 template <bool BR, int MAXF>
 struct ZgfFetch {
   typedef typename std::conditional<BR, float4, float2>::type T;
   T a[MAXF], b[MAXF];
 };
+constexpr unsigned ZG_A_MASK = 0x4001ffffu, ZG_B_MASK = 0x8001ffffu;
+constexpr int ZG_BUF_FLAGS = 0x00020000;  // buffer resource word 3 (gfx9 raw buffer)
 
-template <int THREADS, bool BR, int MAXF>
-__device__ __forceinline__ void zgf_issue(ZgfFetch<BR, MAXF>& f, const ZGemmArgs& g, const int* qa, const int* qb,
-                                          int64_t tile, int r, int total, float inv_q) {
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t zg_rsrc(const void* base, int bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, bytes, ZG_BUF_FLAGS);
+}
+
+template <int THREADS, bool BR, int MAXF, int N1, int N2>
+__device__ __forceinline__ void zgf_items(const ZGemmArgs& g, const int* qa, const int* qb, int Qi, int total,
+                                          unsigned (&lw)[MAXF], unsigned (&fw)[MAXF]) {
+  typedef FixedPlan<N1, N2> P;
+  constexpr int n = N1 * N2, LSC = P::KP + 2;
   const int Q = g.Q;
-  const float4* TA = g.HA + tile * g.sa_slots * Q;
-  const float4* TB = g.HB + tile * g.sb_slots * Q;
 #pragma unroll
   for (int it = 0; it < MAXF; ++it) {
     const int t = threadIdx.x + it * THREADS;
-    const int l = fdiv(t, Q, inv_q), kz = t - l * Q;
-    if constexpr (BR) {
-      float4 a = make_float4(0.f, 0.f, 0.f, 0.f), b = a;
-      if (t < total) {
-        const int s0 = 2 * l, s1 = s0 + 1;
-        if (s0 < g.na && kz < qa[s0]) a = TA[s0 * Q + kz];
-        if (s1 < g.na && kz < qa[s1]) b = TA[s1 * Q + kz];
-      }
-      f.a[it] = a;
-      f.b[it] = b;
+    const int l = t / Qi, kz = t - l * Qi;
+    const bool ok = t < total;
+    const unsigned l1 = ok ? (unsigned)(l * LSC + P::ipos(kz)) : (unsigned)N2;
+    const unsigned l2 = ok ? (unsigned)(l * LSC + (kz ? P::ipos(n - kz) : N2)) : (unsigned)N2;
+    lw[it] = l1 | (l2 << 16) | (ok && kz == 0 ? 0x80000000u : 0u);
+    bool va, vb;
+    int slot;
+    if constexpr (BR) {  // A: shell 2l, "B": shell 2l + 1, both in HA
+      slot = 2 * l;
+      va = ok && 2 * l < g.na && kz < qa[2 * l];
+      vb = ok && 2 * l + 1 < g.na && kz < qa[2 * l + 1];
     } else {
-      float2 a = make_float2(0.f, 0.f), b = a;
-      if (t < total) {
-        if (l < g.na && kz < qa[l]) a = ((const float2*)(TA + l * Q + kz))[r];
-        if (l < g.nb && kz < qb[l]) b = ((const float2*)(TB + l * Q + kz))[r];
-      }
-      f.a[it] = a;
-      f.b[it] = b;
+      slot = l;
+      va = ok && l < g.na && kz < qa[l];
+      vb = ok && l < g.nb && kz < qb[l];
+    }
+    fw[it] = (ok ? (unsigned)((slot * Q + kz) * 16) : 0u) | (va ? 0u : 0x40000000u) | (vb ? 0u : 0x80000000u);
+  }
+}
+
+template <int THREADS, bool BR, int MAXF>
+__device__ __forceinline__ void zgf_issue(ZgfFetch<BR, MAXF>& f, const ZGemmArgs& g, const unsigned (&fw)[MAXF],
+                                          int64_t tile, int r) {
+  const int Q = g.Q;
+  const __amdgpu_buffer_rsrc_t ra = zg_rsrc(g.HA + tile * g.sa_slots * Q, g.sa_slots * Q * 16);
+  const __amdgpu_buffer_rsrc_t rb = zg_rsrc(g.HB + tile * g.sb_slots * Q, g.sb_slots * Q * 16);
+#pragma unroll
+  for (int it = 0; it < MAXF; ++it) {
+    if constexpr (BR) {
+      // shells 2l and 2l + 1 sit Q float4 apart
+      f.a[it] = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(ra, (int)(fw[it] & ZG_A_MASK), 0, 0));
+      f.b[it] = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(ra, (int)(fw[it] & ZG_B_MASK), Q * 16, 0));
+    } else {
+      f.a[it] = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(ra, (int)(fw[it] & ZG_A_MASK), 8 * r, 0));
+      f.b[it] = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(rb, (int)(fw[it] & ZG_B_MASK), 8 * r, 0));
     }
   }
 }
@@ -383,7 +417,6 @@ __global__ void __launch_bounds__(THREADS) zgemm_fixed_kernel(ZGemmArgs g) {
   constexpr int nw = THREADS / 64;
   const int rr = lane & 15, kg = lane >> 4;
   const int Lh = (g.na + 1) >> 1;
-  const float inv_q = 1.0f / (float)Q;
   float acc[2][2][4], acc2[2][2][4];  // acc2: B^T G B (BB only)
 #pragma unroll
   for (int i = 0; i < 2; ++i)
@@ -392,16 +425,22 @@ __global__ void __launch_bounds__(THREADS) zgemm_fixed_kernel(ZGemmArgs g) {
 #pragma unroll
       for (int q = 0; q < 4; ++q) acc[i][j][q] = acc2[i][j][q] = 0.f;
   constexpr int rounds = BR ? 1 : 2;
-  const int total = BR ? Lh * Q : L * Q;  // fetched items per round
+  // items per line: the modes below n/4 when the band fits (zeros past each
+  // shell's radius, which the first FFT pass then reads unconditionally),
+  // else the Q rows
+  const int Qi = g.Qi;
+  const int total = BR ? Lh * Qi : L * Qi;  // fetched items per round
   // Rounds (tile, r0) in order, tile = blockIdx.x, blockIdx.x + gridDim.x,
   // ...: the operands of the next round (lines and G rows) are loaded into
   // registers while this round's FFT and MFMAs run (the barriers order LDS
   // only, so the loads stay in flight across them).
   constexpr int GPT = ((BR ? 2 : 1) * n + THREADS - 1) / THREADS;  // G values per thread and round
+  unsigned lw[MAXF], fw[MAXF];
+  zgf_items<THREADS, BR, MAXF, N1, N2>(g, qa, qb, Qi, total, lw, fw);
   ZgfFetch<BR, MAXF> pf;
   float gv[GPT];
   auto issue_round = [&](int64_t tl, int rr) {
-    zgf_issue<THREADS>(pf, g, qa, qb, tl, rr, total, inv_q);
+    zgf_issue<THREADS, BR, MAXF>(pf, g, fw, tl, rr);
     const int xx = (int)(tl / half), yy = (int)(tl - (int64_t)xx * half);
     const float* Grow = g.G + ((int64_t)xx * n + 2 * yy) * n + (BR ? 0 : rr * n);
 #pragma unroll
@@ -425,36 +464,30 @@ __global__ void __launch_bounds__(THREADS) zgemm_fixed_kernel(ZGemmArgs g) {
           Gs[r * GLS + z2 + RW * z1] = gv[i];
         }
       }
-      // the round's lines from the fetched registers
+      // the round's lines from the fetched registers: mode kz = p + i q
+      // and mode n - kz = conj(p) + i conj(q) of the line (the zero mode
+      // keeps the real parts only)
 #pragma unroll
       for (int it = 0; it < MAXF; ++it) {
-        const int t = threadIdx.x + it * THREADS;
-        if (t < total) {
-          const int l = fdiv(t, Q, inv_q), kz = t - l * Q;
-          if constexpr (BR) {
-            const float4 a = pf.a[it], b = pf.b[it];
+        const unsigned wd = lw[it];
+        const bool z0 = (int)wd < 0;
+        float2* L1 = buf + (wd & 0x3fffu);
+        float2* L2 = buf + ((wd >> 16) & 0x3fffu);
+        if constexpr (BR) {
+          float4 a = pf.a[it], b = pf.b[it];
+          if (z0) a.y = a.w = b.y = b.w = 0.f;
 #pragma unroll
-            for (int r = 0; r < 2; ++r) {
-              const float2 p = r ? make_float2(a.z, a.w) : make_float2(a.x, a.y);
-              const float2 q = r ? make_float2(b.z, b.w) : make_float2(b.x, b.y);
-              float2* Ln = buf + (r * Lh + l) * LSC;
-              if (kz == 0) {  // c2r reads only the real part of the zero mode
-                Ln[0] = make_float2(p.x, q.x);
-              } else {
-                Ln[F::pos(kz)] = make_float2(p.x - q.y, p.y + q.x);
-                Ln[F::pos(n - kz)] = make_float2(p.x + q.y, q.x - p.y);
-              }
-            }
-          } else {
-            const float2 p = pf.a[it], q = pf.b[it];
-            float2* Ln = buf + l * LSC;
-            if (kz == 0) {
-              Ln[0] = make_float2(p.x, q.x);
-            } else {
-              Ln[F::pos(kz)] = make_float2(p.x - q.y, p.y + q.x);
-              Ln[F::pos(n - kz)] = make_float2(p.x + q.y, q.x - p.y);
-            }
+          for (int r = 0; r < 2; ++r) {
+            const float2 p = r ? make_float2(a.z, a.w) : make_float2(a.x, a.y);
+            const float2 q = r ? make_float2(b.z, b.w) : make_float2(b.x, b.y);
+            L1[r * Lh * LSC] = make_float2(p.x - q.y, p.y + q.x);
+            L2[r * Lh * LSC] = make_float2(p.x + q.y, q.x - p.y);
           }
+        } else {
+          float2 p = pf.a[it], q = pf.b[it];
+          if (z0) p.y = q.y = 0.f;
+          L1[0] = make_float2(p.x - q.y, p.y + q.x);
+          L2[0] = make_float2(p.x + q.y, q.x - p.y);
         }
       }
     }
@@ -469,7 +502,10 @@ __global__ void __launch_bounds__(THREADS) zgemm_fixed_kernel(ZGemmArgs g) {
     lds_barrier();
     // the line FFTs, in place (z = z1 + N1 z2 at position z2 + RW z1)
     // (modes Q .. n - Q are zero: pass 1 does not read their positions)
-    fixed_fft_inplace<N1, N2, THREADS>(buf, T1, L, LSC, Q);
+    if (Qi > Q)
+      fixed_fft_inplace<N1, N2, THREADS, true>(buf, T1, L, LSC, Q);
+    else
+      fixed_fft_inplace<N1, N2, THREADS>(buf, T1, L, LSC, Q);
     lds_barrier();
     constexpr int nchunk = BR ? 2 * NCH : NCH;
     if constexpr (BB) {
@@ -645,14 +681,15 @@ static hipError_t zgf_launch_f(const ZGemmArgs& g, int nblk, size_t lds, hipStre
 
 // items a thread fetches per round: B == A, ceil(na / 2) Q float4 pairs;
 // B != A, max(na, nb) Q float2 pairs (at most 4 per thread)
-static int zgf_maxf(int Q, int na, int nb, int same) {
-  const int total = same ? ((na + 1) / 2) * Q : (na > nb ? na : nb) * Q;
+static int zgf_items_per_line(int n, int Q) { return 4 * Q <= n ? n / 4 + 1 : Q; }
+static int zgf_maxf(int Qi, int na, int nb, int same) {
+  const int total = same ? ((na + 1) / 2) * Qi : (na > nb ? na : nb) * Qi;
   return total <= 2 * ZGF_THREADS ? 2 : total <= 4 * ZGF_THREADS ? 4 : total <= 8 * ZGF_THREADS ? 8 : 0;
 }
 
 template <int T, bool BR, bool BB, int N1, int N2>
 static hipError_t zgf_launch(const ZGemmArgs& g, int nblk, size_t lds, hipStream_t st) {
-  switch (zgf_maxf(g.Q, g.na, g.nb, BR ? 1 : 0)) {
+  switch (zgf_maxf(g.Qi, g.na, g.nb, BR ? 1 : 0)) {
     case 2: return zgf_launch_f<T, BR, BB, N1, N2, 2>(g, nblk, lds, st);
     case 4: return zgf_launch_f<T, BR, BB, N1, N2, 4>(g, nblk, lds, st);
     case 8: return zgf_launch_f<T, BR, BB, N1, N2, 8>(g, nblk, lds, st);
@@ -682,7 +719,9 @@ hipError_t launch_zgemm(const SynthArgs& a, int Q, const void* HA, int na, int s
   g.partial = partial;
   g.partial_bb = partial + (int64_t)nblk * 1024;  // used by BB launches only
   hipError_t e;
-  if (a.n == 400 && zgf_maxf(Q, na, g.nb, same) > 0) {  // fixed plan 400 = 20 x 20
+  g.Qi = zgf_items_per_line(a.n, Q);
+  // fixed plan 400 = 20 x 20 (fetch words hold byte offsets below 2^17)
+  if (a.n == 400 && zgf_maxf(g.Qi, na, g.nb, same) > 0 && (32 * Q + g.Qi) * 16 < (1 << 17)) {
     const size_t lds = zgemm_fixed_lds_bytes<20, 20>(zg_lines_alloc(g.nlines, same));
     if (same)
       e = zgf_launch<ZGF_THREADS, true, false, 20, 20>(g, nblk, lds, st);
