@@ -1,23 +1,26 @@
 """Catalog ingestion (``catalog_processor.py``), single process.
 
 Box geometry: a ``.npy`` 2-D array whose columns are named by
-``column_names`` (``x``, ``y``, ``z`` required, ``w`` optional weight;
-``catalog_reader`` box branch, ``catalog_processor.py:293-449``).
+``column_names`` (``x``, ``y``, ``z`` required, ``w`` optional weight), or
+FITS / HDF5 tables of named columns (X/Y/Z or x/y/z, optional velocities
+for RSD, ``w``; ``catalog_reader`` box branch, ``catalog_processor.py:293-449``;
+HDF5 through ``cosmonpc_amd.hdf5``, no h5py).
 
 Survey geometry: FITS binary tables (``:481-563``) read by ``cosmonpc_amd.fits``
 -- redshift cut, completeness weights (``add_completeness_weight``,
 ``:179-282``), RA/DEC/Z -> comoving Cartesian positions on the GPU
 (``ra_dec_z_to_xyz``, ``:589-645``: flat LCDM from the config's h and
 Omega0, else the default Planck18 with photons and neutrinos) and the box
-centre from the first randoms catalog (``:565-574``).  HDF5 needs h5py,
-which is not part of this framework.
+centre from the first randoms catalog (``:565-574``).  Survey catalogs in
+HDF5 are not read by the reference either (its survey branch handles FITS
+only).
 """
 import logging
 import os
 
 import numpy as np
 
-from . import fits
+from . import fits, hdf5
 
 SUPPORTED = {"npy", "fits", "h5", "hdf5"}
 
@@ -85,12 +88,11 @@ def read_box_catalog(path, column_names, apply_rsd=False, para_cosmo=None, boxsi
     ext = _ext(path)
     if ext not in SUPPORTED:
         raise ValueError(f"Unsupported data file type: {ext}")
-    if ext in ("h5", "hdf5"):
-        raise NotImplementedError(f"box-like catalogs in .{ext} format need h5py, which is not part of cosmonpc_amd")
-    if ext == "fits":
+    if ext in ("fits", "h5", "hdf5"):
         # catalog_processor.py:388-449: X/Y/Z (or x/y/z), optional RSD from
-        # VX/VY/VZ (or vx/vy/vz), weight column 'w'
-        arr = read_fits_rows(path, column_names)
+        # VX/VY/VZ (or vx/vy/vz), weight column 'w'.  HDF5 rows hold every
+        # requested column (h5_reader, :16-88), present in the file or not
+        arr = read_fits_rows(path, column_names) if ext == "fits" else hdf5.read_columns(path, column_names)
         names = arr.dtype.names
         pos = np.zeros((len(arr), 3))
         for axes in (("X", "Y", "Z"), ("x", "y", "z")):
@@ -207,7 +209,8 @@ def read_survey_catalog(ctx, catalog, column_names, z_range, comp_weight_plan, p
     if ext == "npy":
         raise NotImplementedError("Survey-like geometry with .npy file is not yet implemented.")
     if ext != "fits":
-        raise NotImplementedError(f"survey-like catalogs in .{ext} format need a reader that is not part of cosmonpc_amd")
+        raise NotImplementedError(f"survey-like catalogs in .{ext} format are not supported (the reference's "
+                                  "survey branch reads FITS only)")
     assert "Z" in column_names, \
         "For survey-like geometry with .fits file, column_names must include 'Z' for redshift slice."
     arr = rows if rows is not None else read_fits_rows(catalog, column_names)
