@@ -97,6 +97,15 @@ def main():
     # the reader refuses by name
     with h5py.File(os.path.join(HERE, "box_extensible.h5"), "w", libver="latest") as f:
         f.create_dataset("x", data=c["x"][:100], maxshape=(None,), chunks=(16,))
+    # chunks allocated at creation, unfiltered: the implicit chunk index
+    with h5py.File(os.path.join(HERE, "box_implicit.h5"), "w", libver="latest") as f:
+        dcpl = h5py.h5p.create(h5py.h5p.DATASET_CREATE)
+        dcpl.set_chunk((64,))
+        dcpl.set_alloc_time(h5py.h5d.ALLOC_TIME_EARLY)
+        space = h5py.h5s.create_simple((1000,))
+        ds = h5py.h5d.create(f.id, b"x", h5py.h5t.IEEE_F64LE, space, dcpl=dcpl)
+        ds.write(h5py.h5s.ALL, h5py.h5s.ALL, np.ascontiguousarray(c["x"][:1000]))
+    expected["box_implicit/x"] = c["x"][:1000]
     expected.update({f"box_resizable/{k}": c[k] for k in ("x", "y")})
     expected.update({f"box_paged/{k}": c[k] for k in ("x", "y")})
     # 6. nested groups
