@@ -293,32 +293,6 @@ struct SynthFixed {
   static constexpr int LSC = P::KP + 2;
 };
 
-// Mode k of every line of the block into its padded position: thread t takes
-// k = t, t + THREADS, ... of every line (no divisions by the line length),
-// all loads of a thread issued before the first LDS write.  get(l, f) is the
-// value of frequency f in [-n/2, n/2) of line l (zero outside the shell).
-template <int N1, int N2, int THREADS, int NL, typename Get>
-__device__ __forceinline__ void fixed_load_lines(float2* __restrict__ buf, int nl, Get get) {
-  typedef FixedPlan<N1, N2> F;
-  constexpr int n = F::n, LSC = SynthFixed<N1, N2>::LSC, KPT = (n + THREADS - 1) / THREADS;
-  float2 v[NL][KPT];
-#pragma unroll
-  for (int l = 0; l < NL; ++l)
-#pragma unroll
-    for (int i = 0; i < KPT; ++i) {
-      const int k = threadIdx.x + i * THREADS;
-      v[l][i] = make_float2(0.f, 0.f);
-      if (l < nl && k < n) v[l][i] = get(l, k < (n >> 1) ? k : k - n);
-    }
-#pragma unroll
-  for (int l = 0; l < NL; ++l)
-#pragma unroll
-    for (int i = 0; i < KPT; ++i) {
-      const int k = threadIdx.x + i * THREADS;
-      if (l < nl && k < n) buf[l * LSC + F::ipos(k)] = v[l][i];
-    }
-}
-
 template <int N1, int N2, int THREADS, int NL>
 __global__ void __launch_bounds__(THREADS) xfft_fixed_kernel(SynthArgs a, const float2* __restrict__ V,
                                                              const signed char* __restrict__ bins,
@@ -331,7 +305,6 @@ __global__ void __launch_bounds__(THREADS) xfft_fixed_kernel(SynthArgs a, const 
   extern __shared__ unsigned char smem[];
   float2* T1 = (float2*)smem;
   float2* buf = T1 + n;
-  fixed_twiddles<N1, N2>(T1, twg);
   const int64_t lbase = ((int64_t)kz * P + ky0) * P;
   // lines with kz or every ky of the block beyond a shell's radius are zero
   // for that shell: skipped (the y pass never reads them)
@@ -361,6 +334,9 @@ __global__ void __launch_bounds__(THREADS) xfft_fixed_kernel(SynthArgs a, const 
         cv[l][i] = V[j];
       }
     }
+  // the twiddle table's loads after the cube's (in-order completion: one
+  // wait covers both)
+  fixed_twiddles<N1, N2>(T1, twg);
   for (int s = 0; s < a.nshell; ++s) {
     if (!live(s)) continue;
     const int want = a.bin0 + s;
@@ -376,10 +352,16 @@ __global__ void __launch_bounds__(THREADS) xfft_fixed_kernel(SynthArgs a, const 
     __syncthreads();
     // U[s][kz][ky block][x][ky % NL] (the fixed plans' layout): one
     // contiguous run of n NL entries per workgroup and shell
-    float2* dst = U + (((int64_t)s * Q + kz) * gridDim.x + blockIdx.x) * n * NL;
-    for (int t = threadIdx.x; t < NL * n; t += THREADS) {
-      const int x = t / NL, l = t - x * NL;
-      if (l < nl) dst[t] = buf[l * LSC + F::opos(x)];
+    // (two lines of one x per 16-byte store)
+    static_assert(NL % 2 == 0, "the x pass stores line pairs");
+    float4* dst = (float4*)(U + (((int64_t)s * Q + kz) * gridDim.x + blockIdx.x) * n * NL);
+    for (int t = threadIdx.x; t < (NL / 2) * n; t += THREADS) {
+      const int x = t / (NL / 2), l = 2 * (t - x * (NL / 2));
+      if (l < nl) {
+        const float2 a = buf[l * LSC + F::opos(x)];
+        const float2 b = l + 1 < nl ? buf[(l + 1) * LSC + F::opos(x)] : make_float2(0.f, 0.f);
+        dst[t] = make_float4(a.x, a.y, b.x, b.y);
+      }
     }
     __syncthreads();
   }
@@ -405,27 +387,52 @@ __global__ void __launch_bounds__(THREADS) yfft_fixed_kernel(SynthArgs a, const 
   extern __shared__ unsigned char smem[];
   float2* T1 = (float2*)smem;
   float2* buf = T1 + n;
-  fixed_twiddles<N1, N2>(T1, twg);
-  // line l: U[s][kz0 + l][ky / NLX][x][ky % NLX] (the x pass's layout, NLX
-  // lines per block)
+  // mode k of every line into its padded position: thread t takes k = t, t +
+  // THREADS, ... of every line; line l is U[s][kz0 + l][ky / NLX][x][ky %
+  // NLX] (the x pass's layout, NLX lines per block).  All loads are issued
+  // before the twiddle table's (in-order completion: one wait covers both)
+  constexpr int KPT = (n + THREADS - 1) / THREADS;
   const int nbx = (P + NLX - 1) / NLX;
+  const int64_t lstride = (int64_t)nbx * n * NLX;
   const float2* src = U + (((int64_t)s * Q + kz0) * nbx * n + x) * NLX;
-  fixed_load_lines<N1, N2, THREADS, NL>(buf, nl, [&](int l, int f) {
-    const int ky = f + R;
-    return (f >= -rs && f <= rs && kz0 + l <= rs)
-               ? src[((int64_t)l * nbx + ky / NLX) * n * NLX + ky % NLX]
-               : make_float2(0.f, 0.f);
-  });
+  int off[KPT];
+  bool ok[KPT];
+#pragma unroll
+  for (int i = 0; i < KPT; ++i) {
+    const int k = threadIdx.x + i * THREADS;
+    const int f = k < (n >> 1) ? k : k - n, ky = f + R;
+    ok[i] = k < n && f >= -rs && f <= rs;
+    off[i] = ok[i] ? (ky / NLX) * n * NLX + ky % NLX : 0;
+  }
+  float2 v[NL][KPT];
+#pragma unroll
+  for (int l = 0; l < NL; ++l)
+#pragma unroll
+    for (int i = 0; i < KPT; ++i) {
+      v[l][i] = make_float2(0.f, 0.f);
+      if (l < nl && kz0 + l <= rs && ok[i]) v[l][i] = src[l * lstride + off[i]];
+    }
+  fixed_twiddles<N1, N2>(T1, twg);
+#pragma unroll
+  for (int l = 0; l < NL; ++l)
+#pragma unroll
+    for (int i = 0; i < KPT; ++i) {
+      const int k = threadIdx.x + i * THREADS;
+      if (l < nl && k < n) buf[l * LSC + F::ipos(k)] = v[l][i];
+    }
   __syncthreads();
   fixed_fft_inplace<N1, N2, THREADS>(buf, T1, nl, LSC, rs + 1);
   __syncthreads();
-  // each y pair writes 2 nl consecutive entries: thread t -> pair t / (2 NL),
-  // entry (l, y % 2) = t % (2 NL)
-  float2* dst = H + (((int64_t)x * (n >> 1) * nslots + slot0 + s) * Q + kz0) * 2;
-  const int64_t ystride = (int64_t)nslots * Q * 2;
-  for (int t = threadIdx.x; t < NL * n; t += THREADS) {
-    const int yp = t / (2 * NL), e = t - yp * (2 * NL), l = e >> 1, j = e & 1;
-    if (l < nl) dst[yp * ystride + e] = buf[l * LSC + F::opos(2 * yp + j)];
+  // each y pair writes 2 nl consecutive entries, both y of a line as one
+  // 16-byte store: thread t -> pair t / NL, line t % NL
+  float4* dst = (float4*)(H + (((int64_t)x * (n >> 1) * nslots + slot0 + s) * Q + kz0) * 2);
+  const int64_t ystride = (int64_t)nslots * Q;  // float4 per y pair
+  for (int t = threadIdx.x; t < NL * (n >> 1); t += THREADS) {
+    const int yp = t / NL, l = t - yp * NL;
+    if (l < nl) {
+      const float2 a = buf[l * LSC + F::opos(2 * yp)], b = buf[l * LSC + F::opos(2 * yp + 1)];
+      dst[yp * ystride + l] = make_float4(a.x, a.y, b.x, b.y);
+    }
   }
 }
 
