@@ -122,10 +122,13 @@ def main():
         configurations (they are [0,0,0]'s legs and [2,0,2]'s second leg)."""
         m, nmax = contraction_grid(n, L, kmax, mode)
         dm = ctx.resample(delta, n, m, prec, nmax) if m != n else delta
-        G = bk_box_g00(ctx, dm, m, prec)
         if shared:
-            fin = bk_box_signal_multi(ctx, dm, G, m, L, prec, k_edge, angus, mode, zsynth=zsynth, defer=True)
+            # G_00 made inside, from the resampled spectrum once the shells
+            # have read it (no copy of it)
+            fin = bk_box_signal_multi(ctx, dm, None, m, L, prec, k_edge, angus, mode, zsynth=zsynth, defer=True,
+                                      consume_delta=m != n)
         else:
+            G = bk_box_g00(ctx, dm, m, prec)
             fins = [bk_box_signal_auto(ctx, dm, G, m, L, prec, k_edge, a, mode, zsynth=zsynth, defer=True)
                     for a in angus]
             fin = lambda: [f() for f in fins]  # noqa: E731

@@ -408,6 +408,16 @@ def _config_blocks(ctx, shells, Gs, kinfo, src1, src2, comps1, comps2, use_sym, 
     return blocks
 
 
+def _fused_rows(ctx, n, mode, zsynth, method, nc1, nc2, k_bins):
+    """(fused, rows per block): whether a configuration with nc1 / nc2 real
+    shell components per bin takes the fused z-synthesis contraction, and
+    the most bins per block it allows."""
+    if not (zsynth == "fused" and mode == "bf16x3" and method == "synth"):
+        return False, 0
+    cap = min(k_bins, 32 // max(nc1, nc2))
+    return cap >= 1 and ctx.zgemm_supported(n, nc1 * cap, nc2 * cap, False), cap
+
+
 def _config_plan(ctx, builder, src1, src2, Gs, kinfo, ell_1, m1, ell_2, m2, sym_eq, sym_cj, mode, block_size,
                  dvm, conj_w=False, zsynth="fused", shell_cache=None, zbatch=None):
     """3j-free signal matrix of one magnetic configuration,
@@ -447,10 +457,7 @@ def _config_plan(ctx, builder, src1, src2, Gs, kinfo, ell_1, m1, ell_2, m2, sym_
         comps2 = _weight_components(ell_2, m2, conj_w)
     nc1, nc2 = len(comps1), len(comps2)
     kmax = float(k_edge[-1])
-    fused = zsynth == "fused" and mode == "bf16x3" and builder.method == "synth"
-    if fused:
-        cap = min(k_bins, 32 // max(nc1, nc2))
-        fused = cap >= 1 and ctx.zgemm_supported(n, nc1 * cap, nc2 * cap, False)
+    fused, cap = _fused_rows(ctx, n, mode, zsynth, builder.method, nc1, nc2, k_bins)
     if fused:
         R = ctx.synth_radius(L, kmax)
         shell_bytes = n * n * (R + 1) * 8  # one slot of the half-synthesised layout
@@ -627,8 +634,21 @@ def bk_box_signal_auto(ctx, delta, G, n, boxsize, prec, k_edge, angu_config, con
                       zsynth=zsynth, defer=defer, shell_cache=shell_cache, zbatch=zbatch)
 
 
+def _all_fused(ctx, n, mode, zsynth, method, angu_configs, k_bins):
+    """Every magnetic configuration of every angular configuration takes the
+    fused contraction (so all contractions wait for ``ZBatch.run``)."""
+    for ell_1, ell_2, L in angu_configs:
+        mcfg, _ = A.get_magnetic_configs_box(ell_1, ell_2, L)
+        for m1, m2, _ in mcfg:
+            nc1 = len(_weight_components(ell_1, m1, False))
+            nc2 = nc1 if ell_1 == ell_2 and abs(m1) == abs(m2) else len(_weight_components(ell_2, m2, False))
+            if not _fused_rows(ctx, n, mode, zsynth, method, nc1, nc2, k_bins)[0]:
+                return False
+    return True
+
+
 def bk_box_signal_multi(ctx, delta, G, n, boxsize, prec, k_edge, angu_configs, contraction, dvm="full",
-                        block_size="full", zsynth="fused", defer=False, shell_method="synth"):
+                        block_size="full", zsynth="fused", defer=False, shell_method="synth", consume_delta=False):
     """``bk_box_signal_auto`` for several angular configurations of the same
     field (the reference runs one ``run_stats`` per configuration).  With the
     fused contraction, shells that several configurations use are
@@ -637,12 +657,25 @@ def bk_box_signal_multi(ctx, delta, G, n, boxsize, prec, k_edge, angu_configs, c
     [0,0,0]'s comes out of [l,0,l]'s launch).  Returns the list of
     ``total_res`` (or, with ``defer``, the function that reads it); each
     equals the single-configuration result up to the FP32 accumulation order
-    of the contraction."""
+    of the contraction.
+
+    ``G`` None: G_00 is made here.  With ``consume_delta`` (the caller no
+    longer needs ``delta``) and every contraction fused, the inverse FFT of
+    G_00 runs on ``delta`` itself after the shells are synthesised (they
+    read it first), which saves the copy of ``delta`` that the transform
+    would otherwise overwrite."""
     cache = {}
     zbatch = ZBatch(ctx)
+    late = False
+    if G is None:
+        mode = _gemm_mode(contraction, prec)
+        late = consume_delta and _all_fused(ctx, n, mode, zsynth, shell_method, angu_configs, len(k_edge) - 1)
+        G = ctx.mesh(n, prec).view(-1) if late else bk_box_g00(ctx, delta, n, prec)
     fins = [bk_box_signal_auto(ctx, delta, G, n, boxsize, prec, k_edge, a, contraction, dvm, block_size, zsynth,
                                defer=True, shell_cache=cache, zbatch=zbatch, shell_method=shell_method)
             for a in angu_configs]
+    if late:
+        ctx.c2r(delta, G.view(n, n, n), n, prec)
     zbatch.run()
     cache.clear()  # freed buffers are reused in stream order, after the work that reads them
 
@@ -694,10 +727,10 @@ def bk_box_signals(field, stat_attrs_list):
     ca = field.delta_k(ctx, 1, scale=1.0 / vol_per_cell, sub0=sa["NZ_a"])
     m, nmax = contraction_grid(n, Lbox, k_max, contraction, sa.get("shell_grid", "reduced"))
     ca_m = reduce_spectra(ctx, [ca], n, prec, m, nmax)[0] if m != n else ca
-    G = bk_box_g00(ctx, ca_m, m, prec)
-    res = bk_box_signal_multi(ctx, ca_m, G, m, Lbox, prec, k_edge, [tuple(s["angu_config"]) for s in stat_attrs_list],
-                              contraction, dvm, block_size, sa.get("zsynth", "fused"),
-                              shell_method=sa.get("shell_method", "synth"))
+    res = bk_box_signal_multi(ctx, ca_m, None, m, Lbox, prec, k_edge,
+                              [tuple(s["angu_config"]) for s in stat_attrs_list], contraction, dvm, block_size,
+                              sa.get("zsynth", "fused"), shell_method=sa.get("shell_method", "synth"),
+                              consume_delta=m != n)
     return [(n / m) ** 3 * r for r in res]
 
 

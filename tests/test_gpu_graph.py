@@ -85,6 +85,28 @@ def test_multi_configuration_pass_without_shared_legs(gpu_ctx):
         assert np.max(np.abs(want)) > 0 and err < 1e-5
 
 
+def test_g00_from_consumed_spectrum(gpu_ctx):
+    """G made inside the multi-configuration call from a spectrum the caller
+    gives up (the inverse FFT runs on it after the shells have read it): the
+    same matrices, bit for bit, as with G_00 made beforehand from a copy;
+    configurations that do not all take the fused contraction ([1,1,0]'s
+    complex components beside [0,0,0]) fall back to the copy and still
+    match."""
+    from cosmonpc_amd.estimators import bk_box_g00, bk_box_signal_multi
+
+    ctx = gpu_ctx
+    X = _spectrum(ctx, 11)
+    for angus in (ANGUS, [(1, 1, 0), (0, 0, 0)]):
+        src = X.clone()
+        G = bk_box_g00(ctx, src, N, 1)
+        want = bk_box_signal_multi(ctx, src, G, N, L, 1, EDGES, angus, "bf16x3")
+        src2 = X.clone()
+        got = bk_box_signal_multi(ctx, src2, None, N, L, 1, EDGES, angus, "bf16x3", consume_delta=True)
+        for a, g, w in zip(angus, got, want):
+            _log(f"G late {a}: max |diff| {np.max(np.abs(g - w)):.3e}")
+            assert np.max(np.abs(w)) > 0 and np.array_equal(g, w), a
+
+
 def test_graph_replays_match_eager(gpu_ctx):
     import torch
 
