@@ -40,8 +40,8 @@ def log(msg):
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=3)
-    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--steps", type=int, default=20)
+    p.add_argument("--warmup", type=int, default=3)
     p.add_argument("--nmesh", type=int, default=512)
     p.add_argument("--nparticles", type=int, default=5_000_000)
     p.add_argument("--k-bins", type=int, default=30)
