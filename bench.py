@@ -50,7 +50,7 @@ def parse():
     p.add_argument("--no-graph", action="store_true", help="time the eager launches only (no hipGraph replay)")
     p.add_argument("--no-mocks", action="store_true", help="skip the batched-mocks comparison of the default run")
     p.add_argument("--roofline-only", action="store_true")
-    p.add_argument("--workload", choices=("metric", "mocks", "kernels"), default="metric",
+    p.add_argument("--workload", choices=("metric", "mocks", "kernels", "survey"), default="metric",
                    help="mocks: 64 box mocks (pk + bk_sugi diagonal at 256^3), batched driver vs per-mock loop; "
                         "kernels: one launch of each hot kernel at 512^3 plus a 1 GiB device copy (for counters)")
     p.add_argument("--mocks", type=int, default=64)
@@ -69,6 +69,8 @@ def main():
         return bench_mocks(args)
     if args.workload == "kernels":
         return bench_kernels(args)
+    if args.workload == "survey":
+        return bench_survey(args)
     # the mock catalogs are drawn in worker processes before anything touches the GPU
     mock_cats = None if args.no_mocks or args.mocks < 1 else mock_catalogs(args.mocks)
     import numpy as np
@@ -773,6 +775,144 @@ def bench_mocks(args):
         "config": {"workload": r["workload"], "mock_batch": args.mock_batch},
         "per_mock_loop": r["per_mock_loop"], "speedup_vs_loop": r["speedup_vs_loop"],
         "max_rel_diff_vs_loop": r["max_rel_diff_vs_loop"], "stages_s": r["stages_s"],
+    }
+    print(json.dumps(line), flush=True)
+
+
+def survey_catalogs(ndata, nrandoms, seed, omega_m):
+    """Survey-shaped catalogs of the 512^3 survey workload, drawn on the GPU
+    (the numpy generator of cosmonpc_amd.synth takes minutes at this size):
+    RA 150-250 deg, DEC 0-60 deg, 0.4 < z < 0.7 with Gaussian n(z) (mean
+    0.55, sigma 0.08, peak 3e-4) times dV/dz; data thinned by a lognormal
+    density (128^3, P(k) = 2e4 (k/0.1)^-1.5), randoms uniform; WEIGHT_FKP =
+    1 / (1 + 1e4 NZ).  Returns host structured arrays (RA, DEC, Z,
+    WEIGHT_FKP, NZ)."""
+    import numpy as np
+    import torch
+
+    from cosmonpc_amd import synth
+
+    dev = torch.device("cuda", 0)
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    ra, dec, zr = (150.0, 250.0), (0.0, 60.0), (0.4, 0.7)
+
+    def nz(z):
+        return 3.0e-4 * np.exp(-0.5 * ((z - 0.55) / 0.08) ** 2)
+
+    zt, chi = synth._chi_table(omega_m, zr[1] * 1.05)
+    m = (zt >= zr[0]) & (zt <= zr[1])
+    zg = zt[m]
+    pdf = nz(zg) * chi[m] ** 2 * np.gradient(chi, zt)[m]
+    cdf = np.concatenate([[0.0], np.cumsum(0.5 * (pdf[1:] + pdf[:-1]) * np.diff(zg))])
+    cdf /= cdf[-1]
+    cdf_t, zg_t = torch.from_numpy(cdf).to(dev), torch.from_numpy(zg).to(dev)
+    zt_t, chi_t = torch.from_numpy(zt).to(dev), torch.from_numpy(chi).to(dev)
+
+    def interp(x, xp, fp):
+        i = torch.clamp(torch.searchsorted(xp, x), 1, len(xp) - 1)
+        t = (x - xp[i - 1]) / (xp[i] - xp[i - 1])
+        return fp[i - 1] + t * (fp[i] - fp[i - 1])
+
+    def draw(count):
+        u = torch.rand(count, generator=g, device=dev, dtype=torch.float64)
+        z = interp(u, cdf_t, zg_t)
+        r = ra[0] + (ra[1] - ra[0]) * torch.rand(count, generator=g, device=dev, dtype=torch.float64)
+        s0, s1 = np.sin(np.deg2rad(dec[0])), np.sin(np.deg2rad(dec[1]))
+        d = torch.rad2deg(torch.asin(s0 + (s1 - s0) * torch.rand(count, generator=g, device=dev,
+                                                                   dtype=torch.float64)))
+        c = interp(z, zt_t, chi_t)
+        rr, dd = torch.deg2rad(r), torch.deg2rad(d)
+        pos = torch.stack([c * torch.cos(dd) * torch.cos(rr), c * torch.cos(dd) * torch.sin(rr), c * torch.sin(dd)], 1)
+        return r, d, z, pos
+
+    def rows(r, d, z):
+        out = np.zeros(len(z), dtype=[("RA", "f8"), ("DEC", "f8"), ("Z", "f8"), ("WEIGHT_FKP", "f8"), ("NZ", "f8")])
+        out["RA"], out["DEC"], out["Z"] = r.cpu().numpy(), d.cpu().numpy(), z.cpu().numpy()
+        out["NZ"] = nz(out["Z"])
+        out["WEIGHT_FKP"] = 1.0 / (1.0 + out["NZ"] * 1.0e4)
+        return out
+
+    r, d, z, pos = draw(int(nrandoms))
+    randoms = rows(r, d, z)
+    lo, hi = pos.min(0).values, pos.max(0).values
+    L = float((hi - lo).max()) * 1.02
+    center = 0.5 * (lo + hi)
+    del r, d, z, pos
+    n = 128
+    _, rho = synth._lognormal_density(n, L, np.random.default_rng(seed), 2.0e4, -1.5)
+    rho_t = torch.from_numpy(rho).to(dev)
+    cmax = float(rho.max())
+    parts, need = [], int(ndata)
+    while need > 0:
+        cnt = max(1024, int(need * cmax * 1.2))
+        r, d, z, pos = draw(cnt)
+        idx = torch.clamp(((pos - center) / L + 0.5) * n, 0, n - 1).long()
+        acc = torch.rand(cnt, generator=g, device=dev, dtype=torch.float64) < rho_t[idx[:, 0], idx[:, 1], idx[:, 2]] / cmax
+        sel = torch.nonzero(acc).flatten()[:need]
+        parts.append((r[sel], d[sel], z[sel]))
+        need -= len(sel)
+    data = rows(*(torch.cat([p[i] for p in parts]) for i in range(3)))
+    return data, randoms
+
+
+def bench_survey(args):
+    """The survey workload end to end through run_stats (catalog rows in
+    host memory -> RA/DEC/Z to Cartesian on the GPU -> paint -> FFTs ->
+    estimators -> saved result): pk poles 0/2/4 and bk_sugi [2,0,2]
+    diagonal, 512^3 TSC interlaced, box 3000, 2e6 data + 2e7 randoms, FP32
+    bulk (alpha, I22, I33 and shot-noise sums in FP64) and FP64.  Each
+    configuration runs once untimed (plans, work areas), then timed."""
+    import tempfile
+
+    import numpy as np
+    import torch
+
+    from cosmonpc_amd import run_stats
+
+    cosmo = {"h": 0.6766, "Omega0": 0.30966}
+    t0 = time.perf_counter()
+    data, randoms = survey_catalogs(2_000_000, 20_000_000, 2025, cosmo["Omega0"])
+    log(f"survey catalogs: {len(data)} data, {len(randoms)} randoms in {time.perf_counter() - t0:.1f}s")
+    n, L = 512, 3000.0
+    out = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        base = dict(correlation_mode="auto", geometry="survey-like",
+                    catalogs={"data_a": "gal.fits", "randoms_a": "ran.fits"},
+                    column_names=["RA", "DEC", "Z", "WEIGHT_FKP", "NZ"], nmesh=[n] * 3, boxsize=[L] * 3,
+                    sampler="tsc", interlaced=True, k_min=0.01, k_max=0.20, k_bins=19, compensation=True,
+                    normalization_scheme="particle", cosmology=dict(cosmo), z_range=None,
+                    comp_weight_plan={"scheme": None, "name_alias": None}, output_dir=tmp,
+                    arrays={"data_a": data, "randoms_a": randoms})
+        cfgs = {"pk_poles_024": dict(base, statistic="pk", poles=[0, 2, 4]),
+                "bk_sugi_202_diagonal": dict(base, statistic="bk_sugi", tracer_type="aaa", angu_config=[2, 0, 2],
+                                             data_vector_mode="diagonal", shotnoise_mode="ana")}
+        for prec in ("fp32", "fp64"):
+            for name, cfg in cfgs.items():
+                c = dict(cfg, precision=prec)
+                run_stats(dict(c))
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                res = run_stats(dict(c))
+                torch.cuda.synchronize()
+                dt = time.perf_counter() - t0
+                out[f"{name}_{prec}_s"] = dt
+                log(f"survey {name} {prec}: {dt:.3f} s end to end")
+                if name.startswith("bk"):
+                    out[f"{name}_{prec}_B0"] = complex(np.asarray(res["B_sugi"]).ravel()[3])
+        b32, b64 = out["bk_sugi_202_diagonal_fp32_B0"], out["bk_sugi_202_diagonal_fp64_B0"]
+        out["bk_fp32_vs_fp64_rel_bin3"] = abs(b32 - b64) / abs(b64)
+        for k in ("bk_sugi_202_diagonal_fp32_B0", "bk_sugi_202_diagonal_fp64_B0"):
+            out[k] = [out[k].real, out[k].imag]
+    total = out["pk_poles_024_fp32_s"] + out["bk_sugi_202_diagonal_fp32_s"]
+    line = {
+        "metric": "survey pk [0,2,4] + bk_sugi [2,0,2] diagonal, end to end through run_stats (s per catalog pair)",
+        "value": total, "unit": "s", "n_gpus": 1, "steps": 1, "warmup": 1, "ms_per_step": total * 1e3,
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32 meshes / FFTs / Ylm, fp64 catalog sums", "data": "synthetic (seeded lognormal survey, GPU-drawn)",
+        "config": {"workload": f"survey {n}^3 tsc interlaced, box {L:g}, 2e6 data + 2e7 randoms, RA 150-250, "
+                               "DEC 0-60, 0.4 < z < 0.7, 19 k-bins 0.01-0.20"},
+        "extra": out,
     }
     print(json.dumps(line), flush=True)
 
