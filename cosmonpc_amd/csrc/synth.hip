@@ -281,6 +281,15 @@ __global__ void __launch_bounds__(SYNTH_THREADS) zfft_kernel(SynthArgs a, int NL
   }
 }
 
+// Streaming 16-byte store (nontemporal): the x and y passes write their
+// outputs once for a later kernel, and scattered pieces written this way
+// reach HBM faster than through the cache hierarchy.
+__device__ __forceinline__ void stream_store(float4* p, float a, float b, float c, float d) {
+  typedef float f32x4s __attribute__((ext_vector_type(4)));
+  const f32x4s v = {a, b, c, d};
+  __builtin_nontemporal_store(v, (f32x4s*)p);
+}
+
 // ------------------------------------------------------------ fixed plans
 // The x and y passes of the half synthesis for grids n = N1 * N2 with the
 // two-pass in-place FFT of lds_fft.h (FixedPlan): inputs are scattered to
@@ -360,7 +369,7 @@ __global__ void __launch_bounds__(THREADS) xfft_fixed_kernel(SynthArgs a, const 
       if (l < nl) {
         const float2 a = buf[l * LSC + F::opos(x)];
         const float2 b = l + 1 < nl ? buf[(l + 1) * LSC + F::opos(x)] : make_float2(0.f, 0.f);
-        dst[t] = make_float4(a.x, a.y, b.x, b.y);
+        stream_store(dst + t, a.x, a.y, b.x, b.y);
       }
     }
     __syncthreads();
@@ -424,14 +433,14 @@ __global__ void __launch_bounds__(THREADS) yfft_fixed_kernel(SynthArgs a, const 
   fixed_fft_inplace<N1, N2, THREADS>(buf, T1, nl, LSC, rs + 1);
   __syncthreads();
   // each y pair writes 2 nl consecutive entries, both y of a line as one
-  // 16-byte store: thread t -> pair t / NL, line t % NL
+  // 16-byte streaming store: thread t -> pair t / NL, line t % NL
   float4* dst = (float4*)(H + (((int64_t)x * (n >> 1) * nslots + slot0 + s) * Q + kz0) * 2);
   const int64_t ystride = (int64_t)nslots * Q;  // float4 per y pair
   for (int t = threadIdx.x; t < NL * (n >> 1); t += THREADS) {
     const int yp = t / NL, l = t - yp * NL;
     if (l < nl) {
       const float2 a = buf[l * LSC + F::opos(2 * yp)], b = buf[l * LSC + F::opos(2 * yp + 1)];
-      dst[yp * ystride + l] = make_float4(a.x, a.y, b.x, b.y);
+      stream_store(dst + yp * ystride + l, a.x, a.y, b.x, b.y);
     }
   }
 }
