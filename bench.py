@@ -291,6 +291,7 @@ def main():
             extra[f"err_gemm_fp64_B{tag}"] = float(np.max(np.abs(rg - r64)) / np.max(np.abs(r64)))
         del d64
     del d32
+    extra["end_to_end"] = end_to_end(pos, w, n, L, kmax, K, sampler)
     if mock_cats is not None:
         extra["mocks"] = mock_comparison(args, mock_cats)
         del mock_cats
@@ -445,6 +446,41 @@ def kernel_roofline(ctx, delta, n, L, k_edge, torch, np, A):
         roof["measured_utilisation"] = dict(util, source="rocprofv3 PMC (profiles/kernels_512_pmc_utilisation.txt)")
     del work, shells, G, H
     return {"roofline": roof, "stages": stages}
+
+
+def end_to_end(pos, w, n, L, kmax, K, sampler):
+    """The workload end to end as a user runs it: one run_stats per angular
+    configuration ([0,0,0], then [2,0,2]) from the catalog in host memory
+    to the saved result (paint, FFTs, shells, contraction, shot noise,
+    normalisation, file), FP32 bulk with the split-bf16 contraction.  Each
+    configuration runs once untimed first (plans, work areas)."""
+    import tempfile
+
+    import torch
+
+    from cosmonpc_amd import run_stats
+
+    out = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        base = dict(statistic="bk_sugi", correlation_mode="auto", geometry="box-like",
+                    catalogs={"data_a": "cat.npy", "randoms_a": None, "data_b": None, "randoms_b": None},
+                    column_names=["x", "y", "z", "w"], rsd=[0, 0, 1], apply_rsd=False, nmesh=[n] * 3,
+                    boxsize=[L] * 3, sampler=sampler, interlaced=True, k_min=0.01, k_max=kmax, k_bins=K,
+                    tracer_type="aaa", data_vector_mode="full", shotnoise_mode="ana", compensation=True,
+                    cosmology={"h": 0.6766, "Omega0": 0.30966}, redshift_box=0.5, output_dir=tmp,
+                    precision="fp32", arrays={"data_a": (pos, w)})
+        for angu in ([0, 0, 0], [2, 0, 2]):
+            cfg = dict(base, angu_config=angu)
+            run_stats(dict(cfg))
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            run_stats(dict(cfg))
+            torch.cuda.synchronize()
+            out["run_stats_" + "".join(map(str, angu)) + "_s"] = time.perf_counter() - t0
+    out["note"] = ("one run_stats per configuration, catalog in host memory -> saved .npy result, fp32 bulk + "
+                   "split-bf16 contraction (the reference's way: no legs shared between configurations)")
+    log(f"end to end: {out}")
+    return out
 
 
 def autotune_choices():
