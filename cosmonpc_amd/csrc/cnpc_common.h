@@ -200,6 +200,15 @@ struct ShellArgs {
 #define CNPC_SYNTH_MAXS 128
 #define CNPC_MAX_DEVICES 64  // per-device launch state kept by the library
 
+// Layout of the half-synthesised shells (x and y transformed, kz rows 0 ..
+// Q-1 with Q = R + 1).  Tile-major, H[x][y/2][slot][kz][y%2] (complex),
+// one contiguous run per (x, y pair) for the fused contraction; or, for the
+// fixed 20 x 20 plans when 4Q <= n, blocked, H[x][slot][kz/8][y/2][kz%8][y%2]:
+// the y pass then writes one contiguous run of n x 8 values per (x, slot,
+// kz block), and the contraction reads rows y, y+1 of a (slot, kz block)
+// as 128 contiguous bytes.
+__host__ __device__ inline int half_blocked(int n, int Q) { return n == 400 && 4 * Q <= n ? 1 : 0; }
+
 struct SynthArgs {
   int n;        // grid
   int R;        // |n_d| <= R holds every shell mode

@@ -433,9 +433,15 @@ __global__ void __launch_bounds__(THREADS) yfft_fixed_kernel(SynthArgs a, const 
   fixed_fft_inplace<N1, N2, THREADS>(buf, T1, nl, LSC, rs + 1);
   __syncthreads();
   // each y pair writes 2 nl consecutive entries, both y of a line as one
-  // 16-byte streaming store: thread t -> pair t / NL, line t % NL
-  float4* dst = (float4*)(H + (((int64_t)x * (n >> 1) * nslots + slot0 + s) * Q + kz0) * 2);
-  const int64_t ystride = (int64_t)nslots * Q;  // float4 per y pair
+  // 16-byte streaming store: thread t -> pair t / NL, line t % NL.
+  // Tile-major H[x][y/2][slot][kz][y%2]: pair yp at + yp nslots Q float4;
+  // blocked H[x][slot][kz/8][y/2][kz%8][y%2]: this workgroup's (x, slot, kz
+  // block) is one contiguous run, pair yp at + 8 yp float4
+  static_assert(NL == 8, "the blocked layout holds kz blocks of NL = 8 lines");
+  const bool blocked = half_blocked(n, Q);
+  float4* dst = (float4*)(H + (blocked ? (((int64_t)x * nslots + slot0 + s) * ((Q + 7) / 8) + blockIdx.x) * n * 8
+                                       : (((int64_t)x * (n >> 1) * nslots + slot0 + s) * Q + kz0) * 2));
+  const int64_t ystride = blocked ? NL : (int64_t)nslots * Q;  // float4 per y pair
   for (int t = threadIdx.x; t < NL * (n >> 1); t += THREADS) {
     const int yp = t / NL, l = t - yp * NL;
     if (l < nl) {

@@ -61,6 +61,8 @@ constexpr int ZG_TAIL_PAD = 32;
 struct ZGemmArgs {
   int n, Q;                 // grid, non-zero kz per line (R + 1)
   int Qi;                   // fixed plan: fetched modes per line (Q, or n/4 + 1 when 4Q <= n)
+  int blocked;              // half shells in the blocked layout (half_blocked)
+  int nkb;                  // blocked layout: kz blocks of 8 per slot
   int na, nb;               // rows of A and B (<= 32)
   int sa_slots, sb_slots;   // slots per tile in HA / HB
   int same;                 // B is A
@@ -320,23 +322,23 @@ __device__ __forceinline__ void zgf_row8(const float2* p, float (&re)[8], float 
 // The operands of one round are fetched into registers, MAXF items per
 // thread: item t = threadIdx.x + it * THREADS is line l = t / Q, mode kz =
 // t % Q.  B == A: the float4 of shells 2l and 2l + 1 (both rows of the
-// tile); B != A: one row's (re, im) of A_l and B_l.  What an item reads and
+// tile); B != A: the float4 of A_l and of B_l (both rows, the second kept
+// for the tile's second round).  What an item reads and
 // where it lands in LDS is the same every round, so it is computed once:
 //   lds word: float2 index of mode kz in its line (bits 0-13), of mode n - kz
 //     (bits 16-29; for kz = 0 the pad column of the line's first row, which
 //     the passes never read) and kz == 0 (bit 31); items past the round's
 //     lines land on the pad column of line 0 (they carry zeros);
-//   fetch word: byte offset of the item in the tile's run of HA (and HB)
-//     (bits 0-16), bit 30 set when the A slot holds no row kz, bit 31 when
-//     the B slot holds none: masking with ZG_A_MASK / ZG_B_MASK leaves the
-//     offset, or an offset beyond the run, which the buffer load answers
-//     with zeros.
+//   fetch word: byte offset of the item in the tile's run of HA (and HB),
+//     or in the x plane for the blocked layout (bits 0-29), bit 30 set when
+//     the A slot holds no row kz, bit 31 when the B slot holds none: masking
+//     with ZG_A_MASK / ZG_B_MASK leaves the offset, or an offset beyond the
+//     run, which the buffer load answers with zeros.
 template <bool BR, int MAXF>
 struct ZgfFetch {
-  typedef typename std::conditional<BR, float4, float2>::type T;
-  T a[MAXF], b[MAXF];
+  float4 a[MAXF], b[MAXF];  // rows y0, y0 + 1 of each item
 };
-constexpr unsigned ZG_A_MASK = 0x4001ffffu, ZG_B_MASK = 0x8001ffffu;
+constexpr unsigned ZG_A_MASK = 0x7fffffffu, ZG_B_MASK = 0xbfffffffu;
 constexpr int ZG_BUF_FLAGS = 0x00020000;  // buffer resource word 3 (gfx9 raw buffer)
 
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t zg_rsrc(const void* base, int bytes) {
@@ -368,26 +370,48 @@ __device__ __forceinline__ void zgf_items(const ZGemmArgs& g, const int* qa, con
       va = ok && l < g.na && kz < qa[l];
       vb = ok && l < g.nb && kz < qb[l];
     }
-    fw[it] = (ok ? (unsigned)((slot * Q + kz) * 16) : 0u) | (va ? 0u : 0x40000000u) | (vb ? 0u : 0x80000000u);
+    // tile-major: (slot, kz) float4 of both rows; blocked: (slot, kz / 8)
+    // runs of n / 2 row pairs of 8 float4, pair yp at + 128 yp bytes (added
+    // per round)
+    const unsigned off = !ok ? 0u
+                         : g.blocked ? (unsigned)(((slot * g.nkb + kz / 8) * n) * 64 + (kz % 8) * 16)
+                                     : (unsigned)((slot * Q + kz) * 16);
+    fw[it] = off | (va ? 0u : 0x40000000u) | (vb ? 0u : 0x80000000u);
   }
 }
 
 template <int THREADS, bool BR, int MAXF>
 __device__ __forceinline__ void zgf_issue(ZgfFetch<BR, MAXF>& f, const ZGemmArgs& g, const unsigned (&fw)[MAXF],
-                                          int64_t tile, int r) {
+                                          int64_t tile) {
   const int Q = g.Q;
+  if (g.blocked) {
+    // blocked layout: the records are the tile's x plane, the tile's row
+    // pair yp = y0 / 2 a scalar offset
+    const int n = g.n, half = n >> 1;
+    const int64_t x = tile / half;
+    const int y0 = 2 * (int)(tile - x * half);
+    const int sb = g.nkb * n * 64;  // bytes per slot
+    const __amdgpu_buffer_rsrc_t ra = zg_rsrc((const char*)g.HA + x * g.sa_slots * (int64_t)sb, g.sa_slots * sb);
+    const __amdgpu_buffer_rsrc_t rb = zg_rsrc((const char*)g.HB + x * g.sb_slots * (int64_t)sb, g.sb_slots * sb);
+#pragma unroll
+    for (int it = 0; it < MAXF; ++it) {
+      // B == A: shells 2l and 2l + 1 sit one slot apart
+      f.a[it] = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(ra, (int)(fw[it] & ZG_A_MASK),
+                                                                                 64 * y0, 0));
+      f.b[it] = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(BR ? ra : rb,
+                                                                                 (int)(fw[it] & ZG_B_MASK),
+                                                                                 (BR ? sb : 0) + 64 * y0, 0));
+    }
+    return;
+  }
   const __amdgpu_buffer_rsrc_t ra = zg_rsrc(g.HA + tile * g.sa_slots * Q, g.sa_slots * Q * 16);
   const __amdgpu_buffer_rsrc_t rb = zg_rsrc(g.HB + tile * g.sb_slots * Q, g.sb_slots * Q * 16);
 #pragma unroll
   for (int it = 0; it < MAXF; ++it) {
-    if constexpr (BR) {
-      // shells 2l and 2l + 1 sit Q float4 apart
-      f.a[it] = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(ra, (int)(fw[it] & ZG_A_MASK), 0, 0));
-      f.b[it] = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(ra, (int)(fw[it] & ZG_B_MASK), Q * 16, 0));
-    } else {
-      f.a[it] = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(ra, (int)(fw[it] & ZG_A_MASK), 8 * r, 0));
-      f.b[it] = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(rb, (int)(fw[it] & ZG_B_MASK), 8 * r, 0));
-    }
+    // B == A: shells 2l and 2l + 1 sit Q float4 apart
+    f.a[it] = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(ra, (int)(fw[it] & ZG_A_MASK), 0, 0));
+    f.b[it] = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(BR ? ra : rb, (int)(fw[it] & ZG_B_MASK),
+                                                                               BR ? Q * 16 : 0, 0));
   }
 }
 
@@ -439,8 +463,10 @@ __global__ void __launch_bounds__(THREADS) zgemm_fixed_kernel(ZGemmArgs g) {
   zgf_items<THREADS, BR, MAXF, N1, N2>(g, qa, qb, Qi, total, lw, fw);
   ZgfFetch<BR, MAXF> pf;
   float gv[GPT];
+  // B != A: one fetch per tile holds both rows; the round of row y0 + 1
+  // fills from the registers the round of row y0 left
   auto issue_round = [&](int64_t tl, int rr) {
-    zgf_issue<THREADS, BR, MAXF>(pf, g, fw, tl, rr);
+    if (BR || rr == 0) zgf_issue<THREADS, BR, MAXF>(pf, g, fw, tl);
     const int xx = (int)(tl / half), yy = (int)(tl - (int64_t)xx * half);
     const float* Grow = g.G + ((int64_t)xx * n + 2 * yy) * n + (BR ? 0 : rr * n);
 #pragma unroll
@@ -484,7 +510,9 @@ __global__ void __launch_bounds__(THREADS) zgemm_fixed_kernel(ZGemmArgs g) {
             L2[r * Lh * LSC] = make_float2(p.x + q.y, q.x - p.y);
           }
         } else {
-          float2 p = pf.a[it], q = pf.b[it];
+          const float4 a = pf.a[it], b = pf.b[it];
+          float2 p = r0 ? make_float2(a.z, a.w) : make_float2(a.x, a.y);
+          float2 q = r0 ? make_float2(b.z, b.w) : make_float2(b.x, b.y);
           if (z0) p.y = q.y = 0.f;
           L1[0] = make_float2(p.x - q.y, p.y + q.x);
           L2[0] = make_float2(p.x + q.y, q.x - p.y);
@@ -720,8 +748,16 @@ hipError_t launch_zgemm(const SynthArgs& a, int Q, const void* HA, int na, int s
   g.partial_bb = partial + (int64_t)nblk * 1024;  // used by BB launches only
   hipError_t e;
   g.Qi = zgf_items_per_line(a.n, Q);
-  // fixed plan 400 = 20 x 20 (fetch words hold byte offsets below 2^17)
-  if (a.n == 400 && zgf_maxf(g.Qi, na, g.nb, same) > 0 && (32 * Q + g.Qi) * 16 < (1 << 17)) {
+  g.blocked = half_blocked(a.n, Q);
+  g.nkb = (Q + 7) / 8;
+  // fixed plan 400 = 20 x 20 (the only reader of the blocked layout); its
+  // fetch words hold offsets into one record (a tile's run, or an x plane
+  // when blocked) below 2^30
+  const int64_t slot_bytes = g.blocked ? (int64_t)g.nkb * a.n * 64 : (int64_t)Q * 16;
+  const bool fixed = a.n == 400 && zgf_maxf(g.Qi, na, g.nb, same) > 0 &&
+                     (int64_t)(g.sa_slots > g.sb_slots ? g.sa_slots : g.sb_slots) * slot_bytes < (1 << 30);
+  if (g.blocked && !fixed) return hipErrorInvalidValue;
+  if (fixed) {
     const size_t lds = zgemm_fixed_lds_bytes<20, 20>(zg_lines_alloc(g.nlines, same));
     if (same)
       e = zgf_launch<ZGF_THREADS, true, false, 20, 20>(g, nblk, lds, st);

@@ -214,10 +214,31 @@ class GpuContext:
         """R = ceil(kmax / kf): every shell mode has |n_d| <= R."""
         return int(math.ceil(float(kmax) / (2.0 * math.pi / float(boxsize))))
 
+    @staticmethod
+    def half_layout(n, R, nslots):
+        """Shape of the half-synthesised shells buffer for grid n and radius
+        R, and the axis holding the slots.  Blocked
+        [x][slot][kz/8][y/2][kz%8][y%2] for the n = 400 plans when 4 (R+1)
+        <= n, else tile-major [x][y/2][slot][kz][y%2] (kz < R+1)."""
+        Q = R + 1
+        if n == 400 and 4 * Q <= n:
+            return (n, nslots, (Q + 7) // 8, n // 2, 8, 2), 1
+        return (n, n // 2, nslots, Q, 2), 2
+
     def half_shells(self, n, R, nslots):
-        """Buffer of the half-synthesised layout [x][y/2][nslots][R+1][y%2]
-        (complex64)."""
-        return torch.empty((n, n // 2, nslots, R + 1, 2), dtype=torch.complex64, device=self.device)
+        """Buffer of the half-synthesised shells (complex64, ``half_layout``)."""
+        return torch.empty(self.half_layout(n, R, nslots)[0], dtype=torch.complex64, device=self.device)
+
+    def _half_slots(self, H, n, R):
+        """Slot count of a half-shells buffer; raises if H is not a
+        contiguous complex64 device buffer of ``half_layout(n, R, .)``."""
+        if H.dtype == torch.complex64 and H.is_contiguous() and H.is_cuda:
+            shape, axis = self.half_layout(n, R, 1)
+            nslots = H.shape[axis]
+            if tuple(H.shape) == self.half_layout(n, R, nslots)[0]:
+                return nslots
+        raise ValueError(f"half shells must be a contiguous complex64 tensor of shape "
+                         f"{self.half_layout(n, R, 'nslots')[0]}")
 
     def shell_synth_half(self, delta, n, boxsize, edges, bin0, nshell, poly, work, out, slot0, reuse_cube=False):
         """x and y passes of the FP32 shell synthesis of bins bin0 .. bin0 +
@@ -230,14 +251,12 @@ class GpuContext:
         if delta is not None:
             self._check_spec(delta, n, 1)
         self._check_spec(work[:nshell], n, 1, nshell)
-        if (out.dtype != torch.complex64 or out.dim() != 5 or tuple(out.shape[:2]) != (n, n // 2)
-                or tuple(out.shape[3:]) != (R + 1, 2) or not out.is_contiguous() or not out.is_cuda):
-            raise ValueError(f"half shells must be a contiguous ({n}, {n // 2}, nslots, {R + 1}, 2) complex64 tensor")
+        nslots = self._half_slots(out, n, R)
         rows = np.zeros(int(nshell), dtype=np.int32)
         check(self.lib.cnpc_shell_synth_half(self.h, n, float(boxsize), _ptr(delta),
                                              edges.ctypes.data_as(ctypes.c_void_p), len(edges) - 1, int(bin0),
                                              int(nshell), ctypes.byref(poly), _ptr(work), _ptr(out),
-                                             out.shape[2], int(slot0), rows.ctypes.data_as(ctypes.c_void_p),
+                                             nslots, int(slot0), rows.ctypes.data_as(ctypes.c_void_p),
                                              int(bool(reuse_cube))),
               "cnpc_shell_synth_half")
         return rows
@@ -265,13 +284,9 @@ class GpuContext:
             raise ValueError("contract_zgemm: with_bb needs a B different from A")
         if G.dtype != torch.float32 or G.numel() != n ** 3 or not G.is_contiguous():
             raise ValueError("G must be a contiguous FP32 n^3 mesh")
-        for H in (HA,) if HB is None else (HA, HB):
-            if H.dtype != torch.complex64 or H.dim() != 5 or tuple(H.shape[:2]) != (n, n // 2):
-                raise ValueError("half shells have the wrong layout")
-        if HB is not None and HB.shape[3] != HA.shape[3]:
-            raise ValueError("HA and HB come from different k_max")
-        if self.synth_radius(boxsize, kmax) + 1 != HA.shape[3]:
-            raise ValueError("kmax does not match the half shells")
+        R = self.synth_radius(boxsize, kmax)
+        sa = self._half_slots(HA, n, R)
+        sb = sa if HB is None else self._half_slots(HB, n, R)
         nb_ = na if HB is None else nb
 
         def rows(kz, count):
@@ -293,9 +308,8 @@ class GpuContext:
                                      or not Cbb.is_contiguous()))):
                 raise ValueError("out must be contiguous float64 (na, nb) [and (nb, nb)] tensors")
         fn = self.lib.cnpc_contract_zgemm_noalloc if noalloc else self.lib.cnpc_contract_zgemm
-        check(fn(self.h, n, float(boxsize), float(kmax), _ptr(G), _ptr(HA), int(na), HA.shape[2],
-                 None if ka is None else ka.ctypes.data_as(ctypes.c_void_p), _ptr(HB), int(nb_),
-                 HA.shape[2] if HB is None else HB.shape[2],
+        check(fn(self.h, n, float(boxsize), float(kmax), _ptr(G), _ptr(HA), int(na), sa,
+                 None if ka is None else ka.ctypes.data_as(ctypes.c_void_p), _ptr(HB), int(nb_), sb,
                  None if kb is None else kb.ctypes.data_as(ctypes.c_void_p), int(nblk), _ptr(C), _ptr(Cbb)),
               "cnpc_contract_zgemm_noalloc" if noalloc else "cnpc_contract_zgemm")
         return (C, Cbb) if with_bb else C

@@ -163,8 +163,11 @@ int cnpc_shell_synth(cnpc_ctx* ctx, int nmesh, double boxsize, int prec, const v
 
 /* The x and y passes of cnpc_shell_synth (FP32 only): shells bin0 ..
  * bin0+nshell-1 into slots slot0 .. slot0+nshell-1 of the half-synthesised
- * layout out[x][y/2][nslots][R+1][y%2] (complex64, n*(n/2)*nslots*(R+1)*2
- * entries), the input of cnpc_contract_zgemm.  A shell's modes have
+ * layout (complex64), the input of cnpc_contract_zgemm, with R = ceil(
+ * edges[nbins] / (2 pi / boxsize)) and Q = R+1: for nmesh == 400 and 4Q <=
+ * nmesh the blocked out[x][nslots][ceil(Q/8)][y/2][kz%8][y%2]
+ * (n*nslots*ceil(Q/8)*n*8 entries, kz block kz/8 on the third axis), else
+ * out[x][y/2][nslots][Q][y%2] (n*(n/2)*nslots*Q*2 entries).  A shell's modes have
  * kz < kz_rows[s] (its own radius): only those rows of its slot are written.
  * kz_rows: host array of nshell ints, or NULL.  work: nshell half spectra.
  * reuse_cube != 0 skips rebuilding the weighted, binned mode cube: the caller
