@@ -8,7 +8,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
-SOURCES = ["cnpc_api.hip", "paint.hip", "spectral.hip", "contract.hip", "synth.hip", "survey.hip", "zgemm.hip"]
+SOURCES = ["cnpc_api.hip", "paint.hip", "spectral.hip", "contract.hip", "synth.hip", "survey.hip", "zgemm.hip", "fft400.hip"]
 LIB = os.path.join(HERE, "libcnpc_hip.so")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 

@@ -67,6 +67,7 @@ struct cnpc_ctx {
   size_t synth_bytes = 0;
   void* pwork = nullptr;    // tiled paint: keys, tile counts, sorted particles
   size_t pwork_bytes = 0;
+  void* tw400 = nullptr;    // line twiddles of the fixed 400^3 c2r (made once)
   // recorded graphs hold the addresses of the work areas and plans: while
   // holds > 0 a work area that grows keeps its old allocation here (freed
   // when the last hold is released) and the plan cache is not cleared
@@ -249,6 +250,7 @@ int cnpc_ctx_destroy(cnpc_ctx* ctx) {
   if (ctx->scratch) (void)hipFree(ctx->scratch);
   if (ctx->pwork) (void)hipFree(ctx->pwork);
   if (ctx->synth) (void)hipFree(ctx->synth);
+  if (ctx->tw400) (void)hipFree(ctx->tw400);
   delete ctx;
   return rc;
 }
@@ -352,6 +354,16 @@ static int run_fft(cnpc_ctx* ctx, int n, int batch, int prec, int dir, void* in,
   REQUIRE(n >= 2 && n % 2 == 0 && batch >= 1, "fft: bad size");
   REQUIRE(prec == 0 || prec == 1, "fft: prec must be 0 or 1");
   REQUIRE(in && out && in != out, "fft: null or aliased buffers (out-of-place only)");
+  if (dir == 1 && dims == 3 && n == 400 && prec == 1) {
+    // FP32 400^3 c2r: three passes of the fixed 20 x 20 line plan
+    if (!ctx->tw400) {
+      if (int rc = check_not_capturing(ctx, "the first 400^3 c2r (twiddle table)")) return rc;
+      CHECK_HIP(hipMalloc(&ctx->tw400, 400 * sizeof(float) * 2));
+      CHECK_HIP(launch_twiddles(400, ctx->tw400, ctx->stream));
+    }
+    CHECK_HIP(launch_c2r400(batch, in, out, ctx->tw400, ctx->stream));
+    return 0;
+  }
   FftPlan* p = nullptr;
   if (int rc = get_plan(ctx, n, batch, prec, dir, &p, dims)) return rc;
   rocfft_status s = rocfft_execution_info_set_stream(p->info, ctx->stream);

@@ -314,6 +314,7 @@ hipError_t launch_synth_half(const SynthArgs& a, const BinEdges& E, const PolySp
                              signed char* bins, void* tw, void* work, void* out, int nslots, int slot0,
                              int reuse_cube, hipStream_t st);
 hipError_t launch_twiddles(int n, void* tw, hipStream_t st);
+hipError_t launch_c2r400(int batch, void* in, void* out, const void* tw, hipStream_t st);
 size_t zgemm_lds_bytes(int n, int nlines);
 int zgemm_supported(const SynthArgs& a, int na, int nb, int same, size_t lds_limit);
 int zgemm_blocks_per_cu(int same);

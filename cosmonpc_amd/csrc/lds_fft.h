@@ -428,6 +428,15 @@ __device__ __forceinline__ void fixed_twiddles(C* __restrict__ T1, const C* __re
   }
 }
 
+// Streaming 16-byte store (nontemporal): for outputs written once for a
+// later kernel; scattered pieces written this way reach HBM faster than
+// through the cache hierarchy.
+__device__ __forceinline__ void stream_store(float4* p, float a, float b, float c, float d) {
+  typedef float f32x4s __attribute__((ext_vector_type(4)));
+  const f32x4s v = {a, b, c, d};
+  __builtin_nontemporal_store(v, (f32x4s*)p);
+}
+
 // Workgroup barrier that orders LDS only: waits for this wave's LDS
 // operations, not for its global loads, so loads issued earlier stay in
 // flight across it (a __syncthreads() fence would wait for them too).

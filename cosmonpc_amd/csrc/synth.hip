@@ -281,14 +281,6 @@ __global__ void __launch_bounds__(SYNTH_THREADS) zfft_kernel(SynthArgs a, int NL
   }
 }
 
-// Streaming 16-byte store (nontemporal): the x and y passes write their
-// outputs once for a later kernel, and scattered pieces written this way
-// reach HBM faster than through the cache hierarchy.
-__device__ __forceinline__ void stream_store(float4* p, float a, float b, float c, float d) {
-  typedef float f32x4s __attribute__((ext_vector_type(4)));
-  const f32x4s v = {a, b, c, d};
-  __builtin_nontemporal_store(v, (f32x4s*)p);
-}
 
 // ------------------------------------------------------------ fixed plans
 // The x and y passes of the half synthesis for grids n = N1 * N2 with the
@@ -376,7 +368,8 @@ __global__ void __launch_bounds__(THREADS) xfft_fixed_kernel(SynthArgs a, const 
   }
 }
 
-// y pass into the half-synthesised layout H[x][y/2][slot0 + s][kz][y%2]
+// y pass into the half-synthesised layout (tile-major or blocked, as
+// half_blocked() chooses)
 template <int N1, int N2, int THREADS, int NL, int NLX>
 __global__ void __launch_bounds__(THREADS) yfft_fixed_kernel(SynthArgs a, const float2* __restrict__ U,
                                                              const float2* __restrict__ twg, float2* __restrict__ H,
@@ -539,7 +532,7 @@ hipError_t launch_twiddles(int n, void* tw, hipStream_t st) {
 }
 
 // x and y passes only (FP32), into slots slot0 .. slot0 + nshell - 1 of the
-// half-synthesised layout H[x][y/2][nslots][R+1][2]
+// half-synthesised layout
 hipError_t launch_synth_half(const SynthArgs& a, const BinEdges& E, const PolySpec& Wp, const void* D, void* V,
                              signed char* bins, void* tw, void* work, void* out, int nslots, int slot0,
                              int reuse_cube, hipStream_t st) {

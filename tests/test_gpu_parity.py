@@ -160,6 +160,50 @@ def test_fft_roundtrip_and_normalisation(gpu_ctx):
     np.testing.assert_allclose(Y.cpu().numpy() / n ** 3, x, atol=1e-12)
 
 
+def test_c2r_400_fixed_plan(gpu_ctx):
+    """The FP32 c2r of a 400^3 grid (the contraction grid of the 512^3
+    workload) runs on the fixed 20 x 20 line plan in three passes: against
+    numpy's FP64 irfftn (times n^3) of an arbitrary half spectrum, whose
+    kz = 0 and n/2 imaginary parts both drop; and a round trip of a real
+    field through the FP64 r2c.  A batch of two equals two single calls bit
+    for bit.  An index or sign mistake gives O(1) errors; FP32 rounding of
+    the two 20-point stages per axis stays near 1e-7 of max|out|."""
+    import torch
+
+    n = 400
+    ctx = gpu_ctx
+    rng = np.random.default_rng(3)
+    s = (rng.standard_normal((n, n, n // 2 + 1)) + 1j * rng.standard_normal((n, n, n // 2 + 1))).astype(np.complex64)
+    ref = np.fft.irfftn(s.astype(np.complex128), s=(n, n, n)) * float(n) ** 3
+    S = ctx.spectrum(n, 1)
+    S.copy_(torch.from_numpy(s))
+    Y = ctx.mesh(n, 1)
+    ctx.c2r(S, Y, n, 1)
+    got = Y.cpu().numpy()
+    err = np.max(np.abs(got - ref)) / np.max(np.abs(ref))
+    _log(f"c2r 400^3 FP32 fixed plan vs numpy irfftn: rel err {err:.2e}")
+    assert err < 2e-6, err
+    del ref
+    x = rng.standard_normal((n, n, n))
+    X = torch.from_numpy(x).cuda()
+    D = ctx.spectrum(n, 0)
+    ctx.r2c(X, D, n, 0)
+    S2 = ctx.spectrum(n, 1, 2)
+    S2[0].copy_(torch.from_numpy(s))
+    S2[1].copy_(D)
+    single = ctx.spectrum(n, 1)
+    single.copy_(D)
+    Y1 = ctx.mesh(n, 1)
+    ctx.c2r(single, Y1, n, 1)
+    rt = Y1.double().cpu().numpy() / n ** 3
+    err_rt = np.max(np.abs(rt - x)) / np.max(np.abs(x))
+    _log(f"c2r 400^3 FP32 fixed plan round trip: rel err {err_rt:.2e}")
+    assert err_rt < 2e-6, err_rt
+    Y2 = ctx.mesh(n, 1, 2)
+    ctx.c2r(S2, Y2, n, 1, 2)
+    assert torch.equal(Y2[0], Y) and torch.equal(Y2[1], Y1)
+
+
 @pytest.mark.parametrize("sampler,interlaced", [("cic", True), ("tsc", False), ("pcs", True), ("ngp", False)])
 def test_pk_box_matches_oracle(gpu_ctx, sampler, interlaced):
     from cosmonpc_amd.estimators import calculate_power_spectrum_box
