@@ -26,25 +26,31 @@ namespace {
 constexpr int C4N = 400, C4H = C4N / 2 + 1;  // grid, kz per half-spectrum row
 typedef FixedPlan<20, 20> C4P;
 constexpr int C4LSC = C4P::KP + 2;  // line pitch (complex)
-constexpr int C4NL = 16;            // lines per workgroup
-constexpr int C4T = C4NL * 20;      // threads
-constexpr size_t C4LDS = sizeof(float2) * ((size_t)C4N + (size_t)C4NL * C4LSC);
+constexpr int C4NL = 16;            // row pairs per workgroup (z pass)
+constexpr int C4T = C4NL * 20;      // threads (z pass)
+#ifndef CNPC_C2R400_LINES
+#define CNPC_C2R400_LINES 16
+#endif
+constexpr int C4L = CNPC_C2R400_LINES;  // kz lines per workgroup (y and x passes)
+constexpr size_t c4_lds(int lines) { return sizeof(float2) * ((size_t)C4N + (size_t)lines * C4LSC); }
 }  // namespace
 
-// c2c pass over the lines kz0 .. kz0 + 15 of block (blockIdx.y, blockIdx.z):
-// position p of line l at base + p * pstride + l
-__global__ void __launch_bounds__(C4T) c2r400_lines_kernel(float2* __restrict__ S, int64_t pstride,
-                                                           int64_t bstride, int64_t mstride,
-                                                           const float2* __restrict__ twg) {
+// c2c pass over the lines kz0 .. kz0 + NL - 1 of block (blockIdx.y,
+// blockIdx.z): position p of line l at base + p * pstride + l
+template <int NL>
+__global__ void __launch_bounds__(NL * 20) c2r400_lines_kernel(float2* __restrict__ S, int64_t pstride,
+                                                               int64_t bstride, int64_t mstride,
+                                                               const float2* __restrict__ twg) {
+  constexpr int T = NL * 20;
   extern __shared__ unsigned char smem[];
   float2* T1 = (float2*)smem;
   float2* buf = T1 + C4N;
-  const int kz0 = blockIdx.x * C4NL, nl = min(C4NL, C4H - kz0);
+  const int kz0 = blockIdx.x * NL, nl = min(NL, C4H - kz0);
   float2* base = S + blockIdx.z * mstride + blockIdx.y * bstride + kz0;
-  // item i of thread t: position t / 16 + 20 i, line t % 16 (16 lanes read
+  // item i of thread t: position t / NL + 20 i, line t % NL (NL lanes read
   // one position's lines as one contiguous piece)
-  constexpr int IPT = C4N * C4NL / C4T;
-  const int l = threadIdx.x % C4NL, p0 = threadIdx.x / C4NL;
+  constexpr int IPT = C4N / 20;
+  const int l = threadIdx.x % NL, p0 = threadIdx.x / NL;
   float2 v[IPT];
 #pragma unroll
   for (int i = 0; i < IPT; ++i) v[i] = l < nl ? base[(p0 + 20 * i) * pstride + l] : make_float2(0.f, 0.f);
@@ -54,7 +60,7 @@ __global__ void __launch_bounds__(C4T) c2r400_lines_kernel(float2* __restrict__ 
     for (int i = 0; i < IPT; ++i) buf[l * C4LSC + C4P::ipos(p0 + 20 * i)] = v[i];
   }
   __syncthreads();
-  fixed_fft_inplace<20, 20, C4T>(buf, T1, nl, C4LSC);
+  fixed_fft_inplace<20, 20, T>(buf, T1, nl, C4LSC);
   __syncthreads();
   if (l < nl) {
 #pragma unroll
@@ -114,19 +120,40 @@ __global__ void __launch_bounds__(C4T) c2r400_rows_kernel(const float2* __restri
   }
 }
 
+// dynamic LDS beyond 64 KiB needs the kernel's attribute raised, once per
+// device
+template <int NL>
+static hipError_t c4_allow_lds() {
+  constexpr size_t lds = c4_lds(NL);
+  if (lds <= 65536) return hipSuccess;
+  static size_t allowed[CNPC_MAX_DEVICES] = {};
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= CNPC_MAX_DEVICES) return hipErrorInvalidDevice;
+  if (allowed[dev] < lds) {
+    e = hipFuncSetAttribute((const void*)c2r400_lines_kernel<NL>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)lds);
+    if (e != hipSuccess) return e;
+    allowed[dev] = lds;
+  }
+  return hipSuccess;
+}
+
 // in: batch half spectra 400 x 400 x 201 (overwritten); out: batch real
 // 400^3 meshes; tw: e^{2 pi i j / 400}, j < 400
 hipError_t launch_c2r400(int batch, void* in, void* out, const void* tw, hipStream_t st) {
+  if (hipError_t e = c4_allow_lds<C4L>()) return e;
   const int64_t plane = (int64_t)C4N * C4N * C4H;
-  const dim3 grid((C4H + C4NL - 1) / C4NL, C4N, batch);
+  const dim3 grid((C4H + C4L - 1) / C4L, C4N, batch);
   float2* S = (float2*)in;
   const float2* T = (const float2*)tw;
-  hipLaunchKernelGGL(c2r400_lines_kernel, grid, dim3(C4T), C4LDS, st, S, (int64_t)C4H, (int64_t)C4N * C4H, plane,
-                     T);
-  hipLaunchKernelGGL(c2r400_lines_kernel, grid, dim3(C4T), C4LDS, st, S, (int64_t)C4N * C4H, (int64_t)C4H, plane,
-                     T);
+  hipLaunchKernelGGL(c2r400_lines_kernel<C4L>, grid, dim3(C4L * 20), c4_lds(C4L), st, S, (int64_t)C4H,
+                     (int64_t)C4N * C4H, plane, T);
+  hipLaunchKernelGGL(c2r400_lines_kernel<C4L>, grid, dim3(C4L * 20), c4_lds(C4L), st, S, (int64_t)C4N * C4H,
+                     (int64_t)C4H, plane, T);
   static_assert((C4N * C4N / 2) % C4NL == 0, "row pairs per workgroup must divide the pairs");
-  hipLaunchKernelGGL(c2r400_rows_kernel, dim3(C4N * C4N / 2 / C4NL, 1, batch), dim3(C4T), C4LDS, st,
+  hipLaunchKernelGGL(c2r400_rows_kernel, dim3(C4N * C4N / 2 / C4NL, 1, batch), dim3(C4T), c4_lds(C4NL), st,
                      (const float2*)S, (float*)out, plane, (int64_t)C4N * C4N * C4N, T);
   return hipGetLastError();
 }

@@ -381,11 +381,12 @@ __global__ void __launch_bounds__(THREADS) yfft_fixed_kernel(SynthArgs a, const 
   // adjacent segments of H, are dispatched together
   const int64_t plane = blockIdx.y;
   const int kz0 = blockIdx.x * NL;
-  const int nl = min(NL, Q - kz0);
   const int s = (int)(plane / n), x = (int)(plane - (int64_t)s * n);
   // kz lines beyond the shell's radius are zero: not computed, not written
+  // (the contraction reads kz <= rs only)
   const int rs = a.rs[s];
   if (kz0 > rs) return;
+  const int nl = min(NL, min(Q, rs + 1) - kz0);
   extern __shared__ unsigned char smem[];
   float2* T1 = (float2*)smem;
   float2* buf = T1 + n;
@@ -412,7 +413,7 @@ __global__ void __launch_bounds__(THREADS) yfft_fixed_kernel(SynthArgs a, const 
 #pragma unroll
     for (int i = 0; i < KPT; ++i) {
       v[l][i] = make_float2(0.f, 0.f);
-      if (l < nl && kz0 + l <= rs && ok[i]) v[l][i] = src[l * lstride + off[i]];
+      if (l < nl && ok[i]) v[l][i] = src[l * lstride + off[i]];
     }
   fixed_twiddles<N1, N2>(T1, twg);
 #pragma unroll
