@@ -397,21 +397,23 @@ __device__ void stockham_lds(C* buf, const C* Wt, int n, int ls, int nl, const S
 
 // ------------------------------------------------------------ fixed plans
 // n = N1 * N2 with both factors radices of SmallDft (e.g. 400 = 20 x 20): an
-// inverse FFT as two in-place passes over a padded line of N1 rows of RW =
-// N2 + 1 complex.  Input mode k = k2 + N2 k1 sits at position k2 + RW k1.
+// inverse FFT as two in-place passes over a line of N1 rows of RW complex
+// (RW = N2 + 1 by default, N2 for unpadded rows).  Input mode k = k2 + N2 k1
+// sits at position k2 + RW k1.
 //   pass 1, butterfly (line, k2): the N1-point DFT over k1 (stride RW),
 //     times e^{2 pi i k2 z1 / n}, written back in place (row z1);
 //   pass 2, butterfly (line, z1): the N2-point DFT over the contiguous row
 //     z1, written back in place: output z = z1 + N1 z2 at position z2 + RW z1.
 // Each butterfly reads and writes only its own positions, so a pass needs no
-// barrier between its reads and writes; rows start RW (odd for even N2)
-// entries apart, which spreads pass 2 over the LDS banks.  The pad column
-// (k2 = N2) is never read.  Consumers either read the output in this order
-// (the fused contraction, with G staged alike) or gather it by opos(z).
-template <int N1, int N2>
+// barrier between its reads and writes; padded rows start RW (odd for even
+// N2) entries apart, which spreads lanes that walk the rows of one line over
+// the LDS banks.  The pad column (k2 = N2) is never read.  Consumers either
+// read the output in this order (the fused contraction, with G staged alike)
+// or gather it by opos(z).
+template <int N1, int N2, int RWP = N2 + 1>
 struct FixedPlan {
   static constexpr int n = N1 * N2;
-  static constexpr int RW = N2 + 1;
+  static constexpr int RW = RWP;
   static constexpr int KP = N1 * RW;  // positions per line
   __device__ static int ipos(int k) { return (k % N2) + RW * (k / N2); }  // input mode k
   __device__ static int opos(int z) { return (z / N1) + RW * (z % N1); }  // output z
@@ -452,10 +454,10 @@ __device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(
 // without per-mode conditions.  Thread t takes line t % nl of butterfly t / nl: lanes of a
 // wave walk the lines first, whose pitch (lsc = 6 mod 32 for the fused
 // contraction's lines) spreads them over the LDS banks.
-template <int N1, int N2, int THREADS, bool BANDZ = false, typename C>
+template <int N1, int N2, int THREADS, bool BANDZ = false, int RW = N2 + 1, typename C>
 __device__ __forceinline__ void fixed_fft_inplace(C* __restrict__ buf, const C* __restrict__ T1, int nl, int lsc,
                                                   int qz = 0) {
-  constexpr int RW = N2 + 1, n = N1 * N2;
+  constexpr int n = N1 * N2;
   const float inv_nl = 1.0f / (float)nl;
   for (int t = threadIdx.x; t < nl * N2; t += THREADS) {
     const int k2 = fdiv(t, nl, inv_nl), l = t - k2 * nl;

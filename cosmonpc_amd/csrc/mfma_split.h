@@ -88,6 +88,23 @@ __device__ __forceinline__ void split8_bf16(const float (&v)[8], bf16x8& hi, bf1
   lo = __builtin_bit_cast(bf16x8, l);
 }
 
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+typedef unsigned u32x2v __attribute__((ext_vector_type(2)));
+
+// split8_bf16 for 4 values (the operands of a 16x16x16 product)
+__device__ __forceinline__ void split4_bf16(const float (&v)[4], bf16x4& hi, bf16x4& lo) {
+  u32x2v h, l;
+#pragma unroll
+  for (int p = 0; p < 2; ++p) {
+    const unsigned hp = pack_bf16x2(v[2 * p], v[2 * p + 1]);
+    const float h0 = __uint_as_float(hp << 16), h1 = __uint_as_float(hp & 0xffff0000u);
+    h[p] = hp;
+    l[p] = pack_bf16x2(v[2 * p] - h0, v[2 * p + 1] - h1);
+  }
+  hi = __builtin_bit_cast(bf16x4, h);
+  lo = __builtin_bit_cast(bf16x4, l);
+}
+
 template <bool F16>
 __device__ __forceinline__ void split8(const float (&v)[8], bf16x8& hi, bf16x8& lo, f16x8& hh, f16x8& ll) {
   if (F16) {

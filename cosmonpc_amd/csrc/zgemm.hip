@@ -269,32 +269,38 @@ __global__ void __launch_bounds__(THREADS) zgemm_bf16x3_kernel(ZGemmArgs g) {
 
 // ---------------------------------------------------------------- fixed plan
 // Grids n = N1 * N2 with both factors composite radices (e.g. 400 = 20 x
-// 20): the line FFT runs as two in-place passes over a padded layout, and
-// the contraction sums over z in the order the FFT leaves it (G is staged in
-// the same order), so no pass sorts its output and no pass needs a barrier
-// between its reads and its writes.
+// 20): the line FFT runs as two in-place passes, and the contraction sums
+// over z in the order the FFT leaves it (G is staged in the same order), so
+// no pass sorts its output and no pass needs a barrier between its reads and
+// its writes.
 //
-// A line holds N1 rows of RW = N2 + 1 complex: input mode k = k2 + N2 k1 sits
-// at position k2 + RW k1.
+// A line holds N1 unpadded rows of RW = N2 complex (every position is an
+// output, so the MFMA phase has no pad columns to skip): input mode k = k2 +
+// N2 k1 sits at position k2 + RW k1.
 //   pass 1, butterfly (line, k2): the N1-point inverse DFT over k1 (stride
 //     RW), times e^{2 pi i k2 z1 / n}, written back in place (row z1);
 //   pass 2, butterfly (line, z1): the N2-point inverse DFT over the
 //     contiguous row z1, written back in place: z = z1 + N1 z2 at position
 //     z2 + RW z1 (real part = first shell, imaginary part = second shell).
-// The pad column (k2 = N2) is never read by the passes and G is zero there
-// and past the KP = N1 RW positions, so the MFMA phase reads its operands
-// without masks (the pad and the last chunk's overhang hold finite values).
+// G is zero past the KP = N1 RW positions, so the MFMA phase reads its
+// operands without masks (the last chunk's overhang holds finite values:
+// the line pitch's two spare positions, the next line, or the zeroed tail).
 #ifndef ZGF_THREADS_DEF
 #define ZGF_THREADS_DEF 768
 #endif
 constexpr int ZGF_THREADS = ZGF_THREADS_DEF;  // threads of the fixed-plan kernels
 
 template <int N1, int N2>
-struct ZgFixed : FixedPlan<N1, N2> {
-  typedef FixedPlan<N1, N2> B;
+struct ZgFixed : FixedPlan<N1, N2, N2> {
+  typedef FixedPlan<N1, N2, N2> B;
   static constexpr int NCH = (B::KP + 31) / 32;    // 32-wide chunks per row
+  static constexpr int NFULL = B::KP / 32;         // chunks of 32 valid positions
+  static constexpr int KT = B::KP - 32 * NFULL;    // positions of the last, partial chunk
   static constexpr int GLS = NCH * 32;             // G row stride in LDS
-  static constexpr int LSC = B::KP + 2;            // line pitch (complex)
+  // line pitch (complex): at least two spare positions, and 6 mod 32, which
+  // spreads the lanes that walk the lines of one position over the LDS banks
+  static constexpr int LSC = (B::KP + 2 - 6 + 31) / 32 * 32 + 6;
+  static constexpr int DUMP = B::KP + 1;           // spare position of a line (never a mode)
   static constexpr int TAIL = 2 * (GLS - B::KP);   // floats read past the last line
   __device__ static int pos(int k) { return B::ipos(k); }
 };
@@ -319,6 +325,52 @@ __device__ __forceinline__ void zgf_row8(const float2* p, float (&re)[8], float 
   }
 }
 
+// 4 consecutive positions of an interleaved line
+__device__ __forceinline__ void zgf_row4(const float2* p, float (&re)[4], float (&im)[4]) {
+  const float4* q = (const float4*)p;
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const float4 v = q[j];
+    re[2 * j] = v.x;
+    im[2 * j] = v.y;
+    re[2 * j + 1] = v.z;
+    im[2 * j + 1] = v.w;
+  }
+}
+
+// Two-output launch, the 16 positions z0 .. z0 + 15 past the last full
+// chunk, output block (TI, TJ) only: C += A_i G B_j and C_bb += B_i G B_j
+// for i in block TI, j in block TJ by 16x16x16 products (lane group kg
+// takes positions z0 + 4 kg .. + 3)
+template <int TI, int TJ>
+__device__ __forceinline__ void zgf_tail_block(const float2* buf, const float* Gz, int lsc, int rr, int kg,
+                                               float (&acc)[2][2][4], float (&acc2)[2][2][4]) {
+  const float4 gq = *(const float4*)(Gz + 4 * kg);
+  const float vg[4] = {gq.x, gq.y, gq.z, gq.w};
+  float re[4], im[4], b[4];
+  zgf_row4(buf + (16 * TJ + rr) * lsc + 4 * kg, re, im);
+#pragma unroll
+  for (int j = 0; j < 4; ++j) b[j] = im[j] * vg[j];
+  bf16x4 gh, gl, ah, al, bh, bl;
+  split4_bf16(b, gh, gl);
+  zgf_row4(buf + (16 * TI + rr) * lsc + 4 * kg, re, im);
+  split4_bf16(re, ah, al);
+  split4_bf16(im, bh, bl);
+  f32x4 c1 = {acc[TI][TJ][0], acc[TI][TJ][1], acc[TI][TJ][2], acc[TI][TJ][3]};
+  f32x4 c2 = {acc2[TI][TJ][0], acc2[TI][TJ][1], acc2[TI][TJ][2], acc2[TI][TJ][3]};
+  c1 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, gh, c1, 0, 0, 0);
+  c2 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bh, gh, c2, 0, 0, 0);
+  c1 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, gl, c1, 0, 0, 0);
+  c2 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bh, gl, c2, 0, 0, 0);
+  c1 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, gh, c1, 0, 0, 0);
+  c2 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bl, gh, c2, 0, 0, 0);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    acc[TI][TJ][q] = c1[q];
+    acc2[TI][TJ][q] = c2[q];
+  }
+}
+
 // The operands of one round are fetched into registers, MAXF items per
 // thread: item t = threadIdx.x + it * THREADS is line l = t / Q, mode kz =
 // t % Q.  B == A: the float4 of shells 2l and 2l + 1 (both rows of the
@@ -326,9 +378,9 @@ __device__ __forceinline__ void zgf_row8(const float2* p, float (&re)[8], float 
 // for the tile's second round).  What an item reads and
 // where it lands in LDS is the same every round, so it is computed once:
 //   lds word: float2 index of mode kz in its line (bits 0-13), of mode n - kz
-//     (bits 16-29; for kz = 0 the pad column of the line's first row, which
+//     (bits 16-29; for kz = 0 the spare position KP + 1 of the line, which
 //     the passes never read) and kz == 0 (bit 31); items past the round's
-//     lines land on the pad column of line 0 (they carry zeros);
+//     lines land on the spare position of line 0 (they carry zeros);
 //   fetch word: byte offset of the item in the tile's run of HA (and HB),
 //     or in the x plane for the blocked layout (bits 0-29), bit 30 set when
 //     the A slot holds no row kz, bit 31 when the B slot holds none: masking
@@ -348,16 +400,16 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t zg_rsrc(const void* base, int 
 template <int THREADS, bool BR, int MAXF, int N1, int N2>
 __device__ __forceinline__ void zgf_items(const ZGemmArgs& g, const int* qa, const int* qb, int Qi, int total,
                                           unsigned (&lw)[MAXF], unsigned (&fw)[MAXF]) {
-  typedef FixedPlan<N1, N2> P;
-  constexpr int n = N1 * N2, LSC = P::KP + 2;
+  typedef ZgFixed<N1, N2> P;
+  constexpr int n = N1 * N2, LSC = P::LSC;
   const int Q = g.Q;
 #pragma unroll
   for (int it = 0; it < MAXF; ++it) {
     const int t = threadIdx.x + it * THREADS;
     const int l = t / Qi, kz = t - l * Qi;
     const bool ok = t < total;
-    const unsigned l1 = ok ? (unsigned)(l * LSC + P::ipos(kz)) : (unsigned)N2;
-    const unsigned l2 = ok ? (unsigned)(l * LSC + (kz ? P::ipos(n - kz) : N2)) : (unsigned)N2;
+    const unsigned l1 = ok ? (unsigned)(l * LSC + P::ipos(kz)) : (unsigned)P::DUMP;
+    const unsigned l2 = ok ? (unsigned)(l * LSC + (kz ? P::ipos(n - kz) : P::DUMP)) : (unsigned)P::DUMP;
     lw[it] = l1 | (l2 << 16) | (ok && kz == 0 ? 0x80000000u : 0u);
     bool va, vb;
     int slot;
@@ -531,16 +583,19 @@ __global__ void __launch_bounds__(THREADS) zgemm_fixed_kernel(ZGemmArgs g) {
     // the line FFTs, in place (z = z1 + N1 z2 at position z2 + RW z1)
     // (modes Q .. n - Q are zero: pass 1 does not read their positions)
     if (Qi > Q)
-      fixed_fft_inplace<N1, N2, THREADS, true>(buf, T1, L, LSC, Q);
+      fixed_fft_inplace<N1, N2, THREADS, true, RW>(buf, T1, L, LSC, Q);
     else
-      fixed_fft_inplace<N1, N2, THREADS>(buf, T1, L, LSC, Q);
+      fixed_fft_inplace<N1, N2, THREADS, false, RW>(buf, T1, L, LSC, Q);
     lds_barrier();
     constexpr int nchunk = BR ? 2 * NCH : NCH;
     if constexpr (BB) {
-      // every wave takes chunks c = w, w + nw, ... and feeds both outputs:
-      // line i holds A_i (real part) and B_i (imaginary part); B_j G is
-      // split once for both, A_i and B_i once each
-      for (int c = w; c < nchunk; c += nw) {
+      // every wave takes chunks c = w, w + nw, ... of 32 positions and feeds
+      // both outputs: line i holds A_i (real part) and B_i (imaginary part);
+      // B_j G is split once for both, A_i and B_i once each.  The last 16
+      // positions are four 16x16x16 output blocks, one on each of the last
+      // four waves (one per SIMD), so no wave takes a second chunk
+      static_assert(F::KT == 16 && F::NFULL <= nw && nw >= 4, "the two-output plan: 32 k + 16 positions");
+      for (int c = w; c < F::NFULL; c += nw) {
         const int z0 = 32 * c + 8 * kg;
         float vg[8];
         {
@@ -578,6 +633,17 @@ __global__ void __launch_bounds__(THREADS) zgemm_fixed_kernel(ZGemmArgs g) {
               acc2[i][j][q] = c2[q];
             }
           }
+      }
+      {
+        const float* Gz = Gs + r0 * GLS + 32 * F::NFULL;
+        const float2* bz = buf + 32 * F::NFULL;
+        switch (w - (nw - 4)) {
+          case 0: zgf_tail_block<0, 0>(bz, Gz, LSC, rr, kg, acc, acc2); break;
+          case 1: zgf_tail_block<0, 1>(bz, Gz, LSC, rr, kg, acc, acc2); break;
+          case 2: zgf_tail_block<1, 0>(bz, Gz, LSC, rr, kg, acc, acc2); break;
+          case 3: zgf_tail_block<1, 1>(bz, Gz, LSC, rr, kg, acc, acc2); break;
+          default: break;
+        }
       }
     } else {
     for (int c = w; c < nchunk; c += nw) {
