@@ -11,7 +11,7 @@ GPU-only options (read from ``stat_attrs``; the reference ignores them):
   contraction : "gemm_fp64" (fp64 default) | "pairwise" | "bf16x3"
                 (fp32 default) | "fp16x3" | "fp8x6"
   shell_method: "synth" (default, pruned shell synthesis) | "filter"
-                (shell filter + rocFFT 3-D c2r)
+                (shell filter + 3-D c2r)
   zsynth      : "fused" (default: with bf16x3 the shells' z pass runs
                 inside the contraction) | "separate" (real shells in HBM)
 """
