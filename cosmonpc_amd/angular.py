@@ -236,6 +236,56 @@ def get_associated_legendre_coefficients_eff(ell, m, k1, k2, k3_eff, mode="13"):
     return res
 
 
+def _assoc_legendre_arr(ell, m, x):
+    """``assoc_legendre`` on an array, element by element the same
+    operations."""
+    am = abs(m)
+    if am > ell:
+        return np.zeros_like(x)
+    d = nppoly.polyval(x, _legendre_deriv_coefs(ell, am))
+    v = (-1) ** am * (1.0 - x * x) ** (0.5 * am) * d
+    if m < 0:
+        v = v * ((-1) ** am * math.factorial(ell - am) / math.factorial(ell + am))
+    return v
+
+
+def q_ells_pairs(k1, k2, ell_1, ell_2, L, k3):
+    """q_{l1 l2 L}(k1, k2, k3) (``math_evaluator.py:418-446``, ``:592-616``)
+    for arrays k1, k2 (broadcast together) at every k3: shape
+    broadcast(k1, k2) + (len(k3),).  Element for element the operations of
+    ``_q_ells_eff`` (zero where k3 lies outside [|k1 - k2|, k1 + k2])."""
+    k1 = np.asarray(k1, dtype=np.float64)[..., None]
+    k2 = np.asarray(k2, dtype=np.float64)[..., None]
+    k3 = np.asarray(k3, dtype=np.float64)
+    inside = np.logical_and(np.abs(k1 - k2) <= k3, k3 <= (k1 + k2))
+    with np.errstate(all="ignore"):
+        mu13 = np.where(inside, (k3 ** 2 + k1 ** 2 - k2 ** 2) / (2 * k1 * k3), 0.0)
+        mu23 = np.where(inside, (k3 ** 2 + k2 ** 2 - k1 ** 2) / (2 * k2 * k3), 0.0)
+    q = np.zeros(inside.shape)
+    lmin = min(ell_1, ell_2)
+    for xx in range(-lmin, lmin + 1):
+        tj = wigner_3j(ell_1, ell_2, L, xx, -xx, 0)
+        a13 = np.where(inside, _assoc_legendre_arr(ell_1, xx, mu13), 0.0) * (-1) ** (ell_1 + xx)
+        a23 = np.where(inside, _assoc_legendre_arr(ell_2, -xx, mu23), 0.0) * (-1) ** (ell_2 - xx)
+        q += (-1) ** xx * tj * a13 * a23
+    return q
+
+
+def k3_bin_centres(k_min, k_max, kbin):
+    """Centres of the 2 kbin bins of linspace(2 k_min, 2 k_max, 2 kbin + 1)
+    (``math_evaluator.py:448-452``)."""
+    e = np.linspace(2 * k_min, 2 * k_max, kbin * 2 + 1)[:-1]
+    return 0.5 * (e[1:] + e[:-1])
+
+
+def valid_k3_bins_pairs(k1, k2, k_min, k_max, kbin):
+    """``get_valid_k3_bins`` for arrays k1, k2 (broadcast together)."""
+    c = k3_bin_centres(k_min, k_max, kbin)
+    k1 = np.asarray(k1, dtype=np.float64)[..., None]
+    k2 = np.asarray(k2, dtype=np.float64)[..., None]
+    return np.logical_and(c >= np.abs(k1 - k2), c <= (k1 + k2))
+
+
 def get_valid_k3_bins(k1, k2, k_min, k_max, kbin):
     """``math_evaluator.py:448-452``."""
     e = np.linspace(2 * k_min, 2 * k_max, kbin * 2 + 1)[:-1]

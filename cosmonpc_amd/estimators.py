@@ -945,30 +945,21 @@ def calculate_bk_sugi_box(rfield_a, rfield_b, rfield_c, correlation_mode, stat_a
             if tracer_type == "abb":
                 return 0.0
 
-            def q_of(i, j):  # effective k (default) or bin centres (:1102-1126)
-                if all_eff:
-                    return A.get_q_ells_eff(i, j, k_eff, ell_1, ell_2, L, k3_bins, k3_eff)
-                return A.get_q_ells(i, j, k_center, k_min, k_max, k_bins, ell_1, ell_2, L, k3_bins)
+            # q at the effective k (default) or the bin centres (:1102-1126),
+            # every (i, j) of the data vector at once
+            k12, k3q = (k_eff, k3_eff) if all_eff else (k_center, A.k3_bin_centres(k_min, k_max, k_bins))
             with np.errstate(invalid="ignore", divide="ignore"):
                 weighted_k_num = k_num_k3 / k3_center
             if dvm == "diagonal":
-                s3 = np.zeros(k_bins, dtype="complex128")
-                for i in range(k_bins):
-                    q = q_of(i, i)
-                    numer = np.sum(q * total_sum / k3_center)
-                    denom = np.sum(weighted_k_num[: 2 * i + 1])
-                    if denom > 0:
-                        s3[i] = numer / denom
+                q = A.q_ells_pairs(k12[:k_bins], k12[:k_bins], ell_1, ell_2, L, k3q)
+                numer = np.sum(q * total_sum / k3_center, axis=-1)
+                denom = np.array([np.sum(weighted_k_num[: 2 * i + 1]) for i in range(k_bins)])
             else:
-                s3 = np.zeros((k_bins, k_bins), dtype="complex128")
-                for i in range(k_bins):
-                    for j in range(k_bins):
-                        q = q_of(i, j)
-                        valid = A.get_valid_k3_bins(k_center[i], k_center[j], k_min, k_max, k_bins)
-                        numer = np.sum(q * total_sum / k3_center)
-                        denom = np.sum(weighted_k_num * valid)
-                        if denom > 0:
-                            s3[i, j] = numer / denom
+                q = A.q_ells_pairs(k12[:k_bins, None], k12[None, :k_bins], ell_1, ell_2, L, k3q)
+                valid = A.valid_k3_bins_pairs(k_center[:, None], k_center[None, :], k_min, k_max, k_bins)
+                numer = np.sum(q * total_sum / k3_center, axis=-1)
+                denom = np.sum(weighted_k_num * valid, axis=-1)
+            s3 = np.where(denom > 0, numer / np.where(denom > 0, denom, 1.0), 0.0).astype("complex128")
             return s3 * (H_ells * N_ells)
 
         def s3_fft():

@@ -641,24 +641,17 @@ def calculate_bk_sugi_survey(rfield_a, rfield_b, rfield_c, correlation_mode, sta
         wkn = np.zeros(k3_bins, dtype="float64")
         ok = k_num_k3 > 0
         wkn[ok] = k_num_k3[ok] / k3_center[ok]
+        # every (i, j) of the data vector at once
         if dvm == "diagonal":
-            s3 = np.zeros(k_bins, dtype="complex128")
-            for i in range(k_bins):
-                q = A.get_q_ells_eff(i, i, k_eff, ell_1, ell_2, L, k3_bins, k3_eff)
-                numer = np.sum(q * Q / k3_center)
-                denom = np.sum(wkn[: 2 * i + 1])
-                if denom > 0:
-                    s3[i] = numer / denom
+            q = A.q_ells_pairs(k_eff[:k_bins], k_eff[:k_bins], ell_1, ell_2, L, k3_eff)
+            numer = np.sum(q * Q / k3_center, axis=-1)
+            denom = np.array([np.sum(wkn[: 2 * i + 1]) for i in range(k_bins)])
         else:
-            s3 = np.zeros((k_bins, k_bins), dtype="complex128")
-            for i in range(k_bins):
-                for j in range(k_bins):
-                    q = A.get_q_ells_eff(i, j, k_eff, ell_1, ell_2, L, k3_bins, k3_eff)
-                    valid = A.get_valid_k3_bins(k_center[i], k_center[j], k_min, k_max, k_bins)
-                    numer = np.sum(q * Q / k3_center)
-                    denom = np.sum(wkn * valid)
-                    if denom > 0:
-                        s3[i, j] = numer / denom
+            q = A.q_ells_pairs(k_eff[:k_bins, None], k_eff[None, :k_bins], ell_1, ell_2, L, k3_eff)
+            valid = A.valid_k3_bins_pairs(k_center[:, None], k_center[None, :], k_min, k_max, k_bins)
+            numer = np.sum(q * Q / k3_center, axis=-1)
+            denom = np.sum(wkn * valid, axis=-1)
+        s3 = np.where(denom > 0, numer / np.where(denom > 0, denom, 1.0), 0.0).astype("complex128")
         return s3 * (H_ells * N_ells)
 
     def S3_fft():
