@@ -454,6 +454,9 @@ __device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(
 // without per-mode conditions.  Thread t takes line t % nl of butterfly t / nl: lanes of a
 // wave walk the lines first, whose pitch (lsc = 6 mod 32 for the fused
 // contraction's lines) spreads them over the LDS banks.
+#ifndef FFT_TW_GROUP
+#define FFT_TW_GROUP 10
+#endif
 template <int N1, int N2, int THREADS, bool BANDZ = false, int RW = N2 + 1, typename C>
 __device__ __forceinline__ void fixed_fft_inplace(C* __restrict__ buf, const C* __restrict__ T1, int nl, int lsc,
                                                   int qz = 0) {
@@ -497,9 +500,22 @@ __device__ __forceinline__ void fixed_fft_inplace(C* __restrict__ buf, const C* 
       }
       SmallDft<N1>::run(u);
     }
+    // the write-back reads its twiddles in groups of TWG before the group's
+    // stores, so it waits once per group (read between the stores, each
+    // twiddle would wait for its own LDS round trip); groups bound the
+    // registers the twiddles hold
     col[0] = u[0];
+    constexpr int TWG = FFT_TW_GROUP;
 #pragma unroll
-    for (int z1 = 1; z1 < N1; ++z1) col[RW * z1] = cmul(u[z1], T1[z1 * N2 + k2]);
+    for (int z0 = 1; z0 < N1; z0 += TWG) {
+      C tw[TWG];
+#pragma unroll
+      for (int j = 0; j < TWG; ++j)
+        if (z0 + j < N1) tw[j] = T1[(z0 + j) * N2 + k2];
+#pragma unroll
+      for (int j = 0; j < TWG; ++j)
+        if (z0 + j < N1) col[RW * (z0 + j)] = cmul(u[z0 + j], tw[j]);
+    }
   }
   lds_barrier();
   for (int t = threadIdx.x; t < nl * N1; t += THREADS) {
