@@ -297,9 +297,13 @@ struct ZgFixed : FixedPlan<N1, N2, N2> {
   static constexpr int NFULL = B::KP / 32;         // chunks of 32 valid positions
   static constexpr int KT = B::KP - 32 * NFULL;    // positions of the last, partial chunk
   static constexpr int GLS = NCH * 32;             // G row stride in LDS
-  // line pitch (complex): at least two spare positions, and 6 mod 32, which
-  // spreads the lanes that walk the lines of one position over the LDS banks
-  static constexpr int LSC = (B::KP + 2 - 6 + 31) / 32 * 32 + 6;
+  // line pitch (complex): at least two spare positions, and odd (5 mod 32):
+  // the lanes of an FFT pass walk the lines of one position, and an odd
+  // pitch in 8-byte units puts them on all the LDS banks (an even pitch
+  // leaves half the banks unused and doubles the cycles of every access of
+  // both passes).  Lines are then 8-byte aligned only, so the MFMA phase
+  // reads them 8 bytes at a time
+  static constexpr int LSC = (B::KP + 2 - 5 + 31) / 32 * 32 + 5;
   static constexpr int DUMP = B::KP + 1;           // spare position of a line (never a mode)
   static constexpr int TAIL = 2 * (GLS - B::KP);   // floats read past the last line
   __device__ static int pos(int k) { return B::ipos(k); }
@@ -312,29 +316,24 @@ size_t zgemm_fixed_lds_bytes(int nlines_alloc) {
          sizeof(float2) * (size_t)nlines_alloc * F::LSC + sizeof(float) * (size_t)F::TAIL;
 }
 
-// 8 consecutive positions of an interleaved line: real and imaginary parts
+// 8 consecutive positions of an interleaved line (8-byte aligned): real and
+// imaginary parts
 __device__ __forceinline__ void zgf_row8(const float2* p, float (&re)[8], float (&im)[8]) {
-  const float4* q = (const float4*)p;
 #pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const float4 v = q[j];
-    re[2 * j] = v.x;
-    im[2 * j] = v.y;
-    re[2 * j + 1] = v.z;
-    im[2 * j + 1] = v.w;
+  for (int j = 0; j < 8; ++j) {
+    const float2 v = p[j];
+    re[j] = v.x;
+    im[j] = v.y;
   }
 }
 
-// 4 consecutive positions of an interleaved line
+// 4 consecutive positions of an interleaved line (8-byte aligned)
 __device__ __forceinline__ void zgf_row4(const float2* p, float (&re)[4], float (&im)[4]) {
-  const float4* q = (const float4*)p;
 #pragma unroll
-  for (int j = 0; j < 2; ++j) {
-    const float4 v = q[j];
-    re[2 * j] = v.x;
-    im[2 * j] = v.y;
-    re[2 * j + 1] = v.z;
-    im[2 * j + 1] = v.w;
+  for (int j = 0; j < 4; ++j) {
+    const float2 v = p[j];
+    re[j] = v.x;
+    im[j] = v.y;
   }
 }
 
