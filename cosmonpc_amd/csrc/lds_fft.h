@@ -444,90 +444,128 @@ __device__ __forceinline__ void stream_store(float4* p, float a, float b, float 
 // flight across it (a __syncthreads() fence would wait for them too).
 __device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 
-// Both passes over nl lines of pitch lsc (complex); the caller synchronises
-// before (inputs written) and after (outputs read).  qz > 0: the modes qz ..
-// n - qz are zero and not read (whatever the positions hold; for 20 x 20
-// plans with qz <= n / 4 the first pass's 20-point DFTs also skip their
-// arithmetic); qz <= 0: every mode is read.  BANDZ (20 x 20, qz <= n / 4):
-// the caller has written zeros at the modes qz .. n/4 - 1 and 3n/4 .. n - qz,
-// so the first pass reads its ten band inputs (k1 = 0 .. 4, 15 .. 19)
-// without per-mode conditions.  Thread t takes line t % nl of butterfly t / nl: lanes of a
-// wave walk the lines first, whose pitch (lsc = 6 mod 32 for the fused
-// contraction's lines) spreads them over the LDS banks.
+// First-pass butterfly (line at col - k2, column k2) of a fixed plan: the
+// N1-point DFT over the column's inputs (stride RW) times e^{2 pi i k2 z1 /
+// n}, in place.  qz > 0: the modes qz .. n - qz are zero and not read
+// (whatever the positions hold; for 20 x 20 plans with qz <= n / 4 the
+// 20-point DFT also skips their arithmetic); qz <= 0: every mode is read.
+// BANDZ (20 x 20, qz <= n / 4): the caller has written zeros at the modes qz
+// .. n/4 - 1 and 3n/4 .. n - qz, so the butterfly reads its ten band inputs
+// (k1 = 0 .. 4, 15 .. 19) without per-mode conditions.
 #ifndef FFT_TW_GROUP
 #define FFT_TW_GROUP 10
 #endif
-template <int N1, int N2, int THREADS, bool BANDZ = false, int RW = N2 + 1, typename C>
-__device__ __forceinline__ void fixed_fft_inplace(C* __restrict__ buf, const C* __restrict__ T1, int nl, int lsc,
-                                                  int qz = 0) {
+template <int N1, int N2, bool BANDZ, int RW, typename C>
+__device__ __forceinline__ void fixed_pass1(C* __restrict__ col, const C* __restrict__ T1, int k2, int qz) {
   constexpr int n = N1 * N2;
-  const float inv_nl = 1.0f / (float)nl;
-  for (int t = threadIdx.x; t < nl * N2; t += THREADS) {
-    const int k2 = fdiv(t, nl, inv_nl), l = t - k2 * nl;
-    C* col = buf + l * lsc + k2;
-    C u[N1];
-    bool pruned = false;
-    if constexpr (N1 == 20) {
-      if (qz > 0 && 4 * qz <= n) {
-        // modes k2 + N2 k1 with k1 = 5 .. 14 lie in [qz, n - qz]: zero, not read
-        pruned = true;
-#pragma unroll
-        for (int k1 = 0; k1 < N1; ++k1) {
-          const int k = k2 + N2 * k1;
-          if (k1 >= 5 && k1 <= 14) {
-            u[k1].x = 0;
-            u[k1].y = 0;
-          } else if (!BANDZ && k >= qz && k <= n - qz) {
-            u[k1].x = 0;
-            u[k1].y = 0;
-          } else {
-            u[k1] = col[RW * k1];
-          }
-        }
-        Pfa20<true>::run(u);
-      }
-    }
-    if (!pruned) {
+  C u[N1];
+  bool pruned = false;
+  if constexpr (N1 == 20) {
+    if (qz > 0 && 4 * qz <= n) {
+      // modes k2 + N2 k1 with k1 = 5 .. 14 lie in [qz, n - qz]: zero, not read
+      pruned = true;
 #pragma unroll
       for (int k1 = 0; k1 < N1; ++k1) {
         const int k = k2 + N2 * k1;
-        if (qz > 0 && k >= qz && k <= n - qz) {
+        if (k1 >= 5 && k1 <= 14) {
+          u[k1].x = 0;
+          u[k1].y = 0;
+        } else if (!BANDZ && k >= qz && k <= n - qz) {
           u[k1].x = 0;
           u[k1].y = 0;
         } else {
           u[k1] = col[RW * k1];
         }
       }
-      SmallDft<N1>::run(u);
+      Pfa20<true>::run(u);
     }
-    // the write-back reads its twiddles in groups of TWG before the group's
-    // stores, so it waits once per group (read between the stores, each
-    // twiddle would wait for its own LDS round trip); groups bound the
-    // registers the twiddles hold
-    col[0] = u[0];
-    constexpr int TWG = FFT_TW_GROUP;
+  }
+  if (!pruned) {
 #pragma unroll
-    for (int z0 = 1; z0 < N1; z0 += TWG) {
-      C tw[TWG];
-#pragma unroll
-      for (int j = 0; j < TWG; ++j)
-        if (z0 + j < N1) tw[j] = T1[(z0 + j) * N2 + k2];
-#pragma unroll
-      for (int j = 0; j < TWG; ++j)
-        if (z0 + j < N1) col[RW * (z0 + j)] = cmul(u[z0 + j], tw[j]);
+    for (int k1 = 0; k1 < N1; ++k1) {
+      const int k = k2 + N2 * k1;
+      if (qz > 0 && k >= qz && k <= n - qz) {
+        u[k1].x = 0;
+        u[k1].y = 0;
+      } else {
+        u[k1] = col[RW * k1];
+      }
     }
+    SmallDft<N1>::run(u);
+  }
+  // the write-back reads its twiddles in groups of TWG before the group's
+  // stores, so it waits once per group (read between the stores, each
+  // twiddle would wait for its own LDS round trip); groups bound the
+  // registers the twiddles hold
+  col[0] = u[0];
+  constexpr int TWG = FFT_TW_GROUP;
+#pragma unroll
+  for (int z0 = 1; z0 < N1; z0 += TWG) {
+    C tw[TWG];
+#pragma unroll
+    for (int j = 0; j < TWG; ++j)
+      if (z0 + j < N1) tw[j] = T1[(z0 + j) * N2 + k2];
+#pragma unroll
+    for (int j = 0; j < TWG; ++j)
+      if (z0 + j < N1) col[RW * (z0 + j)] = cmul(u[z0 + j], tw[j]);
+  }
+}
+
+// Second-pass butterfly: the N2-point DFT over the contiguous row z1 of a
+// line (row = line + RW z1), in place (output z = z1 + N1 z2 at position z2 +
+// RW z1)
+template <int N2, typename C>
+__device__ __forceinline__ void fixed_pass2(C* __restrict__ row) {
+  C u[N2];
+#pragma unroll
+  for (int k2 = 0; k2 < N2; ++k2) u[k2] = row[k2];
+  SmallDft<N2>::run(u);
+#pragma unroll
+  for (int z2 = 0; z2 < N2; ++z2) row[z2] = u[z2];
+}
+
+// Both passes over nl lines of pitch lsc (complex); the caller synchronises
+// before (inputs written) and after (outputs read).  Thread t takes line t %
+// nl of butterfly t / nl: lanes of a wave walk the lines first, whose pitch
+// spreads them over the LDS banks.
+template <int N1, int N2, int THREADS, bool BANDZ = false, int RW = N2 + 1, typename C>
+__device__ __forceinline__ void fixed_fft_inplace(C* __restrict__ buf, const C* __restrict__ T1, int nl, int lsc,
+                                                  int qz = 0) {
+  const float inv_nl = 1.0f / (float)nl;
+  for (int t = threadIdx.x; t < nl * N2; t += THREADS) {
+    const int k2 = fdiv(t, nl, inv_nl), l = t - k2 * nl;
+    fixed_pass1<N1, N2, BANDZ, RW>(buf + l * lsc + k2, T1, k2, qz);
   }
   lds_barrier();
   for (int t = threadIdx.x; t < nl * N1; t += THREADS) {
     const int z1 = fdiv(t, nl, inv_nl), l = t - z1 * nl;
-    C* row = buf + l * lsc + RW * z1;
-    C u[N2];
-#pragma unroll
-    for (int k2 = 0; k2 < N2; ++k2) u[k2] = row[k2];
-    SmallDft<N2>::run(u);
-#pragma unroll
-    for (int z2 = 0; z2 < N2; ++z2) row[z2] = u[z2];
+    fixed_pass2<N2>(buf + l * lsc + RW * z1);
   }
+}
+
+// The same with at most one butterfly per thread and pass, its place
+// computed once by the caller (FixedSlots): for launches that run the same
+// line count round after round.
+struct FixedSlots {
+  int c1, k2;  // first pass: column offset (line * lsc + k2) and k2, c1 < 0: none
+  int r2;      // second pass: row offset (line * lsc + RW z1), < 0: none
+};
+template <int N1, int N2, int RW>
+__device__ __forceinline__ FixedSlots fixed_slots(int nl, int lsc) {
+  FixedSlots s;
+  const int t = threadIdx.x;
+  s.k2 = t / nl;
+  s.c1 = t < nl * N2 ? (t - s.k2 * nl) * lsc + s.k2 : -1;
+  const int z1 = t / nl;
+  s.r2 = t < nl * N1 ? (t - z1 * nl) * lsc + RW * z1 : -1;
+  return s;
+}
+template <int N1, int N2, bool BANDZ, int RW, typename C>
+__device__ __forceinline__ void fixed_fft_slots(C* __restrict__ buf, const C* __restrict__ T1, const FixedSlots& s,
+                                                int qz) {
+  if (s.c1 >= 0) fixed_pass1<N1, N2, BANDZ, RW>(buf + s.c1, T1, s.k2, qz);
+  lds_barrier();
+  if (s.r2 >= 0) fixed_pass2<N2>(buf + s.r2);
 }
 
 // butterflies per thread of a pass (bounds the registers of the staged pass)

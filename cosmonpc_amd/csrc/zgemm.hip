@@ -512,6 +512,9 @@ __global__ void __launch_bounds__(THREADS) zgemm_fixed_kernel(ZGemmArgs g) {
   constexpr int GPT = ((BR ? 2 : 1) * n + THREADS - 1) / THREADS;  // G values per thread and round
   unsigned lw[MAXF], fw[MAXF];
   zgf_items<THREADS, BR, MAXF, N1, N2>(g, qa, qb, Qi, total, lw, fw);
+  // each thread's butterfly of either FFT pass, the same every round (the
+  // host admits at most THREADS / N2 lines)
+  const FixedSlots fs = fixed_slots<N1, N2, RW>(L, LSC);
   ZgfFetch<BR, MAXF> pf;
   float gv[GPT];
   // B != A: one fetch per tile holds both rows; the round of row y0 + 1
@@ -582,9 +585,9 @@ __global__ void __launch_bounds__(THREADS) zgemm_fixed_kernel(ZGemmArgs g) {
     // the line FFTs, in place (z = z1 + N1 z2 at position z2 + RW z1)
     // (modes Q .. n - Q are zero: pass 1 does not read their positions)
     if (Qi > Q)
-      fixed_fft_inplace<N1, N2, THREADS, true, RW>(buf, T1, L, LSC, Q);
+      fixed_fft_slots<N1, N2, true, RW>(buf, T1, fs, Q);
     else
-      fixed_fft_inplace<N1, N2, THREADS, false, RW>(buf, T1, L, LSC, Q);
+      fixed_fft_slots<N1, N2, false, RW>(buf, T1, fs, Q);
     lds_barrier();
     constexpr int nchunk = BR ? 2 * NCH : NCH;
     if constexpr (BB) {
@@ -822,6 +825,8 @@ hipError_t launch_zgemm(const SynthArgs& a, int Q, const void* HA, int na, int s
   const bool fixed = a.n == 400 && zgf_maxf(g.Qi, na, g.nb, same) > 0 &&
                      (int64_t)(g.sa_slots > g.sb_slots ? g.sa_slots : g.sb_slots) * slot_bytes < (1 << 30);
   if (g.blocked && !fixed) return hipErrorInvalidValue;
+  // the fixed plan's FFT passes take one butterfly per thread
+  if (fixed && g.nlines * 20 > ZGF_THREADS) return hipErrorInvalidValue;
   if (fixed) {
     const size_t lds = zgemm_fixed_lds_bytes<20, 20>(zg_lines_alloc(g.nlines, same));
     if (same)
