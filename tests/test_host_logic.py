@@ -386,3 +386,16 @@ def test_planck18_parameters_and_missing_h():
         read_survey_catalog(None, None, ["RA", "DEC", "Z"], None, NO_COMP, None, "data", rows=d)
     with pytest.raises(KeyError):
         read_survey_catalog(None, None, ["RA", "DEC", "Z"], None, NO_COMP, {"Omega0": 0.3}, "data", rows=d)
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: --gpus N > 1 exits with status 2 and says
+    so, before touching a device."""
+    import subprocess
+    import sys
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr and r.stdout == ""
