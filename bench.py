@@ -444,6 +444,15 @@ def kernel_roofline(ctx, delta, n, L, k_edge, torch, np, A):
     util = measured_utilisation(dom)
     if util:
         roof["measured_utilisation"] = dict(util, source="rocprofv3 PMC (profiles/kernels_512_pmc_utilisation.txt)")
+        # what the counters say limits the kernel: the busiest of its pipes
+        busy = {"valu": util.get("valu_busy_pct", 0.0), "lds": util.get("lds_active_pct", 0.0),
+                "mfma": util.get("mfma_busy_pct", 0.0)}
+        pipe = max(busy, key=busy.get)
+        roof["limiter"] = (f"{pipe}: {busy[pipe]:.0f} % busy (" +
+                           ", ".join(f"{k} {v:.0f} %" for k, v in busy.items() if k != pipe) +
+                           f" busy; the bound's roof at {100 * roof['frac']:.0f} %)")
+        if dom == "zgemm_bf16x3_bb":
+            roof["limiter"] += "; the in-LDS z FFT and the bf16 hi/lo splits between barriers"
     del work, shells, G, H
     return {"roofline": roof, "stages": stages}
 
